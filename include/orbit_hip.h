@@ -1,0 +1,197 @@
+/*
+ * orbit_hip.h -- C ABI of the MI355X periodic-orbit transport solve (liborbit_hip.so).
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point
+ * returns ORBIT_OK (0), ORBIT_WARN_NOT_CONVERGED (>0: finished but an iteration
+ * cap was hit, results are returned) or a negative error; the message of the
+ * last error on the calling thread is orbit_last_error().
+ *
+ * Vectors at the boundary are FP64, flat (N,) in k-major order
+ * n = k*ny*nx + j*nx + i  (reference orbit/core/indexing.py:23-25).
+ * Matrices are CSR with int32 indices (reference orbit/core/solve.py:136-137
+ * passes int32 Ap/Ai to UMFPACK).
+ *
+ * Threading: an orbit_ctx owns one HIP stream and is not thread-safe; use one
+ * context per host thread.  Opsets are owned by the context that made them.
+ */
+#ifndef ORBIT_HIP_H
+#define ORBIT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORBIT_OK 0
+#define ORBIT_WARN_NOT_CONVERGED 1
+#define ORBIT_ERR_INVALID (-1)
+#define ORBIT_ERR_HIP (-2)
+#define ORBIT_ERR_STRUCTURE (-3)
+#define ORBIT_ERR_NOMEM (-4)
+
+#define ORBIT_PREC_FP64 0   /* plain: FP64 inner solves, FP64 GMRES */
+#define ORBIT_PREC_MIXED 1  /* FP32 inner solves, BF16 column blocks, FP64 GMRES + refinement */
+
+typedef struct orbit_ctx orbit_ctx;
+typedef struct orbit_opset orbit_opset;
+typedef struct orbit_fct orbit_fct;
+
+typedef struct {
+    double tol;          /* ||s - (I-M)c0|| <= tol*||s||  (reference atol = tol*||s||, orbit.py:177-182) */
+    int maxiter;         /* GMRES restart cycles (reference maxiter, orbit.py:180) */
+    int restart;         /* Krylov dimension per cycle (reference 30) */
+    int precision;       /* ORBIT_PREC_FP64 or ORBIT_PREC_MIXED */
+    int inner_sweeps;    /* red-black column sweeps per inexact P apply; 0 = opset default */
+    int acc_sweeps;      /* sweeps per accurate FP64 P apply; 0 = opset default */
+    int use_graph;       /* replay the inner sweeps of M from a captured HIP graph */
+    int verbose;
+    int max_rounds;      /* mixed: refinement rounds cap (0 = maxiter) */
+} orbit_opts;
+
+typedef struct {
+    int gmres_iters;     /* total Arnoldi steps (reference pr_norm callback count) */
+    int rounds;          /* accurate residual evaluations after the first */
+    int converged;       /* 1 when ||r|| <= tol*||s|| */
+    int zero_source;     /* 1 when ||s|| < 1e-30 (orbit is identically zero) */
+    double residual;     /* ||(I-M)c0 - s|| / ||s|| with the accurate M */
+    double periodicity;  /* ||c8 - c0|| / (||c0|| + 1e-30)  (orbit.py:199-200) */
+    double neg_frac;     /* fraction of negative entries over the 9 states (orbit.py:202-203) */
+    double s_norm;
+} orbit_info;
+
+typedef struct {
+    int nz, ny, nx, n_bins, n_tracers, n_groups;
+    int default_inner_sweeps, default_acc_sweeps;
+    double dtau;
+    double device_bytes;  /* HBM held by the opset (blocks, inverses, stencils) */
+} orbit_opset_stats;
+
+/* Air-mass-balance statistics (reference orbit/core/operator.py:598-652). */
+typedef struct {
+    double interior_median, interior_p90, interior_p99, interior_max;
+    double interior_frac_above_0_01, interior_frac_above_0_1;
+    double top_median, top_p90;
+    double layer_p90[32];
+    double layer_mean_signed[32];
+} orbit_mb_stats;
+
+/* Grid metrics of one bin for the FCT source (reference GridData fields used
+ * by orbit/core/fct.py:256-357).  All (nz,ny,nx) arrays are k-major; dx is (ny,).
+ * With split_fluxes=0, u_plus/v_plus hold UAvg/VAvg and u_minus/v_minus are NULL. */
+typedef struct {
+    int nz, ny, nx;
+    int split_fluxes;
+    int concentration_state;  /* 1: transported variable is c/rho with rho = dP/(g Dz) */
+    const double* u_plus;
+    const double* u_minus;
+    const double* v_plus;
+    const double* v_minus;
+    const double* dx;
+    double dy;
+    const double* dP;
+    const double* Dz;
+} orbit_grid_metrics;
+
+const char* orbit_last_error(void);
+void orbit_opts_default(orbit_opts* o);
+
+int orbit_ctx_create(int device, orbit_ctx** out);
+int orbit_ctx_destroy(orbit_ctx* ctx);
+int orbit_ctx_sync(orbit_ctx* ctx);
+/* The HIP stream the context launches on (hipStream_t as void*), for callers
+ * that pass device pointers produced on another stream. */
+void* orbit_ctx_stream(orbit_ctx* ctx);
+
+/* Factor backend: replaces _factor_one_bin x 8 x species
+ * (reference orbit/core/orbit.py:79-96, orbit/core/solve.py:210-244).
+ * One CSR of L_{bin,t} per (bin, tracer), index [bin*n_tracers + t].  A is
+ * built as I + dtau*L.  The pattern must couple each cell only to cells of its
+ * own column (any levels) and to its 4 horizontal neighbours on its own level;
+ * anything else is ORBIT_ERR_STRUCTURE.  nz <= 16. */
+int orbit_opset_create_csr(orbit_ctx* ctx, int nz, int ny, int nx, int n_bins, int n_tracers,
+                           double dtau, const int32_t* const* rowptr,
+                           const int32_t* const* colidx, const double* const* vals,
+                           orbit_opset** out);
+/* Same, for the shared-transport form L_{bin,t} = T_bin + diag(D_{bin,t})
+ * (reference orbit/core/operator.py:572-585): one CSR per bin and a dense
+ * diagonal D laid out [bin][t][N]. */
+int orbit_opset_create_shared(orbit_ctx* ctx, int nz, int ny, int nx, int n_bins, int n_tracers,
+                              double dtau, const int32_t* const* rowptr,
+                              const int32_t* const* colidx, const double* const* T_vals,
+                              const double* D, orbit_opset** out);
+int orbit_opset_destroy(orbit_opset* op);
+int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st);
+/* Replace the per-tracer diagonals (D laid out [bin][t][N]) and refactor the
+ * column blocks in place: the outer chemistry loop changes only loss rates. */
+int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const double* D);
+
+/* LU-protocol shim: x = A_{bin,t}^{-1} b (trans=0) or A^{-T} b (trans=1), FP64,
+ * host buffers of N doubles.  Replaces _UmfpackLU.solve(b, trans)
+ * (reference orbit/core/solve.py:246-277). */
+int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int tracer, int trans,
+                  const double* b, double* x);
+
+/* Periodic forward solve for nt tracers (indices tracers[0..nt)):
+ * e[nt][n_bins][N] emission rates, c_warm[nt][N] or NULL, orbit_out[nt][n_bins+1][N],
+ * info[nt].  Replaces solve_orbit_one_species (reference orbit/core/orbit.py:123-210)
+ * for a batch of species sharing a month's operator set. */
+int orbit_solve_forward(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt,
+                        const double* e, const double* c_warm, const orbit_opts* opts,
+                        double* orbit_out, orbit_info* info);
+/* Periodic adjoint solve (I - M^T) lam0 = s_adj: R[nt][n_bins][N] receptors,
+ * lam_out[nt][n_bins+1][N].  Replaces solve_orbit_adjoint_one_species
+ * (reference orbit/core/orbit_adjoint.py:70-144). */
+int orbit_solve_adjoint(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt,
+                        const double* R, const double* lam_warm, const orbit_opts* opts,
+                        double* lam_out, orbit_info* info);
+/* Same two solves with device pointers on the context's stream (inputs already
+ * resident in HBM).  info is a host array. */
+int orbit_solve_forward_dev(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt,
+                            const double* e_dev, const double* c_warm_dev,
+                            const orbit_opts* opts, double* orbit_out_dev, orbit_info* info);
+int orbit_solve_adjoint_dev(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt,
+                            const double* R_dev, const double* lam_warm_dev,
+                            const orbit_opts* opts, double* lam_out_dev, orbit_info* info);
+
+/* Air-mass-balance pre-flight over the shared transport blocks of an opset
+ * made with orbit_opset_create_shared (reference orbit/core/operator.py:598-652);
+ * rho is (N,).  The caller applies the abort rule of orbit/cli.py:1604-1614. */
+int orbit_mass_balance(orbit_ctx* ctx, orbit_opset* op, const double* rho, orbit_mb_stats* st);
+
+/* Horizontal FCT anti-diffusive source d_AD = rho*(-div(C*A)) for one species
+ * and bin (reference compute_horizontal_fct_source, orbit/core/fct.py:459-492).
+ * c, ghost (nullable) and d_out are host buffers of N doubles. */
+int orbit_fct_source(orbit_ctx* ctx, const orbit_grid_metrics* g, const double* c, double dt,
+                     int limited, const double* ghost, double* d_out);
+int orbit_fct_source_dev(orbit_ctx* ctx, const orbit_grid_metrics* g, const double* c_dev,
+                         double dt, int limited, const double* ghost_dev, double* d_dev);
+
+/* Frozen-coefficient FCT tangent-linear operator L_AD at a baseline state
+ * (reference assemble_fct_linear_operator, orbit/core/fct.py:495-572), kept
+ * as per-face gains on the device; apply y = L_AD x (trans=0) or L_AD^T x. */
+int orbit_fct_linop_create(orbit_ctx* ctx, const orbit_grid_metrics* g, const double* c_base,
+                           double dt, orbit_fct** out);
+int orbit_fct_linop_apply(orbit_ctx* ctx, orbit_fct* L, int trans, const double* x, double* y);
+int orbit_fct_linop_apply_dev(orbit_ctx* ctx, orbit_fct* L, int trans, const double* x_dev,
+                              double* y_dev);
+/* Dense export of the 5-point operator as CSR arrays sized by the caller
+ * (N+1 row pointers, up to 5*N entries); returns the entry count in *nnz. */
+int orbit_fct_linop_csr(orbit_ctx* ctx, orbit_fct* L, int32_t* rowptr, int32_t* colidx,
+                        double* vals, int* nnz);
+int orbit_fct_linop_destroy(orbit_fct* L);
+
+/* Diagnostic with no counterpart in the reference: average device time (ms) of
+ * one red-black half-sweep launch of `bin` on nslots tracer slots (tracers
+ * 0..nslots-1), precision ORBIT_PREC_FP64 (FP64 blocks) or ORBIT_PREC_MIXED (FP32
+ * vectors, BF16 blocks), measured with HIP events over `reps` back-to-back
+ * launches on the context's stream.  Also returns the algorithmic bytes one
+ * launch moves (each operand read once, the written colour once). */
+int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, int precision, int nslots,
+                          int reps, double* ms_per_launch, double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORBIT_HIP_H */

@@ -1,0 +1,234 @@
+"""Device context and per-month operator sets on the MI355X.
+
+``Context`` owns one HIP stream (orbit_ctx); ``OperatorSet`` owns the device form
+of one month's 8 per-bin operators for a set of tracers (orbit_opset): shared
+column blocks and 4-point stencils, per-tracer diagonals and the factored
+(inverted) column blocks in FP64 and BF16, plus their transposes for the adjoint.
+
+Replaces the reference's factor backend and LU cache: _factor_one_bin x 8 per
+species (orbit/core/orbit.py:79-96, 330-349) and _UmfpackLU
+(orbit/core/solve.py:210-283).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import check, lib, ptr
+
+N_BINS = 8
+DTAU = 10800.0
+
+
+class Context:
+    """One device and one HIP stream; not thread-safe (one per host thread)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().orbit_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def sync(self):
+        check(lib().orbit_ctx_sync(self.handle))
+
+    @property
+    def stream(self) -> int:
+        return lib().orbit_ctx_stream(self.handle) or 0
+
+    def close(self):
+        if self.handle:
+            lib().orbit_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = {}
+
+
+def default_context(device: int = 0) -> Context:
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+def _csr32(m):
+    m = sp.csr_matrix(m)
+    m.sum_duplicates()
+    m.sort_indices()
+    return (np.ascontiguousarray(m.indptr, dtype=np.int32),
+            np.ascontiguousarray(m.indices, dtype=np.int32),
+            np.ascontiguousarray(m.data, dtype=np.float64))
+
+
+class OperatorSet:
+    """Device-resident operators of one month: n_bins x n_tracers.
+
+    Build from per-(bin, tracer) CSR matrices L (``from_csr``) or from the
+    shared-transport form T_bin + diag(D_{bin,t}) (``from_shared``).
+    """
+
+    def __init__(self, ctx: Context, handle, nz, ny, nx, n_bins, n_tracers, dtau):
+        self.ctx = ctx
+        self.handle = handle
+        self.nz, self.ny, self.nx = nz, ny, nx
+        self.n_bins = n_bins
+        self.n_tracers = n_tracers
+        self.dtau = dtau
+        self.N = nz * ny * nx
+
+    @classmethod
+    def from_csr(cls, L_per_bin: Sequence[Sequence], nz: int, ny: int, nx: int,
+                 dtau: float = DTAU, ctx: Optional[Context] = None) -> "OperatorSet":
+        """L_per_bin[bin][tracer]: (N, N) sparse L (A = I + dtau*L)."""
+        ctx = ctx or default_context()
+        nb = len(L_per_bin)
+        nt = len(L_per_bin[0])
+        keep, rp, ci, vv = [], [], [], []
+        for b in range(nb):
+            if len(L_per_bin[b]) != nt:
+                raise ValueError("every bin needs the same number of tracers")
+            for t in range(nt):
+                a = _csr32(L_per_bin[b][t])
+                keep.append(a)
+                rp.append(ptr(a[0])); ci.append(ptr(a[1])); vv.append(ptr(a[2]))
+        arr = lambda xs: (C.c_void_p * len(xs))(*xs)
+        h = C.c_void_p()
+        check(lib().orbit_opset_create_csr(ctx.handle, nz, ny, nx, nb, nt, float(dtau),
+                                           arr(rp), arr(ci), arr(vv), C.byref(h)), allow_warn=False)
+        return cls(ctx, h, nz, ny, nx, nb, nt, dtau)
+
+    @classmethod
+    def from_shared(cls, T_per_bin: Sequence, D: np.ndarray, nz: int, ny: int, nx: int,
+                    dtau: float = DTAU, ctx: Optional[Context] = None) -> "OperatorSet":
+        """T_per_bin[bin]: shared (N, N) transport; D: (n_bins, n_tracers, N) diagonal."""
+        ctx = ctx or default_context()
+        nb = len(T_per_bin)
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        if D.ndim != 3 or D.shape[0] != nb:
+            raise ValueError("D must be (n_bins, n_tracers, N)")
+        nt = D.shape[1]
+        keep = [_csr32(T) for T in T_per_bin]
+        arr = lambda xs: (C.c_void_p * len(xs))(*xs)
+        h = C.c_void_p()
+        check(lib().orbit_opset_create_shared(
+            ctx.handle, nz, ny, nx, nb, nt, float(dtau),
+            arr([ptr(a[0]) for a in keep]), arr([ptr(a[1]) for a in keep]),
+            arr([ptr(a[2]) for a in keep]), ptr(D), C.byref(h)), allow_warn=False)
+        return cls(ctx, h, nz, ny, nx, nb, nt, dtau)
+
+    @classmethod
+    def from_workload(cls, wl, ctx: Optional[Context] = None) -> "OperatorSet":
+        return cls.from_shared(wl.T, wl.D, wl.nz, wl.ny, wl.nx, wl.dtau, ctx)
+
+    def stats(self) -> dict:
+        s = _lib.OpsetStats()
+        check(lib().orbit_opset_stats_get(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def update_diag(self, D: np.ndarray):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        check(lib().orbit_opset_update_diag(self.ctx.handle, self.handle, ptr(D)))
+
+    def apply_P(self, bin: int, tracer: int, b: np.ndarray, trans: bool = False) -> np.ndarray:
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if b.shape != (self.N,):
+            raise ValueError(f"b must have shape ({self.N},)")
+        x = np.empty(self.N)
+        check(lib().orbit_apply_P(self.ctx.handle, self.handle, bin, tracer, int(trans), ptr(b), ptr(x)))
+        return x
+
+    # -- periodic solves ----------------------------------------------------
+    def _solve(self, fn, tracers, inp, warm, opts, device: bool):
+        tracers = np.ascontiguousarray(tracers, dtype=np.int32)
+        nt = len(tracers)
+        info = (_lib.Info * nt)()
+        if device:
+            import torch
+            inp = inp.contiguous()
+            out = torch.empty((nt, self.n_bins + 1, self.N), dtype=torch.float64, device=inp.device)
+            w = 0 if warm is None else warm.contiguous().data_ptr()
+            # the library runs on its own stream: order it after torch's work
+            torch.cuda.current_stream(inp.device).synchronize()
+            rc = fn(self.ctx.handle, self.handle, ptr(tracers), nt, inp.data_ptr(), w or None,
+                    C.byref(opts), out.data_ptr(), info)
+        else:
+            inp = np.ascontiguousarray(inp, dtype=np.float64)
+            if inp.shape != (nt, self.n_bins, self.N):
+                raise ValueError(f"input must be ({nt}, {self.n_bins}, {self.N})")
+            out = np.empty((nt, self.n_bins + 1, self.N))
+            wp = None
+            if warm is not None:
+                warm = np.ascontiguousarray(warm, dtype=np.float64).reshape(nt, self.N)
+                wp = ptr(warm)
+            rc = fn(self.ctx.handle, self.handle, ptr(tracers), nt, ptr(inp), wp,
+                    C.byref(opts), ptr(out), info)
+        check(rc)
+        return out, [i.as_dict() for i in info]
+
+    def solve_forward(self, e, tracers=None, c_warm=None, tol=1e-8, maxiter=200, restart=30,
+                      precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
+                      max_rounds=0, verbose=False):
+        """Periodic orbits of tracers for emissions e (nt, n_bins, N).
+
+        Returns (orbit (nt, n_bins+1, N), info list).  e may be a numpy array
+        (host) or a CUDA torch tensor (device path, no host copies)."""
+        tracers = list(range(self.n_tracers)) if tracers is None else list(tracers)
+        opts = self._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps,
+                          use_graph, max_rounds, verbose)
+        dev = hasattr(e, "is_cuda") and e.is_cuda
+        fn = lib().orbit_solve_forward_dev if dev else lib().orbit_solve_forward
+        return self._solve(fn, tracers, e, c_warm, opts, dev)
+
+    def solve_adjoint(self, R, tracers=None, lam_warm=None, tol=1e-8, maxiter=200, restart=30,
+                      precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
+                      max_rounds=0, verbose=False):
+        """Adjoint orbits for receptors R (nt, n_bins, N); returns (lam (nt, n_bins+1, N), info)."""
+        tracers = list(range(self.n_tracers)) if tracers is None else list(tracers)
+        opts = self._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps,
+                          use_graph, max_rounds, verbose)
+        dev = hasattr(R, "is_cuda") and R.is_cuda
+        fn = lib().orbit_solve_adjoint_dev if dev else lib().orbit_solve_adjoint
+        return self._solve(fn, tracers, R, lam_warm, opts, dev)
+
+    @staticmethod
+    def _opts(tol, maxiter, restart, precision, inner, acc, use_graph, max_rounds, verbose):
+        prec = {"fp64": _lib.ORBIT_PREC_FP64, "mixed": _lib.ORBIT_PREC_MIXED}[precision]
+        if use_graph is None:
+            use_graph = prec == _lib.ORBIT_PREC_MIXED
+        return _lib.default_opts(tol=float(tol), maxiter=int(maxiter), restart=int(restart),
+                                 precision=prec, inner_sweeps=int(inner), acc_sweeps=int(acc),
+                                 use_graph=int(bool(use_graph)), max_rounds=int(max_rounds),
+                                 verbose=int(bool(verbose)))
+
+    def mass_balance(self, rho: np.ndarray) -> dict:
+        """Air-mass-balance row-sum statistics (orbit/core/operator.py:598-652)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        st = _lib.MbStats()
+        check(lib().orbit_mass_balance(self.ctx.handle, self.handle, ptr(rho), C.byref(st)))
+        out = {k: getattr(st, k) for k, _ in st._fields_ if not k.startswith("layer")}
+        out["interior_frac_above_0.01"] = out.pop("interior_frac_above_0_01")
+        out["interior_frac_above_0.1"] = out.pop("interior_frac_above_0_1")
+        out["layer_p90"] = list(st.layer_p90[: self.nz])
+        out["layer_mean_signed"] = list(st.layer_mean_signed[: self.nz])
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().orbit_opset_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

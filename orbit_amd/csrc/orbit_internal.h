@@ -1,0 +1,112 @@
+// Internal declarations shared by the HIP translation units of liborbit_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "orbit_hip.h"
+
+namespace orbit {
+
+// Column blocks are padded to 16 levels: a column is 16 contiguous values in the
+// device layout idx = col*16 + k (k-major vectors at the boundary are permuted).
+constexpr int NZP = 16;
+constexpr int MAXM = 64;  // largest GMRES restart supported
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define ORBIT_HIP_CHECK(expr)                                                         \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess)                                                         \
+            return ::orbit::fail(ORBIT_ERR_HIP, std::string(#expr) + ": " +          \
+                                                    hipGetErrorString(_e));           \
+    } while (0)
+
+// bf16 storage as raw 16-bit words: round-to-nearest-even on the way in.
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf/nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+    uint32_t u = ((uint32_t)h) << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;  // scratch owned by the context
+    // reusable scratch, grown on demand
+    DevBuf scratch[8];
+    int ensure(int slot, size_t bytes);
+    template <class T>
+    T* buf(int slot) { return reinterpret_cast<T*>(scratch[slot].p); }
+};
+
+// Per-bin device operator data.
+struct BinOps {
+    // shared (per group) column blocks (identity + dtau*offdiag of the column; the
+    // diagonal lives in d) and 4-point horizontal stencil (W,E,S,N per level)
+    double* V64 = nullptr;   float* V32 = nullptr;
+    double* VT64 = nullptr;  float* VT32 = nullptr;
+    double* H64 = nullptr;   float* H32 = nullptr;
+    double* HT64 = nullptr;  float* HT32 = nullptr;
+    // per tracer: dtau * diagonal of L, padded layout
+    double* d64 = nullptr;   float* d32 = nullptr;
+    // per tracer: inverse column blocks
+    double* P64 = nullptr;   uint16_t* P16 = nullptr;
+    double* PT64 = nullptr;  uint16_t* PT16 = nullptr;
+    // shared transport diagonal (dtau*T_nn) for the mass-balance check
+    double* tdiag = nullptr;
+};
+
+struct GraphKey {
+    int dir, prec, sweeps, nslots;
+    std::vector<int> tracers;
+    const void* in;
+    const void* out;
+    bool operator<(const GraphKey& o) const {
+        return std::tie(dir, prec, sweeps, nslots, tracers, in, out) <
+               std::tie(o.dir, o.prec, o.sweeps, o.nslots, o.tracers, o.in, o.out);
+    }
+};
+
+struct Opset {
+    Ctx* ctx = nullptr;
+    int nz = 0, ny = 0, nx = 0, nbins = 0, ntr = 0, ngroups = 1;
+    int ncol = 0, n16 = 0, N = 0;
+    double dtau = 0.0;
+    bool has_tdiag = false;
+    int def_inner = 6, def_acc = 14;
+    std::vector<BinOps> bins;
+    int* cols_dev[2] = {nullptr, nullptr};  // column ids of each colour
+    int ncols_c[2] = {0, 0};
+    std::vector<void*> owned;
+    double bytes = 0.0;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    template <class T>
+    T* alloc(size_t n);
+};
+
+}  // namespace orbit
+
+struct orbit_ctx : orbit::Ctx {};
+struct orbit_opset : orbit::Opset {};

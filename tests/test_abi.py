@@ -1,0 +1,56 @@
+"""CPU checks of the C-ABI library: it builds, loads, and exports every symbol
+include/orbit_hip.h declares (no compute calls; this container has no GPU)."""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(orbit_[A-Za-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_header_declares_the_binding_list():
+    from orbit_amd import _lib
+    assert _declared() == set(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    from orbit_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("liborbit_hip.so not built")
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (orbit_[A-Za-z0-9_]+)", out))
+    missing = _declared() - exported
+    assert not missing, missing
+    h = _lib.lib()
+    for name in _lib.EXPORTS:
+        assert hasattr(h, name)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from orbit_amd import _lib
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.OrbitLibraryMissing):
+        _lib.lib()
+
+
+def test_oracle_is_not_imported_by_the_product():
+    """The product package never imports the oracle."""
+    pkg = os.path.join(ROOT, "orbit_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith(".py"):
+                txt = open(os.path.join(dirpath, fn)).read()
+                assert not re.search(r"^\s*(from|import)\s+oracle\b", txt, flags=re.M), fn
