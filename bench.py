@@ -67,6 +67,10 @@ def main():
     ap.add_argument("--tracers", type=int, default=12)
     ap.add_argument("--plain-tracers", type=int, default=3,
                     help="tracers timed one at a time with the plain FP64 solve")
+    ap.add_argument("--roof-reps", type=int, default=400,
+                    help="back-to-back half-sweep launches timed for the roofline")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 256 MiB device copy first (scales profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -89,6 +93,12 @@ def main():
         f"{op.stats()['device_bytes'] / 1e9:.2f} GB")
     e_dev = torch.from_numpy(wl.e).to("cuda:0")
     T = args.tracers
+    if args.calib_copy:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     def step():
         return op.solve_forward(e_dev, tol=1e-8, precision="mixed")
@@ -133,13 +143,13 @@ def main():
     # roofline of the dominant kernel: mixed half-sweep on all slots, live HIP events
     ms = C.c_double()
     by = C.c_double()
-    _lib.check(_lib.lib().orbit_bench_halfsweep(ctx.handle, op.handle, 0, _lib.ORBIT_PREC_MIXED, T, 400,
+    _lib.check(_lib.lib().orbit_bench_halfsweep(ctx.handle, op.handle, 0, _lib.ORBIT_PREC_MIXED, T, args.roof_reps,
                                                 C.byref(ms), C.byref(by)))
     achieved = by.value / (ms.value * 1e-3) / 1e9
     log(f"halfsweep mixed: {ms.value * 1e3:.2f} us/launch, {by.value / 1e6:.2f} MB, {achieved:.0f} GB/s")
     ms64 = C.c_double()
     by64 = C.c_double()
-    _lib.check(_lib.lib().orbit_bench_halfsweep(ctx.handle, op.handle, 0, _lib.ORBIT_PREC_FP64, 1, 400,
+    _lib.check(_lib.lib().orbit_bench_halfsweep(ctx.handle, op.handle, 0, _lib.ORBIT_PREC_FP64, 1, args.roof_reps,
                                                 C.byref(ms64), C.byref(by64)))
     log(f"halfsweep fp64 1 slot: {ms64.value * 1e3:.2f} us/launch, "
         f"{by64.value / (ms64.value * 1e-3) / 1e9:.0f} GB/s")

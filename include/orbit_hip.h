@@ -181,6 +181,21 @@ int orbit_fct_linop_csr(orbit_ctx* ctx, orbit_fct* L, int32_t* rowptr, int32_t* 
                         double* vals, int* nnz);
 int orbit_fct_linop_destroy(orbit_fct* L);
 
+/* Device-resident metrics of one bin for repeated FCT sources (the forward
+ * deferred-correction passes of orbit/cli.py:1212-1241 evaluate the source for
+ * every species and bin on every outer pass). */
+typedef struct orbit_fct_grid orbit_fct_grid;
+int orbit_fct_grid_create(orbit_ctx* ctx, const orbit_grid_metrics* g, orbit_fct_grid** out);
+int orbit_fct_grid_destroy(orbit_fct_grid* G);
+/* Batched compute_horizontal_fct_source (reference orbit/core/fct.py:459-492) on
+ * nvec device vectors: vector v is c_dev + v*c_stride (ghost likewise, nullable);
+ * out + v*out_stride receives base + v*base_stride + d_AD (base nullable: d_AD).
+ * All pointers are device pointers on the context's stream. */
+int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, const double* c_dev,
+                               long long c_stride, const double* ghost_dev, double dt, int limited,
+                               const double* base_dev, long long base_stride, double* out_dev,
+                               long long out_stride);
+
 /* Diagnostic with no counterpart in the reference: average device time (ms) of
  * one red-black half-sweep launch of `bin` on nslots tracer slots (tracers
  * 0..nslots-1), precision ORBIT_PREC_FP64 (FP64 blocks) or ORBIT_PREC_MIXED (FP32

@@ -30,6 +30,7 @@ EXPORTS = [
     "orbit_solve_adjoint_dev", "orbit_mass_balance", "orbit_fct_source", "orbit_fct_source_dev",
     "orbit_fct_linop_create", "orbit_fct_linop_apply", "orbit_fct_linop_apply_dev",
     "orbit_fct_linop_csr", "orbit_fct_linop_destroy", "orbit_bench_halfsweep",
+    "orbit_fct_grid_create", "orbit_fct_grid_destroy", "orbit_fct_source_batch_dev",
 ]
 
 
@@ -116,6 +117,10 @@ def _declare(lib):
         "orbit_fct_linop_csr": (_I, [_P, _P, _P, _P, _P, C.POINTER(_I)]),
         "orbit_fct_linop_destroy": (_I, [_P]),
         "orbit_bench_halfsweep": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
+        "orbit_fct_grid_create": (_I, [_P, C.POINTER(GridMetrics), C.POINTER(_P)]),
+        "orbit_fct_grid_destroy": (_I, [_P]),
+        "orbit_fct_source_batch_dev": (_I, [_P, _P, _I, _P, C.c_longlong, _P, _D, _I, _P, C.c_longlong,
+                                            _P, C.c_longlong]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -174,16 +174,34 @@ struct SweepArgs {
     const int* active;  // nullable
     int own_zero;       // x of this colour is known to be zero (not read)
     int nbr_zero;       // x of the other colour is known to be zero (not read)
+    int nslots;
+    int tiles_per_xcd;  // column tiles (16 columns) given to each of the 8 XCDs
+    const int* counts;  // nullable: per-slot sweep count; slots past theirs skip
+    int sweep_idx;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+// block L runs on XCD L%8.  Give each XCD a contiguous range of column tiles and
+// run all slots of a tile there back to back, so the shared column blocks and
+// stencil of a tile, and the neighbouring tiles' x, are fetched into one L2.
+__device__ __forceinline__ void xcd_tile_slot(int L, int nslots, int tiles_per_xcd, int& tile, int& slot) {
+    const int xcd = L & 7, r = L >> 3;
+    const int lt = r / nslots;
+    slot = r - lt * nslots;
+    tile = xcd * tiles_per_xcd + lt;
+}
 
 template <class XT, class AT, class PT, class CT>
 __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     __shared__ CT sx[256];
     __shared__ CT sr[256];
-    const int slot = blockIdx.y;
+    int tile, slot;
+    xcd_tile_slot(blockIdx.x, a.nslots, a.tiles_per_xcd, tile, slot);
+    if (tile * 16 >= a.ncols) return;          // uniform over the block
     if (a.active && !a.active[slot]) return;  // uniform over the block
+    if (a.counts && a.sweep_idx >= a.counts[slot]) return;
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    const int idx = blockIdx.x * 16 + grp;
+    const int idx = tile * 16 + grp;
     const bool valid = idx < a.ncols;
     const int col = a.cols[valid ? idx : 0];
     const int tr = a.tmap[slot];
@@ -195,17 +213,23 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     const PT* __restrict__ P = (const PT*)a.P + (size_t)tr * a.pstride;
     const size_t base = (size_t)col * 16 + lane;
 
+    // Issue every load that does not depend on the exchange first, so their
+    // latencies overlap: P row, V row, stencil, neighbours, rhs, diagonal, own x.
+    const PT* Pr = P + (size_t)col * 256 + lane * 16;
+    PT prow[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) prow[m] = Pr[m];
+    const CT bv = (CT)b[base];
     const CT xo = a.own_zero ? CT(0) : (CT)x[base];
-    CT r = (CT)b[base];
-    sx[threadIdx.x] = xo;
-    __syncthreads();
+    AT vrow[16];
+    CT dv = 0;
     if (!a.own_zero) {
         const AT* Vr = V + (size_t)col * 256 + lane * 16;
-        CT acc = (CT)d[base] * xo;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) acc += (CT)Vr[m] * sx[(grp << 4) + m];
-        r -= acc;
+        for (int m = 0; m < 16; ++m) vrow[m] = Vr[m];
+        dv = (CT)d[base];
     }
+    CT hx = 0;
     if (!a.nbr_zero) {
         const int j = col / a.nx, i = col - j * a.nx;
         const int cW = i > 0 ? col - 1 : col;
@@ -213,15 +237,23 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
         const int cS = j > 0 ? col - a.nx : col;
         const int cN = j < a.ny - 1 ? col + a.nx : col;
         const AT* Hc = H + (size_t)col * 64 + lane;
-        r -= (CT)Hc[0] * (CT)x[(size_t)cW * 16 + lane] + (CT)Hc[16] * (CT)x[(size_t)cE * 16 + lane] +
+        hx = (CT)Hc[0] * (CT)x[(size_t)cW * 16 + lane] + (CT)Hc[16] * (CT)x[(size_t)cE * 16 + lane] +
              (CT)Hc[32] * (CT)x[(size_t)cS * 16 + lane] + (CT)Hc[48] * (CT)x[(size_t)cN * 16 + lane];
+    }
+    CT r = bv - hx;
+    if (!a.own_zero) {
+        sx[threadIdx.x] = xo;
+        __syncthreads();
+        CT acc = dv * xo;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) acc += (CT)vrow[m] * sx[(grp << 4) + m];
+        r -= acc;
     }
     sr[threadIdx.x] = r;
     __syncthreads();
-    const PT* Pr = P + (size_t)col * 256 + lane * 16;
     CT dx = 0;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) dx += (CT)ldv(Pr[m]) * sr[(grp << 4) + m];
+    for (int m = 0; m < 16; ++m) dx += (CT)ldv(prow[m]) * sr[(grp << 4) + m];
     if (valid) x[base] = (XT)(xo + dx);
 }
 
@@ -465,7 +497,7 @@ struct GState {
     double beta;
     double resid;
     double hnext;
-    int pad;
+    double ynorm;  // ||y|| = ||delta|| of the last cycle (orthonormal basis)
 };
 
 // beta = ||r||, g = beta e1; the caller then scales V_0 = r / beta.
@@ -540,7 +572,12 @@ __global__ void k_gmres_solve(GState* st, const int* kcols, int nslots, double* 
         double dgl = s.H[(size_t)i * MAXM1 + i];
         s.y[i] = dgl != 0.0 ? v / dgl : 0.0;
     }
-    for (int i = 0; i < k; ++i) coef[(size_t)slot * MAXM1 + i] = s.y[i];
+    double yn = 0.0;
+    for (int i = 0; i < k; ++i) {
+        coef[(size_t)slot * MAXM1 + i] = s.y[i];
+        yn += s.y[i] * s.y[i];
+    }
+    s.ynorm = sqrt(yn);
 }
 
 static inline dim3 grid1(size_t n, int threads = 256) { return dim3((unsigned)((n + threads - 1) / threads)); }
@@ -608,6 +645,8 @@ extern "C" void* orbit_ctx_stream(orbit_ctx* c) { return c ? (void*)c->stream : 
 // ------------------------------------------------------------------------- //
 // opset construction
 // ------------------------------------------------------------------------- //
+int calibrate_opset(orbit_ctx* ctx, orbit_opset* op);
+
 namespace {
 
 struct HostBin {
@@ -732,7 +771,7 @@ int finish_opset(orbit_ctx* ctx, orbit_opset* op, std::vector<HostBin>& hb) {
         }
     }
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
-    return ORBIT_OK;
+    return calibrate_opset(ctx, op);
 }
 
 int check_dims(int nz, int ny, int nx, int nbins, int ntr) {
@@ -903,7 +942,7 @@ extern "C" int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const do
     }
     for (auto& kv : op->graphs) hipGraphExecDestroy(kv.second);
     op->graphs.clear();
-    return ORBIT_OK;
+    return calibrate_opset(ctx, op);
 }
 
 // ------------------------------------------------------------------------- //
@@ -914,6 +953,17 @@ __global__ void k_zero(double* __restrict__ x, long long xslot, int n) {
     const int slot = blockIdx.y;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) x[(size_t)slot * xslot + g] = 0.0;
+}
+
+// out = (in ? in : 0) + (float)e   per slot (FP32 forced sweeps)
+__global__ void k_add_f32(const float* __restrict__ in, long long is, const double* __restrict__ e, long long es,
+                          float* __restrict__ out, long long os, int n, const int* active) {
+    const int slot = blockIdx.y;
+    if (active && !active[slot]) return;
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const float v = (float)e[(size_t)slot * es + g];
+    out[(size_t)slot * os + g] = in ? in[(size_t)slot * is + g] + v : v;
 }
 }  // namespace orbit
 
@@ -933,7 +983,7 @@ struct Batch {
 
 int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, const void* rhs,
                      long long bstride, void* x, long long xstride, bool own_zero, bool nbr_zero,
-                     const int* mask) {
+                     const int* mask, const int* counts = nullptr, int sweep_idx = 0) {
     orbit_opset* op = B.op;
     const BinOps& O = op->bins[bin];
     SweepArgs a;
@@ -954,8 +1004,13 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     a.active = mask;
     a.own_zero = own_zero;
     a.nbr_zero = nbr_zero;
+    a.counts = counts;
+    a.sweep_idx = sweep_idx;
     const bool T = dir == ADJ;
-    dim3 grid((a.ncols + 15) / 16, B.nslots), block(256);
+    const int ntiles = (a.ncols + 15) / 16;
+    a.nslots = B.nslots;
+    a.tiles_per_xcd = (ntiles + 7) / 8;
+    dim3 grid(8 * a.tiles_per_xcd * B.nslots), block(256);
     hipStream_t st = B.ctx->stream;
     if (prec == SP_F64) {
         a.V = T ? O.VT64 : O.V64; a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT64 : O.P64;
@@ -972,15 +1027,82 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
 
 // x = P_bin(rhs) approximately: `sweeps` red-black sweeps starting from x = 0.
 int inner_solve(const Batch& B, int bin, int dir, int prec, int sweeps, const void* rhs,
-                long long bstride, void* x, long long xstride, const int* mask) {
+                long long bstride, void* x, long long xstride, const int* mask, bool warm = false,
+                const int* counts = nullptr) {
     for (int s = 0; s < sweeps; ++s) {
-        int rc = launch_halfsweep(B, bin, dir, prec, 0, rhs, bstride, x, xstride, s == 0, s == 0, mask);
+        const bool z = s == 0 && !warm;
+        int rc = launch_halfsweep(B, bin, dir, prec, 0, rhs, bstride, x, xstride, z, z, mask, counts, s);
         if (rc) return rc;
-        rc = launch_halfsweep(B, bin, dir, prec, 1, rhs, bstride, x, xstride, s == 0, false, mask);
+        rc = launch_halfsweep(B, bin, dir, prec, 1, rhs, bstride, x, xstride, z, false, mask, counts, s);
         if (rc) return rc;
     }
     return ORBIT_OK;
 }
+
+}  // namespace
+
+// Contraction of one red-black sweep, measured on every bin of tracer 0 with a
+// fixed smooth right-hand side: rho = (||r_8|| / ||r_3||)^(1/5).  The sweep
+// counts of the solves follow from it: 1e-12 for an accurate apply (20% margin
+// on the rate), 1e-6 for the inexact FP32 one (5% margin).
+int calibrate_opset(orbit_ctx* ctx, orbit_opset* op) {
+    hipStream_t st = ctx->stream;
+    const size_t n16 = op->n16;
+    const int nb = (int)((n16 + RED_CHUNK - 1) / RED_CHUNK);
+    const int nbins = op->nbins;
+    int rc = ctx->ensure(7, (3 * n16 + (size_t)nb + 2 * (size_t)nbins + 8) * 8 + 64);
+    if (rc) return rc;
+    double* b = ctx->buf<double>(7);
+    double* x = b + n16;
+    double* y = b + 2 * n16;
+    double* red = b + 3 * n16;
+    double* out = red + nb;
+    int* tmap = (int*)(out + 2 * nbins + 4);
+    std::vector<double> hb(n16, 0.0);
+    for (int c = 0; c < op->ncol; ++c)
+        for (int k = 0; k < op->nz; ++k) hb[(size_t)c * 16 + k] = 1.0 + 0.5 * std::sin(0.37 * c + 1.3 * k);
+    const int zero = 0;
+    ORBIT_HIP_CHECK(hipMemcpyAsync(tmap, &zero, 4, hipMemcpyHostToDevice, st));
+    ORBIT_HIP_CHECK(hipMemcpyAsync(b, hb.data(), n16 * 8, hipMemcpyHostToDevice, st));
+    Batch B{ctx, op, 1, tmap, nullptr, nullptr};
+    const long long vstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 256;
+    const long long hstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 64;
+    for (int bin = 0; bin < nbins; ++bin) {
+        const BinOps& O = op->bins[bin];
+        for (int k = 1; k <= 8; ++k) {
+            rc = launch_halfsweep(B, bin, FWD, SP_F64, 0, b, (long long)n16, x, (long long)n16, k == 1, k == 1, nullptr);
+            if (!rc) rc = launch_halfsweep(B, bin, FWD, SP_F64, 1, b, (long long)n16, x, (long long)n16, k == 1, false, nullptr);
+            if (rc) return rc;
+            if (k == 3 || k == 8) {
+                hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, 1), dim3(256), 0, st, O.V64, vstride, O.H64,
+                                   hstride, O.d64, (long long)n16, x, y, (long long)n16, tmap, op->ncol, op->nx, op->ny);
+                hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, 1), dim3(256), 0, st, b, (long long)n16, y,
+                                   (long long)n16, -1.0, y, (long long)n16, (int)n16, nullptr);
+                hipLaunchKernelGGL(k_norm2, dim3(nb, 1), dim3(RED_THREADS), 0, st, y, (long long)n16, (int)n16, nb, red);
+                hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, red, nb, out + 2 * bin + (k == 8), 1);
+            }
+        }
+    }
+    ORBIT_HIP_CHECK(hipGetLastError());
+    std::vector<double> ho(2 * nbins);
+    ORBIT_HIP_CHECK(hipMemcpyAsync(ho.data(), out, 2 * nbins * 8, hipMemcpyDeviceToHost, st));
+    ORBIT_HIP_CHECK(hipStreamSynchronize(st));
+    double rmax = 0.0;
+    for (int bin = 0; bin < nbins; ++bin) {
+        const double r3 = std::sqrt(ho[2 * bin]), r8 = std::sqrt(ho[2 * bin + 1]);
+        if (!(r3 > 0.0) || !(r8 > 0.0)) continue;  // converged to round-off already
+        rmax = std::max(rmax, std::pow(r8 / r3, 0.2));
+    }
+    if (!(rmax < 1.0)) return fail(ORBIT_ERR_INVALID, "opset: red-black sweeps do not contract on this operator");
+    op->rho = std::min(0.97, std::max(0.02, 1.2 * rmax));
+    op->def_acc = std::min(80, std::max(4, (int)std::ceil(std::log(1e-12) / std::log(op->rho))));
+    // the inexact FP32 apply only has to reach the accuracy the first refinement
+    // cycle asks for (1e-6 of the residual), with a 5% margin on the rate
+    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(1e-6) / std::log(std::min(0.97, 1.05 * rmax)))));
+    return ORBIT_OK;
+}
+
+namespace {
 
 struct Solver {
     Batch B;
@@ -1000,6 +1122,10 @@ struct Solver {
     double* Vk = nullptr;     // [slot][m+1][n16]
     float* fa = nullptr;
     float* fb = nullptr;
+    float* fc = nullptr;
+    double rho = 0.1;
+    double eps_acc = 1e-12;
+    int* nsw_dev = nullptr;  // per-slot sweep counts of a warm-started forced sweep
     double* partials = nullptr;
     double* h1 = nullptr;
     double* h2 = nullptr;
@@ -1026,7 +1152,8 @@ int norms(Solver& S, const double* x, long long xslot, int n, std::vector<double
 // Forced accurate sweep.  Forward: orbit[0] = c (or 0), orbit[t+1] = P_t(orbit[t] + dtau e_t).
 // Adjoint: orbit[nb] = c (or 0), orbit[t] = P_t^T orbit[t+1] + R_t for t = nb-1..0.
 // Either way the outer residual s - (I - M)c is the last state minus the first.
-int forced_sweep(Solver& S, bool zero_start) {
+// warm: start every inner solve from the previous sweep's state (same slots).
+int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
     hipStream_t st = S.B.ctx->stream;
     const int nbn = S.nbins;
     const size_t n16 = S.n16;
@@ -1042,7 +1169,8 @@ int forced_sweep(Solver& S, bool zero_start) {
         for (int t = 0; t < nbn; ++t) {
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)t * n16, os,
                                S.E + (size_t)t * n16, es, 1.0, S.ta, vs, (int)n16, mask);
-            int rc = inner_solve(S.B, t, FWD, sp, S.acc, S.ta, vs, S.orbit + (size_t)(t + 1) * n16, os, mask);
+            int rc = inner_solve(S.B, t, FWD, sp, sweeps, S.ta, vs, S.orbit + (size_t)(t + 1) * n16, os, mask, warm,
+                                 warm ? S.nsw_dev : nullptr);
             if (rc) return rc;
         }
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, S.orbit, os,
@@ -1054,7 +1182,11 @@ int forced_sweep(Solver& S, bool zero_start) {
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, 0.0,
                                S.orbit + (size_t)nbn * n16, os, (int)n16, mask);
         for (int t = nbn - 1; t >= 0; --t) {
-            int rc = inner_solve(S.B, t, ADJ, sp, S.acc, S.orbit + (size_t)(t + 1) * n16, os, S.ta, vs, mask);
+            if (warm)  // previous P_t^T output = orbit[t] - R_t
+                hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)t * n16, os, S.E + (size_t)t * n16,
+                                   es, -1.0, S.ta, vs, (int)n16, mask);
+            int rc = inner_solve(S.B, t, ADJ, sp, sweeps, S.orbit + (size_t)(t + 1) * n16, os, S.ta, vs, mask, warm,
+                                 warm ? S.nsw_dev : nullptr);
             if (rc) return rc;
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.ta, vs, S.E + (size_t)t * n16, es, 1.0,
                                S.orbit + (size_t)t * n16, os, (int)n16, mask);
@@ -1062,6 +1194,41 @@ int forced_sweep(Solver& S, bool zero_start) {
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os, S.orbit + (size_t)nbn * n16, os,
                            -1.0, S.r, vs, (int)n16, mask);
     }
+    ORBIT_HIP_CHECK(hipGetLastError());
+    return ORBIT_OK;
+}
+
+// Inexact forced sweep from zero in FP32 (the mixed path's s~, used for ||s|| and
+// as the first right-hand side): result (FP64) in S.r.
+int forced_sweep_f32(Solver& S) {
+    hipStream_t st = S.B.ctx->stream;
+    const int nbn = S.nbins;
+    const size_t n16 = S.n16;
+    const long long es = (long long)nbn * n16, vs = (long long)n16;
+    dim3 g(grid1(n16).x, S.B.nslots);
+    const int* mask = S.B.part_dev;
+    float* x = S.fa;   // running state
+    float* y = S.fb;
+    float* rhs = S.fc;
+    if (S.dir == FWD) {
+        for (int t = 0; t < nbn; ++t) {
+            hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, t == 0 ? nullptr : x, vs, S.E + (size_t)t * n16, es,
+                               rhs, vs, (int)n16, mask);
+            int rc = inner_solve(S.B, t, FWD, SP_F32, S.inner, rhs, vs, y, vs, mask);
+            if (rc) return rc;
+            std::swap(x, y);
+        }
+    } else {
+        // x = R_{nb-1}; then x = P_t^T x + R_t for t = nb-2..0
+        hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.E + (size_t)(nbn - 1) * n16, es, x, vs,
+                           (int)n16, mask);
+        for (int t = nbn - 2; t >= 0; --t) {
+            int rc = inner_solve(S.B, t, ADJ, SP_F32, S.inner, x, vs, y, vs, mask);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, y, vs, S.E + (size_t)t * n16, es, x, vs, (int)n16, mask);
+        }
+    }
+    hipLaunchKernelGGL((k_convert_slots<float, double>), g, dim3(256), 0, st, x, vs, S.r, vs, (int)n16, mask);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
@@ -1231,8 +1398,11 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     Solver S;
     S.prec = o.precision;
     S.dir = dir;
+    S.rho = op->rho;
+    S.eps_acc = std::min(1e-12, 0.01 * o.tol);
     S.inner = o.inner_sweeps > 0 ? o.inner_sweeps : op->def_inner;
-    S.acc = o.acc_sweeps > 0 ? o.acc_sweeps : op->def_acc;
+    S.acc = o.acc_sweeps > 0 ? o.acc_sweeps
+                             : std::min(80, std::max(4, (int)std::ceil(std::log(S.eps_acc) / std::log(S.rho))));
     S.m = o.restart;
     S.use_graph = o.use_graph != 0 && S.prec == ORBIT_PREC_MIXED;
     S.n16 = op->n16;
@@ -1246,9 +1416,9 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     if (!rc) rc = ctx->ensure(1, (size_t)nt * (nbn + 1) * n16 * 8);
     if (!rc) rc = ctx->ensure(2, (size_t)nt * (S.m + 1) * n16 * 8);
     if (!rc) rc = ctx->ensure(3, (size_t)nt * 5 * n16 * 8);
-    if (!rc) rc = ctx->ensure(4, (size_t)nt * 2 * n16 * 4);
+    if (!rc) rc = ctx->ensure(4, (size_t)nt * 3 * n16 * 4);
     const size_t small = (size_t)nt * MAXM1 * S.nb * 8 * 2 + (size_t)nt * MAXM1 * 8 * 2 + (size_t)nt * 8 * 2 +
-                         (size_t)nt * 4 * 4 + 1024;
+                         (size_t)nt * 4 * 6 + 1024;
     if (!rc) rc = ctx->ensure(5, small);
     if (!rc) rc = ctx->ensure(6, (size_t)nt * sizeof(GState));
     if (rc) return rc;
@@ -1263,6 +1433,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.tb = v5 + (size_t)4 * nt * n16;
     S.fa = ctx->buf<float>(4);
     S.fb = S.fa + (size_t)nt * n16;
+    S.fc = S.fa + (size_t)2 * nt * n16;
     char* sm = (char*)ctx->scratch[5].p;
     S.partials = (double*)sm; sm += (size_t)nt * MAXM1 * S.nb * 8 * 2;
     S.h1 = (double*)sm; sm += (size_t)nt * MAXM1 * 8;
@@ -1272,7 +1443,8 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.kcols = (int*)sm; sm += (size_t)nt * 4;
     S.B.tmap_dev = (int*)sm; sm += (size_t)nt * 4;
     S.B.active_dev = (int*)sm; sm += (size_t)nt * 4;
-    S.B.part_dev = (int*)sm;
+    S.B.part_dev = (int*)sm; sm += (size_t)nt * 4;
+    S.nsw_dev = (int*)sm;
     S.gst = ctx->buf<GState>(6);
     S.B.ctx = ctx;
     S.B.op = op;
@@ -1292,54 +1464,78 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                        (long long)nbn * op->N, S.E, (long long)nbn * n16, nbn, op->nz, op->ncol, scale);
     ORBIT_HIP_CHECK(hipGetLastError());
 
-    // s (forward) / s_adj (adjoint): forced sweep from zero
-    rc = forced_sweep(S, true);
-    if (rc) return rc;
     std::vector<double> snorm;
-    const double* sptr = dir == FWD ? S.orbit + (size_t)nbn * n16 : S.orbit;
-    rc = norms(S, sptr, (long long)(nbn + 1) * n16, (int)n16, snorm);
-    if (rc) return rc;
-    // c = warm or s
-    if (warm_dev) {
-        hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
-                           S.c, vs, 1, op->nz, op->ncol, 1.0);
+    const int max_cycles = o.max_rounds > 0 ? o.max_rounds : o.maxiter;
+    bool need_check = true;
+    if (S.prec == ORBIT_PREC_MIXED) {
+        // s~ by an FP32 forced sweep: only its norm and its direction are used; the
+        // residuals that decide convergence are accurate FP64 sweeps.
+        rc = forced_sweep_f32(S);
+        if (rc) return rc;
+        rc = norms(S, S.r, vs, (int)n16, snorm);
+        if (rc) return rc;
+        if (warm_dev) {
+            hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
+                               S.c, vs, 1, op->nz, op->ncol, 1.0);
+        } else {
+            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.c, vs, (int)n16);  // r = s~ - (I-M)0
+            need_check = false;
+        }
     } else {
-        hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, sptr, (long long)(nbn + 1) * n16, sptr,
-                           (long long)(nbn + 1) * n16, 0.0, S.c, vs, (int)n16, nullptr);
+        // s (forward) / s_adj (adjoint): accurate forced sweep from zero; x0 = s
+        rc = forced_sweep(S, true, S.acc, false);
+        if (rc) return rc;
+        const double* sptr = dir == FWD ? S.orbit + (size_t)nbn * n16 : S.orbit;
+        rc = norms(S, sptr, (long long)(nbn + 1) * n16, (int)n16, snorm);
+        if (rc) return rc;
+        if (warm_dev) {
+            hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
+                               S.c, vs, 1, op->nz, op->ncol, 1.0);
+        } else {
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, sptr, (long long)(nbn + 1) * n16, sptr,
+                               (long long)(nbn + 1) * n16, 0.0, S.c, vs, (int)n16, nullptr);
+        }
     }
     std::vector<int> part(nt, 1), conv(nt, 0), zero(nt, 0), iters(nt, 0), rounds(nt, 0);
-    std::vector<double> rnorm(nt, 0.0), last_rel(nt, 0.0);
+    std::vector<double> rnorm(snorm), last_rel(nt, 1.0);
     for (int s = 0; s < nt; ++s)
         if (snorm[s] < 1e-30) {
             zero[s] = 1;
             part[s] = 0;
             conv[s] = 1;
         }
-    const int max_cycles = o.max_rounds > 0 ? o.max_rounds : o.maxiter;
     int cycles = 0;
+    bool have_orbit = false;
+    int next_sweeps = S.acc;
     while (true) {
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.part_dev, part.data(), nt * 4, hipMemcpyHostToDevice, st));
-        rc = forced_sweep(S, false);
-        if (rc) return rc;
-        rc = norms(S, S.r, vs, (int)n16, rnorm);
-        if (rc) return rc;
-        bool any = false;
-        for (int s = 0; s < nt; ++s) {
-            if (!part[s]) continue;
-            last_rel[s] = rnorm[s] / snorm[s];
-            if (!(rnorm[s] > o.tol * snorm[s])) {
-                conv[s] = 1;
-                part[s] = 0;
-            } else {
-                any = true;
+        if (need_check) {
+            const bool warm = have_orbit && S.prec == ORBIT_PREC_MIXED;
+            rc = forced_sweep(S, false, warm ? next_sweeps : S.acc, warm);
+            if (rc) return rc;
+            have_orbit = true;
+            rc = norms(S, S.r, vs, (int)n16, rnorm);
+            if (rc) return rc;
+            bool any = false;
+            for (int s = 0; s < nt; ++s) {
+                if (!part[s]) continue;
+                last_rel[s] = rnorm[s] / snorm[s];
+                if (!(rnorm[s] > o.tol * snorm[s])) {
+                    conv[s] = 1;
+                    part[s] = 0;
+                } else {
+                    any = true;
+                }
             }
+            if (o.verbose) {
+                fprintf(stderr, "[orbit] cycle %d (%d sweeps%s):", cycles, warm ? next_sweeps : S.acc,
+                        warm ? ", warm" : "");
+                for (int s = 0; s < nt; ++s) fprintf(stderr, " %.2e", last_rel[s]);
+                fprintf(stderr, "\n");
+            }
+            if (!any || cycles >= max_cycles) break;
         }
-        if (o.verbose) {
-            fprintf(stderr, "[orbit] cycle %d:", cycles);
-            for (int s = 0; s < nt; ++s) fprintf(stderr, " %.2e", last_rel[s]);
-            fprintf(stderr, "\n");
-        }
-        if (!any || cycles >= max_cycles) break;
+        need_check = true;
         std::vector<double> target(nt, 0.0);
         for (int s = 0; s < nt; ++s) {
             if (S.prec == ORBIT_PREC_MIXED)
@@ -1357,8 +1553,28 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                 rounds[s] += 1;
             }
         ++cycles;
-        // the final forced sweep must run for every slot that moved
-        for (int s = 0; s < nt; ++s) (void)s;
+        if (S.prec == ORBIT_PREC_MIXED && have_orbit) {
+            // sweeps for the warm-started residual: the previous states are off by
+            // about ||delta||, and each sweep contracts the error by rho.
+            std::vector<double> cn;
+            rc = norms(S, S.c, vs, (int)n16, cn);
+            if (rc) return rc;
+            next_sweeps = 2;
+            std::vector<int> nsw(nt, 0);
+            for (int s = 0; s < nt; ++s) {
+                if (!part[s]) continue;
+                double yn = 0.0;
+                ORBIT_HIP_CHECK(hipMemcpyAsync(&yn, &S.gst[s].ynorm, 8, hipMemcpyDeviceToHost, st));
+                ORBIT_HIP_CHECK(hipStreamSynchronize(st));
+                const double rel = cn[s] > 0.0 ? yn / cn[s] : 1.0;
+                int k = S.acc;
+                if (rel > 0.0 && rel < 1.0)
+                    k = (int)std::ceil(std::log(S.eps_acc / rel) / std::log(S.rho)) + 1;
+                nsw[s] = std::min(S.acc, std::max(2, k));
+                next_sweeps = std::max(next_sweeps, nsw[s]);
+            }
+            ORBIT_HIP_CHECK(hipMemcpyAsync(S.nsw_dev, nsw.data(), nt * 4, hipMemcpyHostToDevice, st));
+        }
     }
     // re-run the last forced sweep for all non-zero slots so every orbit matches its c
     // (slots that converged earlier kept their orbit; slots still part[] have the latest)

@@ -142,10 +142,14 @@ __device__ Face face_y(const FctGrid& g, const double* c, const double* gh, int 
 
 // R+ and R- per cell (Zalesak 1979 in the steady-state form of fct.py:381-447)
 __global__ void k_fct_R(FctGrid g, const double* __restrict__ c, const double* __restrict__ gh, double dt,
-                        double* __restrict__ Rp, double* __restrict__ Rm) {
+                        double* __restrict__ Rp, double* __restrict__ Rm, long long cs, long long rs) {
     const size_t N = (size_t)g.nz * g.ny * g.nx;
     size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    c += (size_t)blockIdx.y * cs;
+    if (gh) gh += (size_t)blockIdx.y * cs;
+    Rp += (size_t)blockIdx.y * rs;
+    Rm += (size_t)blockIdx.y * rs;
     const int nx = g.nx, ny = g.ny;
     const int i = (int)(n % nx), j = (int)((n / nx) % ny), k = (int)(n / ((size_t)nx * ny));
     const double x = xat(g, c, n);
@@ -197,10 +201,16 @@ __device__ __forceinline__ double face_C(const Face& f, double Rp_hi, double Rm_
 // d = rho * sum over faces of +-tr*C*A/dx   (C = 1 when not limited)
 __global__ void k_fct_div(FctGrid g, const double* __restrict__ c, const double* __restrict__ gh,
                           const double* __restrict__ Rp, const double* __restrict__ Rm, int limited,
-                          double* __restrict__ d) {
+                          double* __restrict__ d, long long cs, long long rs, long long ds,
+                          const double* __restrict__ base, long long bs) {
     const size_t N = (size_t)g.nz * g.ny * g.nx;
     size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
+    c += (size_t)blockIdx.y * cs;
+    if (gh) gh += (size_t)blockIdx.y * cs;
+    Rp += (size_t)blockIdx.y * rs;
+    Rm += (size_t)blockIdx.y * rs;
+    d += (size_t)blockIdx.y * ds;
     const int nx = g.nx, ny = g.ny;
     const int i = (int)(n % nx), j = (int)((n / nx) % ny), k = (int)(n / ((size_t)nx * ny));
     double acc = 0.0;
@@ -228,7 +238,8 @@ __global__ void k_fct_div(FctGrid g, const double* __restrict__ c, const double*
             if (f.valid) acc -= f.tr_lo * C * f.A * f.inv;
         }
     }
-    d[n] = acc * rho_at(g, n);
+    const double v = acc * rho_at(g, n);
+    d[n] = base ? base[(size_t)blockIdx.y * bs + n] + v : v;
 }
 
 // Frozen gains of L_AD: per x face (nz,ny,nx-1) and y face (nz,ny-1,nx), w_lo and w_hi.
@@ -423,8 +434,9 @@ int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, con
         dd = o;
     }
     if (!rc) {
-        if (limited) hipLaunchKernelGGL(k_fct_R, g1(N), dim3(256), 0, st, M.fg, dc, dg, dt, R, R + N);
-        hipLaunchKernelGGL(k_fct_div, g1(N), dim3(256), 0, st, M.fg, dc, dg, R, R + N, limited, dd);
+        if (limited) hipLaunchKernelGGL(k_fct_R, g1(N), dim3(256), 0, st, M.fg, dc, dg, dt, R, R + N, 0LL, 0LL);
+        hipLaunchKernelGGL(k_fct_div, g1(N), dim3(256), 0, st, M.fg, dc, dg, R, R + N, limited, dd, 0LL, 0LL, 0LL,
+                           (const double*)nullptr, 0LL);
         if (hipGetLastError() != hipSuccess) rc = fail(ORBIT_ERR_HIP, "fct: kernel launch failed");
     }
     if (!rc && host && hipMemcpyAsync(d, dd, N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -480,8 +492,9 @@ extern "C" int orbit_fct_linop_create(orbit_ctx* ctx, const orbit_grid_metrics* 
     if (gm->concentration_state)
         hipMemcpyAsync(L->Dz, gm->Dz, N * 8, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(cb, c_base, N * 8, hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(k_fct_R, g1(N), dim3(256), 0, st, M.fg, cb, (const double*)nullptr, dt, R, R + N);
-    hipLaunchKernelGGL(k_fct_gains, g1(nfx + nfy), dim3(256), 0, st, M.fg, cb, R, R + N, L->wx, L->wy);
+    hipLaunchKernelGGL(k_fct_R, g1(N), dim3(256), 0, st, M.fg, cb, (const double*)nullptr, dt, R, R + N, 0LL, 0LL);
+    if (nfx + nfy > 0)
+        hipLaunchKernelGGL(k_fct_gains, g1(nfx + nfy), dim3(256), 0, st, M.fg, cb, R, R + N, L->wx, L->wy);
     hipError_t e = hipGetLastError();
     hipStreamSynchronize(st);
     hipFree(cb);
@@ -692,5 +705,78 @@ extern "C" int orbit_mass_balance(orbit_ctx* ctx, orbit_opset* op, const double*
         for (double v : lays[k]) sum += v;
         st->layer_mean_signed[k] = lays[k].empty() ? std::nan("") : sum / lays[k].size();
     }
+    return ORBIT_OK;
+}
+
+
+// ------------------------------------------------------------------------- //
+// device-resident grid metrics of one bin and the batched source
+// ------------------------------------------------------------------------- //
+struct orbit_fct_grid {
+    orbit_ctx* ctx = nullptr;
+    std::vector<void*> bufs;
+    FctGrid fg{};
+    double* R = nullptr;   // 2 * N * cap scratch for R+ and R-
+    int cap = 0;
+};
+
+extern "C" int orbit_fct_grid_create(orbit_ctx* ctx, const orbit_grid_metrics* gm, orbit_fct_grid** out) {
+    int rc = check_metrics(gm);
+    if (rc) return rc;
+    if (!ctx || !out) return fail(ORBIT_ERR_INVALID, "fct_grid: NULL argument");
+    hipSetDevice(ctx->device);
+    auto* G = new orbit_fct_grid();
+    G->ctx = ctx;
+    DevMetrics M;
+    rc = M.make(ctx->stream, gm, true);
+    if (rc) {
+        delete G;
+        return rc;
+    }
+    G->fg = M.fg;
+    G->bufs.swap(M.bufs);  // ownership moves to the grid object
+    ORBIT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = G;
+    return ORBIT_OK;
+}
+
+extern "C" int orbit_fct_grid_destroy(orbit_fct_grid* G) {
+    if (!G) return ORBIT_OK;
+    if (G->ctx) {
+        hipSetDevice(G->ctx->device);
+        hipStreamSynchronize(G->ctx->stream);
+    }
+    for (void* p : G->bufs) hipFree(p);
+    if (G->R) hipFree(G->R);
+    delete G;
+    return ORBIT_OK;
+}
+
+extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, const double* c_dev,
+                                          long long c_stride, const double* ghost_dev, double dt, int limited,
+                                          const double* base_dev, long long base_stride, double* out_dev,
+                                          long long out_stride) {
+    if (!ctx || !G || !c_dev || !out_dev || nvec < 1) return fail(ORBIT_ERR_INVALID, "fct_source_batch: bad argument");
+    if (!(dt > 0.0)) return fail(ORBIT_ERR_INVALID, "fct_source_batch: dt must be positive");
+    if (nvec > 65535) return fail(ORBIT_ERR_INVALID, "fct_source_batch: nvec > 65535");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const size_t N = (size_t)G->fg.nz * G->fg.ny * G->fg.nx;
+    if (G->cap < nvec) {
+        if (G->R) {
+            hipStreamSynchronize(st);
+            hipFree(G->R);
+            G->R = nullptr;
+        }
+        ORBIT_HIP_CHECK(hipMalloc((void**)&G->R, 2 * N * (size_t)nvec * 8));
+        G->cap = nvec;
+    }
+    dim3 grid(g1(N).x, nvec);
+    if (limited)
+        hipLaunchKernelGGL(k_fct_R, grid, dim3(256), 0, st, G->fg, c_dev, ghost_dev, dt, G->R, G->R + N * nvec,
+                           c_stride, (long long)N);
+    hipLaunchKernelGGL(k_fct_div, grid, dim3(256), 0, st, G->fg, c_dev, ghost_dev, G->R, G->R + N * nvec, limited,
+                       out_dev, c_stride, (long long)N, out_stride, base_dev, base_stride);
+    ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }

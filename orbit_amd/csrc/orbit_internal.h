@@ -96,6 +96,7 @@ struct Opset {
     double dtau = 0.0;
     bool has_tdiag = false;
     int def_inner = 6, def_acc = 14;
+    double rho = 0.1;  // calibrated contraction of one red-black sweep (with margin)
     std::vector<BinOps> bins;
     int* cols_dev[2] = {nullptr, nullptr};  // column ids of each colour
     int ncols_c[2] = {0, 0};
