@@ -47,6 +47,9 @@ typedef struct {
     int use_graph;       /* replay the inner sweeps of M from a captured HIP graph */
     int verbose;
     int max_rounds;      /* mixed: refinement rounds cap (0 = maxiter) */
+    int rhs_is_s;        /* 1: the input is s itself ([nt][N]) for (I-M)c0 = s (or
+                            (I-M^T)lam0 = s_adj); states 1..n_bins are then the
+                            unforced propagation of c0 */
 } orbit_opts;
 
 typedef struct {
@@ -195,6 +198,10 @@ int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, cons
                                long long c_stride, const double* ghost_dev, double dt, int limited,
                                const double* base_dev, long long base_stride, double* out_dev,
                                long long out_stride);
+
+/* Frozen L_AD from a device-resident grid and a device baseline state. */
+int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,
+                               orbit_fct** out);
 
 /* Diagnostic with no counterpart in the reference: average device time (ms) of
  * one red-black half-sweep launch of `bin` on nslots tracer slots (tracers

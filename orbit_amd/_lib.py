@@ -31,6 +31,7 @@ EXPORTS = [
     "orbit_fct_linop_create", "orbit_fct_linop_apply", "orbit_fct_linop_apply_dev",
     "orbit_fct_linop_csr", "orbit_fct_linop_destroy", "orbit_bench_halfsweep",
     "orbit_fct_grid_create", "orbit_fct_grid_destroy", "orbit_fct_source_batch_dev",
+    "orbit_fct_linop_create_dev",
 ]
 
 
@@ -47,7 +48,8 @@ class OrbitError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("restart", C.c_int),
                 ("precision", C.c_int), ("inner_sweeps", C.c_int), ("acc_sweeps", C.c_int),
-                ("use_graph", C.c_int), ("verbose", C.c_int), ("max_rounds", C.c_int)]
+                ("use_graph", C.c_int), ("verbose", C.c_int), ("max_rounds", C.c_int),
+                ("rhs_is_s", C.c_int)]
 
 
 class Info(C.Structure):
@@ -119,6 +121,7 @@ def _declare(lib):
         "orbit_bench_halfsweep": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
         "orbit_fct_grid_create": (_I, [_P, C.POINTER(GridMetrics), C.POINTER(_P)]),
         "orbit_fct_grid_destroy": (_I, [_P]),
+        "orbit_fct_linop_create_dev": (_I, [_P, _P, _P, _D, C.POINTER(_P)]),
         "orbit_fct_source_batch_dev": (_I, [_P, _P, _I, _P, C.c_longlong, _P, _D, _I, _P, C.c_longlong,
                                             _P, C.c_longlong]),
     }

@@ -152,19 +152,24 @@ class OperatorSet:
         tracers = np.ascontiguousarray(tracers, dtype=np.int32)
         nt = len(tracers)
         info = (_lib.Info * nt)()
+        in_shape = (nt, self.N) if opts.rhs_is_s else (nt, self.n_bins, self.N)
+        if tuple(inp.shape) != in_shape:
+            raise ValueError(f"input must have shape {in_shape}, got {tuple(inp.shape)}")
         if device:
             import torch
+            if inp.dtype != torch.float64:
+                raise TypeError("device inputs must be float64 tensors")
             inp = inp.contiguous()
             out = torch.empty((nt, self.n_bins + 1, self.N), dtype=torch.float64, device=inp.device)
-            w = 0 if warm is None else warm.contiguous().data_ptr()
+            wt = None
+            if warm is not None:
+                wt = warm.to(torch.float64).reshape(nt, self.N).contiguous()  # kept alive for the call
             # the library runs on its own stream: order it after torch's work
             torch.cuda.current_stream(inp.device).synchronize()
-            rc = fn(self.ctx.handle, self.handle, ptr(tracers), nt, inp.data_ptr(), w or None,
-                    C.byref(opts), out.data_ptr(), info)
+            rc = fn(self.ctx.handle, self.handle, ptr(tracers), nt, inp.data_ptr(),
+                    None if wt is None else wt.data_ptr(), C.byref(opts), out.data_ptr(), info)
         else:
             inp = np.ascontiguousarray(inp, dtype=np.float64)
-            if inp.shape != (nt, self.n_bins, self.N):
-                raise ValueError(f"input must be ({nt}, {self.n_bins}, {self.N})")
             out = np.empty((nt, self.n_bins + 1, self.N))
             wp = None
             if warm is not None:
@@ -177,38 +182,43 @@ class OperatorSet:
 
     def solve_forward(self, e, tracers=None, c_warm=None, tol=1e-8, maxiter=200, restart=30,
                       precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
-                      max_rounds=0, verbose=False):
+                      max_rounds=0, verbose=False, rhs="e"):
         """Periodic orbits of tracers for emissions e (nt, n_bins, N).
 
+        rhs="s": e is s itself, (nt, N), for (I - M)c0 = s.
         Returns (orbit (nt, n_bins+1, N), info list).  e may be a numpy array
         (host) or a CUDA torch tensor (device path, no host copies)."""
         tracers = list(range(self.n_tracers)) if tracers is None else list(tracers)
         opts = self._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps,
-                          use_graph, max_rounds, verbose)
+                          use_graph, max_rounds, verbose, rhs)
         dev = hasattr(e, "is_cuda") and e.is_cuda
         fn = lib().orbit_solve_forward_dev if dev else lib().orbit_solve_forward
         return self._solve(fn, tracers, e, c_warm, opts, dev)
 
     def solve_adjoint(self, R, tracers=None, lam_warm=None, tol=1e-8, maxiter=200, restart=30,
                       precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
-                      max_rounds=0, verbose=False):
-        """Adjoint orbits for receptors R (nt, n_bins, N); returns (lam (nt, n_bins+1, N), info)."""
+                      max_rounds=0, verbose=False, rhs="e"):
+        """Adjoint orbits for receptors R (nt, n_bins, N); returns (lam (nt, n_bins+1, N), info).
+
+        rhs="s": R is s_adj itself, (nt, N), for (I - M^T) lam0 = s_adj."""
         tracers = list(range(self.n_tracers)) if tracers is None else list(tracers)
         opts = self._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps,
-                          use_graph, max_rounds, verbose)
+                          use_graph, max_rounds, verbose, rhs)
         dev = hasattr(R, "is_cuda") and R.is_cuda
         fn = lib().orbit_solve_adjoint_dev if dev else lib().orbit_solve_adjoint
         return self._solve(fn, tracers, R, lam_warm, opts, dev)
 
     @staticmethod
-    def _opts(tol, maxiter, restart, precision, inner, acc, use_graph, max_rounds, verbose):
+    def _opts(tol, maxiter, restart, precision, inner, acc, use_graph, max_rounds, verbose, rhs="e"):
         prec = {"fp64": _lib.ORBIT_PREC_FP64, "mixed": _lib.ORBIT_PREC_MIXED}[precision]
+        if rhs not in ("e", "s"):
+            raise ValueError("rhs must be 'e' or 's'")
         if use_graph is None:
             use_graph = prec == _lib.ORBIT_PREC_MIXED
         return _lib.default_opts(tol=float(tol), maxiter=int(maxiter), restart=int(restart),
                                  precision=prec, inner_sweeps=int(inner), acc_sweeps=int(acc),
                                  use_graph=int(bool(use_graph)), max_rounds=int(max_rounds),
-                                 verbose=int(bool(verbose)))
+                                 verbose=int(bool(verbose)), rhs_is_s=int(rhs == "s"))
 
     def mass_balance(self, rho: np.ndarray) -> dict:
         """Air-mass-balance row-sum statistics (orbit/core/operator.py:598-652)."""

@@ -602,6 +602,7 @@ extern "C" void orbit_opts_default(orbit_opts* o) {
     o->use_graph = 0;
     o->verbose = 0;
     o->max_rounds = 0;
+    o->rhs_is_s = 0;
 }
 
 extern "C" int orbit_ctx_create(int device, orbit_ctx** out) {
@@ -1126,6 +1127,7 @@ struct Solver {
     double rho = 0.1;
     double eps_acc = 1e-12;
     int* nsw_dev = nullptr;  // per-slot sweep counts of a warm-started forced sweep
+    double* sv = nullptr;    // s given directly (rhs_is_s), else null
     double* partials = nullptr;
     double* h1 = nullptr;
     double* h2 = nullptr;
@@ -1175,6 +1177,7 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
         }
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, S.orbit, os,
                            -1.0, S.r, vs, (int)n16, mask);
+        if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask);
     } else {
         if (zero_start)
             hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, (int)n16);
@@ -1193,6 +1196,7 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
         }
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os, S.orbit + (size_t)nbn * n16, os,
                            -1.0, S.r, vs, (int)n16, mask);
+        if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask);
     }
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
@@ -1415,7 +1419,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     int rc = ctx->ensure(0, (size_t)nt * nbn * n16 * 8);
     if (!rc) rc = ctx->ensure(1, (size_t)nt * (nbn + 1) * n16 * 8);
     if (!rc) rc = ctx->ensure(2, (size_t)nt * (S.m + 1) * n16 * 8);
-    if (!rc) rc = ctx->ensure(3, (size_t)nt * 5 * n16 * 8);
+    if (!rc) rc = ctx->ensure(3, (size_t)nt * 6 * n16 * 8);
     if (!rc) rc = ctx->ensure(4, (size_t)nt * 3 * n16 * 4);
     const size_t small = (size_t)nt * MAXM1 * S.nb * 8 * 2 + (size_t)nt * MAXM1 * 8 * 2 + (size_t)nt * 8 * 2 +
                          (size_t)nt * 4 * 6 + 1024;
@@ -1431,6 +1435,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.w = v5 + (size_t)2 * nt * n16;
     S.ta = v5 + (size_t)3 * nt * n16;
     S.tb = v5 + (size_t)4 * nt * n16;
+    if (o.rhs_is_s) S.sv = v5 + (size_t)5 * nt * n16;
     S.fa = ctx->buf<float>(4);
     S.fb = S.fa + (size_t)nt * n16;
     S.fc = S.fa + (size_t)2 * nt * n16;
@@ -1460,8 +1465,16 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.active_dev, ones.data(), nt * 4, hipMemcpyHostToDevice, st));
     // inputs -> device layout (forward: dtau*e; adjoint: R)
     const double scale = dir == FWD ? op->dtau : 1.0;
-    hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt * nbn), dim3(256), 0, st, in_dev,
-                       (long long)nbn * op->N, S.E, (long long)nbn * n16, nbn, op->nz, op->ncol, scale);
+    if (o.rhs_is_s) {
+        // homogeneous sweeps: no forcing; s itself enters every residual
+        hipLaunchKernelGGL(k_zero, dim3(grid1((size_t)nbn * n16).x, nt), dim3(256), 0, st, S.E, (long long)nbn * n16,
+                           (int)(nbn * n16));
+        hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, in_dev, (long long)op->N, S.sv,
+                           (long long)n16, 1, op->nz, op->ncol, 1.0);
+    } else {
+        hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt * nbn), dim3(256), 0, st, in_dev,
+                           (long long)nbn * op->N, S.E, (long long)nbn * n16, nbn, op->nz, op->ncol, scale);
+    }
     ORBIT_HIP_CHECK(hipGetLastError());
 
     std::vector<double> snorm;
@@ -1470,7 +1483,10 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     if (S.prec == ORBIT_PREC_MIXED) {
         // s~ by an FP32 forced sweep: only its norm and its direction are used; the
         // residuals that decide convergence are accurate FP64 sweeps.
-        rc = forced_sweep_f32(S);
+        if (S.sv)
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.sv, vs, S.sv, vs, 0.0, S.r, vs, (int)n16, nullptr);
+        else
+            rc = forced_sweep_f32(S);
         if (rc) return rc;
         rc = norms(S, S.r, vs, (int)n16, snorm);
         if (rc) return rc;
@@ -1483,17 +1499,25 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         }
     } else {
         // s (forward) / s_adj (adjoint): accurate forced sweep from zero; x0 = s
-        rc = forced_sweep(S, true, S.acc, false);
-        if (rc) return rc;
-        const double* sptr = dir == FWD ? S.orbit + (size_t)nbn * n16 : S.orbit;
-        rc = norms(S, sptr, (long long)(nbn + 1) * n16, (int)n16, snorm);
+        const double* sptr;
+        long long sstride;
+        if (S.sv) {
+            sptr = S.sv;
+            sstride = vs;
+        } else {
+            rc = forced_sweep(S, true, S.acc, false);
+            if (rc) return rc;
+            sptr = dir == FWD ? S.orbit + (size_t)nbn * n16 : S.orbit;
+            sstride = (long long)(nbn + 1) * n16;
+        }
+        rc = norms(S, sptr, sstride, (int)n16, snorm);
         if (rc) return rc;
         if (warm_dev) {
             hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
                                S.c, vs, 1, op->nz, op->ncol, 1.0);
         } else {
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, sptr, (long long)(nbn + 1) * n16, sptr,
-                               (long long)(nbn + 1) * n16, 0.0, S.c, vs, (int)n16, nullptr);
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, sptr, sstride, sptr, sstride, 0.0, S.c, vs, (int)n16,
+                               nullptr);
         }
     }
     std::vector<int> part(nt, 1), conv(nt, 0), zero(nt, 0), iters(nt, 0), rounds(nt, 0);
@@ -1633,7 +1657,8 @@ int host_solve(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, cons
                const double* warm, const orbit_opts* opts, double* out, orbit_info* info, int dir) {
     if (!ctx || !op || !in || !out || nt < 1) return fail(ORBIT_ERR_INVALID, "solve: NULL argument");
     hipSetDevice(ctx->device);
-    const size_t nin = (size_t)nt * op->nbins * op->N, nout = (size_t)nt * (op->nbins + 1) * op->N,
+    const size_t nin = (size_t)nt * ((opts && opts->rhs_is_s) ? 1 : op->nbins) * op->N,
+                 nout = (size_t)nt * (op->nbins + 1) * op->N,
                  nw = (size_t)nt * op->N;
     double *din = nullptr, *dout = nullptr, *dw = nullptr;
     ORBIT_HIP_CHECK(hipMalloc(&din, nin * 8));

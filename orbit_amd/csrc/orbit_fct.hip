@@ -780,3 +780,43 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
+
+
+extern "C" int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,
+                                          orbit_fct** out) {
+    if (!ctx || !G || !c_base_dev || !out) return fail(ORBIT_ERR_INVALID, "fct_linop_create_dev: NULL argument");
+    if (!(dt > 0.0)) return fail(ORBIT_ERR_INVALID, "fct_linop_create_dev: dt must be positive");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const FctGrid& fg = G->fg;
+    const int nz = fg.nz, ny = fg.ny, nx = fg.nx;
+    const size_t N = (size_t)nz * ny * nx;
+    const size_t nfx = nx > 1 ? (size_t)nz * ny * (nx - 1) : 0, nfy = ny > 1 ? (size_t)nz * (ny - 1) * nx : 0;
+    auto* L = new orbit_fct();
+    L->ctx = ctx;
+    L->nz = nz; L->ny = ny; L->nx = nx; L->conc = fg.conc;
+    double* R = nullptr;
+    bool ok = hipMalloc((void**)&L->wx, std::max<size_t>(2 * nfx, 1) * 8) == hipSuccess &&
+              hipMalloc((void**)&L->wy, std::max<size_t>(2 * nfy, 1) * 8) == hipSuccess &&
+              hipMalloc((void**)&L->dP, N * 8) == hipSuccess && hipMalloc((void**)&L->Dz, N * 8) == hipSuccess &&
+              hipMalloc((void**)&L->tmp, 2 * N * 8) == hipSuccess && hipMalloc((void**)&R, 2 * N * 8) == hipSuccess;
+    if (!ok) {
+        if (R) hipFree(R);
+        orbit_fct_linop_destroy(L);
+        return fail(ORBIT_ERR_NOMEM, "fct_linop_create_dev: device allocation failed");
+    }
+    hipMemcpyAsync(L->dP, fg.dP, N * 8, hipMemcpyDeviceToDevice, st);
+    if (fg.conc && fg.Dz) hipMemcpyAsync(L->Dz, fg.Dz, N * 8, hipMemcpyDeviceToDevice, st);
+    hipLaunchKernelGGL(k_fct_R, g1(N), dim3(256), 0, st, fg, c_base_dev, (const double*)nullptr, dt, R, R + N, 0LL, 0LL);
+    if (nfx + nfy > 0)
+        hipLaunchKernelGGL(k_fct_gains, g1(nfx + nfy), dim3(256), 0, st, fg, c_base_dev, R, R + N, L->wx, L->wy);
+    hipError_t e = hipGetLastError();
+    hipStreamSynchronize(st);
+    hipFree(R);
+    if (e != hipSuccess) {
+        orbit_fct_linop_destroy(L);
+        return fail(ORBIT_ERR_HIP, std::string("fct_linop_create_dev: ") + hipGetErrorString(e));
+    }
+    *out = L;
+    return ORBIT_OK;
+}

@@ -1,0 +1,176 @@
+"""FCT deferred-correction drivers around the device periodic solve.
+
+* forward_fct: the forward month's horizontal FCT as a RHS source recomputed
+  from the previous pass's end-of-bin orbit, for a fixed number of passes
+  (orbit/cli.py:1207-1241; pass 0 is the low-order orbit).
+* marginal_fct: tangent-linear orbit with the frozen L_AD as a deferred RHS
+  source, extra_rhs_tau = -L_AD,tau dc[tau+1], solved by outer GMRES on
+  (I - K)x = b over the flattened 9-state orbits (orbit/modes/marginal.py:627-730).
+* adjoint_fct: adjoint orbit with the transposed frozen L_AD as a receptor
+  source, nu_tau = -dtau L_AD,(tau-1)^T mu_(tau-1), mu = lambda - R, outer GMRES on
+  mu (orbit/modes/adjoint.py:814-962).
+
+K is not a contraction in the reference's production runs (dominant eigenvalue
+about 1.33, orbit/modes/adjoint.py:793-802), hence GMRES rather than Picard.
+All vectors are CUDA float64 tensors; every inner solve runs on the device.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Sequence
+
+import torch
+
+from .core import N_BINS, OperatorSet
+from .fct import FctGrid, FctLinearOperator, apply_dev
+
+
+def make_fct_grids(grids: Sequence, ctx=None) -> List[FctGrid]:
+    return [FctGrid(g, ctx) for g in grids]
+
+
+def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, n_passes: int = 6,
+                tol: float = 1e-8, precision: str = "mixed", ghost=None, maxiter: int = 200, **kw):
+    """e: (nt, n_bins, N) CUDA float64.  ghost: optional (nt, n_bins, N) lateral
+    inflow concentration per bin.  Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
+    if any(g.ctx is not op.ctx for g in fct_grids):
+        raise ValueError("FCT grids and the operator set must share one context (stream)")
+    orbit, info = op.solve_forward(e, tracers, tol=tol, precision=precision, maxiter=maxiter, **kw)
+    hist = [info]
+    e_eff = torch.empty_like(e)
+    for _ in range(1, n_passes):
+        for tau in range(op.n_bins):
+            fct_grids[tau].source_batch(orbit[:, tau + 1], e_eff[:, tau], base=e[:, tau], dt=op.dtau,
+                                        ghost=None if ghost is None else ghost[:, tau])
+        orbit, info = op.solve_forward(e_eff, tracers, c_warm=orbit[:, 0], tol=tol, precision=precision,
+                                       maxiter=maxiter, **kw)
+        hist.append(info)
+    return orbit, hist
+
+
+def frozen_operators(op: OperatorSet, fct_grids: Sequence[FctGrid], orbit_base) -> List[List[FctLinearOperator]]:
+    """L_AD[tau][slot] frozen at the baseline end-of-bin state orbit_base[slot, tau+1]
+    (orbit/modes/marginal.py:450-473)."""
+    out = []
+    for tau in range(op.n_bins):
+        row = []
+        for s in range(orbit_base.shape[0]):
+            cb = orbit_base[s, tau + 1].contiguous()
+            row.append(fct_grids[tau].linop(cb, dt=op.dtau))
+        out.append(row)
+    return out
+
+
+def gmres_torch(matvec, b, x0, atol: float, restart: int, max_cycles: int, callback=None):
+    """Restarted GMRES(restart) with modified Gram-Schmidt and Givens rotations on
+    device vectors (scipy.sparse.linalg.gmres semantics for rtol=0: stop when the
+    residual estimate is <= atol; max_cycles restart cycles).  Returns (x, code, iters)."""
+    x = x0.clone()
+    iters = 0
+    for _ in range(max_cycles):
+        r = b - matvec(x)
+        beta = float(torch.linalg.vector_norm(r))
+        if beta <= atol:
+            return x, 0, iters
+        V = [r / beta]
+        H = torch.zeros((restart + 1, restart), dtype=torch.float64)
+        cs = torch.zeros(restart, dtype=torch.float64)
+        sn = torch.zeros(restart, dtype=torch.float64)
+        g = torch.zeros(restart + 1, dtype=torch.float64)
+        g[0] = beta
+        k = 0
+        for j in range(restart):
+            w = matvec(V[j])
+            for i in range(j + 1):
+                h = float(torch.dot(V[i], w))
+                H[i, j] = h
+                w = w - h * V[i]
+            hn = float(torch.linalg.vector_norm(w))
+            H[j + 1, j] = hn
+            V.append(w / hn if hn > 0 else w)
+            for i in range(j):
+                a, bb = float(H[i, j]), float(H[i + 1, j])
+                H[i, j] = cs[i] * a + sn[i] * bb
+                H[i + 1, j] = -sn[i] * a + cs[i] * bb
+            a, bb = float(H[j, j]), float(H[j + 1, j])
+            rr = math.hypot(a, bb)
+            cs[j], sn[j] = (a / rr, bb / rr) if rr > 0 else (1.0, 0.0)
+            H[j, j] = rr
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            iters += 1
+            k = j + 1
+            res = abs(float(g[j + 1]))
+            if callback is not None:
+                callback(res)
+            if res <= atol:
+                break
+        y = torch.linalg.solve_triangular(H[:k, :k], g[:k, None], upper=True)[:, 0]
+        for i in range(k):
+            x = x + float(y[i]) * V[i]
+    r = b - matvec(x)
+    return x, 0 if float(torch.linalg.vector_norm(r)) <= atol else 1, iters
+
+
+def marginal_fct(op: OperatorSet, L_AD, de, tracers=None, tol: float = 1e-10, precision: str = "fp64",
+                 max_iters: int = 40, rel_tol: float = 2.0e-3, verbose: bool = False):
+    """Tangent-linear orbits dc (nt, n_bins+1, N) with the frozen FCT source."""
+    nt, nb, N = de.shape
+    tmp = torch.empty(N, dtype=torch.float64, device=de.device)
+
+    def solve_with(prev):
+        e = de.clone()
+        if prev is not None:
+            prev = prev.view(nt, nb + 1, N)
+            for tau in range(nb):
+                for s in range(nt):
+                    apply_dev(L_AD[tau][s], prev[s, tau + 1].contiguous(), tmp)
+                    e[s, tau] -= tmp
+        orbit, _ = op.solve_forward(e, tracers, tol=tol, precision=precision)
+        return orbit.reshape(-1)
+
+    b = solve_with(None)
+    b_norm = float(torch.linalg.vector_norm(b))
+    restart = min(max_iters, 30)
+    hist = []
+    x, code, iters = gmres_torch(lambda v: v - (solve_with(v) - b), b, b, 0.3 * rel_tol * b_norm, restart,
+                                 max(1, -(-max_iters // restart)), callback=hist.append)
+    g = solve_with(x)
+    rel = float((g - x).abs().max() / max(float(g.abs().max()), 1e-30))
+    if verbose:
+        print(f"  [FCT] marginal outer GMRES: {iters} iters, max rel dc residual {rel:.3e}")
+    return g.view(nt, nb + 1, N), {"outer_iters": iters, "rel_final": rel, "converged": rel < rel_tol,
+                                   "history": hist}
+
+
+def adjoint_fct(op: OperatorSet, L_AD, R, tracers=None, tol: float = 1e-10, precision: str = "fp64",
+                max_iters: int = 40, rel_tol: float = 2.0e-3, verbose: bool = False):
+    """Adjoint orbits with the transposed frozen FCT source.  Returns
+    (lam (nt, n_bins+1, N), R_eff (nt, n_bins, N), info); dJ/de_tau = dtau (lam_tau - R_eff_tau)."""
+    nt, nb, N = R.shape
+    tmp = torch.empty(N, dtype=torch.float64, device=R.device)
+
+    def run(mu):
+        Re = R.clone()
+        if mu is not None:
+            mu = mu.view(nt, nb, N)
+            for tau in range(nb):
+                src = (tau - 1) % nb
+                for s in range(nt):
+                    apply_dev(L_AD[src][s], mu[s, src].contiguous(), tmp, trans=True)
+                    Re[s, tau] -= op.dtau * tmp
+        lam, _ = op.solve_adjoint(Re, tracers, tol=tol, precision=precision)
+        return (lam[:, :nb] - Re).reshape(-1), lam, Re
+
+    b = run(None)[0]
+    b_norm = float(torch.linalg.vector_norm(b))
+    restart = min(max_iters, 30)
+    hist = []
+    x, code, iters = gmres_torch(lambda v: v - (run(v)[0] - b), b, b, 0.3 * rel_tol * b_norm, restart,
+                                 max(1, -(-max_iters // restart)), callback=hist.append)
+    g, lam, Re = run(x)
+    rel = float((g - x).abs().max() / max(float(g.abs().max()), 1e-30))
+    if verbose:
+        print(f"  [FCT] adjoint outer GMRES: {iters} iters, max rel mu residual {rel:.3e}")
+    return lam, Re, {"outer_iters": iters, "rel_final": rel, "converged": rel < rel_tol, "history": hist}

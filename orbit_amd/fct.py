@@ -133,3 +133,71 @@ class _Transposed:
 
 def assemble_fct_linear_operator(grid, indexer, c_baseline_flat, dt, ctx: Context = None):
     return FctLinearOperator(grid, c_baseline_flat, dt, ctx)
+
+
+class FctGrid:
+    """Device-resident metrics of one bin for repeated, batched FCT sources."""
+
+    def __init__(self, grid, ctx: Context = None):
+        self.ctx = ctx or default_context()
+        keep = []
+        g = _metrics(grid, keep)
+        self.N = g.nz * g.ny * g.nx
+        h = C.c_void_p()
+        check(lib().orbit_fct_grid_create(self.ctx.handle, C.byref(g), C.byref(h)))
+        self.handle = h
+
+    def source_batch(self, c, out, base=None, dt=10800.0, limited=True, ghost=None):
+        """out[v] = base[v] + d_AD(c[v]) for the leading index v of 2-D views
+        (CUDA float64 tensors whose rows may be strided, e.g. orbit[:, tau+1])."""
+        nvec = c.shape[0]
+        for t in (c, out) + ((base,) if base is not None else ()) + ((ghost,) if ghost is not None else ()):
+            if t.stride(-1) != 1 or t.shape[-1] != self.N:
+                raise ValueError("FCT batch views need unit-stride rows of length N")
+        if ghost is not None and ghost.stride(0) != c.stride(0):
+            raise ValueError("ghost rows must share c's row stride")
+        _torch_sync(c)
+        check(lib().orbit_fct_source_batch_dev(
+            self.ctx.handle, self.handle, nvec, c.data_ptr(), c.stride(0),
+            None if ghost is None else ghost.data_ptr(), float(dt), int(bool(limited)),
+            None if base is None else base.data_ptr(), 0 if base is None else base.stride(0),
+            out.data_ptr(), out.stride(0)))
+        self.ctx.sync()
+
+    def linop(self, c_base, dt=10800.0):
+        """Frozen L_AD at the device state c_base (CUDA float64, length N)."""
+        L = FctLinearOperator.__new__(FctLinearOperator)
+        L.ctx = self.ctx
+        L.N = self.N
+        L.shape = (self.N, self.N)
+        h = C.c_void_p()
+        _torch_sync(c_base)
+        check(lib().orbit_fct_linop_create_dev(self.ctx.handle, self.handle, c_base.data_ptr(), float(dt),
+                                               C.byref(h)))
+        L.handle = h
+        return L
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().orbit_fct_grid_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _torch_sync(t):
+    """The library launches on its own stream: finish torch's pending work on t first."""
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
+
+
+def apply_dev(L: FctLinearOperator, x, out, trans=False):
+    """out = L_AD x (or L_AD^T x) for CUDA float64 vectors of length N (synchronous)."""
+    _torch_sync(x)
+    check(lib().orbit_fct_linop_apply_dev(L.ctx.handle, L.handle, int(bool(trans)), x.data_ptr(),
+                                          out.data_ptr()))
+    L.ctx.sync()
