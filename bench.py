@@ -1,21 +1,27 @@
 """Benchmark: periodic-orbit solves/s at 64x70x15 with 12 tracers on one MI355X.
 
-One step = one batched mixed-precision periodic forward solve of the 12
-tracers of a month's operator set (FP32 red-black column sweeps with BF16
-column blocks inside FP64 GMRES, FP64 refinement to 1e-8 relative residual),
-inputs resident in HBM.  value = orbit solves/s = 12 / step time.
+Headline (the production configuration, FCT on): one step = the forward
+periodic orbits of the 12 tracers of a month's operator set with the horizontal
+FCT deferred correction (6 passes, each a periodic solve whose RHS carries the
+anti-diffusive source of the previous pass's end-of-bin orbit), solved as one
+batch in mixed precision (FP32 red-black column sweeps with BF16 column blocks
+inside FP64 GMRES, FP64 refinement to 1e-8 relative residual), inputs resident
+in HBM.  value = orbit solves/s = 12 / step time.
 
-Also reported: the plain FP64 GPU solve (same algorithm in FP64, one orbit at
-a time, no batching, no graphs), the max relative c0 difference of the mixed
-solve from it per tracer, p50/p99 per-step latency, the roofline of the
-dominant kernel (red-black half-sweep) from live HIP-event timing, and a CPU
-baseline (the oracle: SuperLU factors + scipy GMRES) on a bounded sample.
+Beside it: the plain FP64 GPU solve of the same work (same algorithm in FP64,
+one orbit at a time, no batching, no graphs), the per-tracer c0 difference,
+p50/p99 per-step latency, the same without FCT, the adjoint (config 3), 144
+uneven batched orbits (config 2), a Poisson serving sweep (config 4), the
+roofline of the dominant kernel from live HIP-event timing and a CPU baseline
+(the oracle: SuperLU factors + scipy GMRES) on a bounded sample.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
 import argparse
+import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "orbit solves/s (64×70×15, 12 tracers) vs plain FP64 GPU; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+N_PASSES = 6
 
 
 def log(*a):
@@ -49,13 +56,23 @@ def cpu_baseline(wl, iters_hint):
     n_solves = 8 * (iters_hint + 4)
     t_orbit = 8 * t_fac + n_solves * t_sol
     return {
-        "value": 1.0 / t_orbit, "unit": "orbit solves/s", "cores": 1, "kind": "port",
+        "value": 1.0 / t_orbit, "unit": "orbit solves/s (no FCT)", "cores": 1, "kind": "port",
         "solve_only_value": 1.0 / (n_solves * t_sol),
         "sample": (f"SuperLU factor of 2 of 8 bins ({t_fac:.2f} s each) and 16 back-solves "
                    f"({t_sol * 1e3:.1f} ms each) of tracer 0 at 64x70x15, extrapolated to "
                    f"8 factors + {n_solves} solves per orbit (GMRES(30) iterations of the FP64 "
-                   f"solve + s, restart residual and recovery sweeps)"),
+                   f"GPU solve + s, restart residual and recovery sweeps); 1 core"),
     }
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the mixed half-sweep from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by a calibration copy), or None."""
+    path = os.path.join(ROOT, "profiles", "halfsweep_mixed_64x70x15_12tracers.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
 
 
 def main():
@@ -64,8 +81,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only (profiling runs)")
     ap.add_argument("--tracers", type=int, default=12)
-    ap.add_argument("--plain-tracers", type=int, default=3,
+    ap.add_argument("--plain-tracers", type=int, default=2,
                     help="tracers timed one at a time with the plain FP64 solve")
     ap.add_argument("--roof-reps", type=int, default=400,
                     help="back-to-back half-sweep launches timed for the roofline")
@@ -78,21 +96,18 @@ def main():
         sys.exit(2)
 
     import torch
-    from orbit_amd import synth
-    from orbit_amd import _lib
+    from orbit_amd import _lib, drivers, serve, synth
     from orbit_amd.core import Context, OperatorSet
-    import ctypes as C
 
-    t0 = time.perf_counter()
-    wl = synth.config_workload(1, n_tracers=args.tracers)
-    log(f"workload 64x70x15 x {args.tracers} tracers generated in {time.perf_counter() - t0:.1f}s")
-    ctx = Context(0)
-    t0 = time.perf_counter()
-    op = OperatorSet.from_workload(wl, ctx)
-    log(f"operator set on device in {time.perf_counter() - t0:.2f}s, "
-        f"{op.stats()['device_bytes'] / 1e9:.2f} GB")
-    e_dev = torch.from_numpy(wl.e).to("cuda:0")
     T = args.tracers
+    t0 = time.perf_counter()
+    wl = synth.config_workload(1, n_tracers=T)
+    ctx = Context(0)
+    op = OperatorSet.from_workload(wl, ctx)
+    grids = drivers.make_fct_grids(wl.grids, ctx)
+    log(f"config 1: 64x70x15 x {T} tracers, operator set {op.stats()['device_bytes'] / 1e9:.2f} GB "
+        f"on device, setup {time.perf_counter() - t0:.1f}s")
+    e_dev = torch.from_numpy(wl.e).to("cuda:0")
     if args.calib_copy:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
         b = torch.empty_like(a)
@@ -101,7 +116,7 @@ def main():
         del a, b
 
     def step():
-        return op.solve_forward(e_dev, tol=1e-8, precision="mixed")
+        return drivers.forward_fct(op, grids, e_dev, n_passes=N_PASSES, tol=1e-8, precision="mixed")
 
     for _ in range(args.warmup):
         step()
@@ -110,79 +125,149 @@ def main():
     t_start = time.perf_counter()
     for _ in range(args.steps):
         t1 = time.perf_counter()
-        orbit_mixed, info_mixed = step()
+        orbit_mixed, hist = step()
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t1)
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     ms_per_step = 1e3 * wall / args.steps
     value = T * args.steps / wall
-    log(f"mixed: {ms_per_step:.2f} ms/step, {value:.1f} orbit solves/s; iters "
-        f"{[i['gmres_iters'] for i in info_mixed]} rounds {[i['rounds'] for i in info_mixed]}")
+    iters_m = [[i["gmres_iters"] for i in h] for h in hist]
+    log(f"mixed + FCT: {ms_per_step:.2f} ms/step, {value:.1f} orbit solves/s; iters per pass {iters_m[0]}..{iters_m[-1]}")
 
-    # plain FP64: one orbit at a time, no graphs
+    # plain FP64 with the same FCT passes: one orbit at a time, no graphs
     npl = min(args.plain_tracers, T)
-    op.solve_forward(e_dev[:1], tracers=[0], tol=1e-8, precision="fp64")  # warm
+    drivers.forward_fct(op, grids, e_dev[:1], tracers=[0], n_passes=1, tol=1e-8, precision="fp64")  # warm
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     plain = []
     for s in range(npl):
-        o, inf = op.solve_forward(e_dev[s:s + 1], tracers=[s], tol=1e-8, precision="fp64")
-        plain.append((o, inf[0]))
+        o, h = drivers.forward_fct(op, grids, e_dev[s:s + 1], tracers=[s], n_passes=N_PASSES, tol=1e-8,
+                                   precision="fp64")
+        plain.append((o, h))
     torch.cuda.synchronize()
     t_plain = (time.perf_counter() - t1) / npl
     plain_value = 1.0 / t_plain
-    errs = []
-    for s in range(npl):
-        a = orbit_mixed[s, 0].double()
-        b = plain[s][0][0, 0].double()
-        errs.append(float((a - b).abs().max() / b.abs().max()))
-    log(f"plain fp64: {t_plain * 1e3:.1f} ms/orbit, {plain_value:.2f} orbit solves/s; "
-        f"iters {[p[1]['gmres_iters'] for p in plain]}; mixed-vs-fp64 c0 rel err {errs}")
+    errs = [float((orbit_mixed[s, 0] - plain[s][0][0, 0]).abs().max() / plain[s][0][0, 0].abs().max())
+            for s in range(npl)]
+    log(f"plain fp64 + FCT: {t_plain * 1e3:.1f} ms/orbit, {plain_value:.2f} orbit solves/s; "
+        f"mixed-vs-fp64 c0 rel err {errs}")
 
-    # roofline of the dominant kernel: mixed half-sweep on all slots, live HIP events
-    ms = C.c_double()
-    by = C.c_double()
-    _lib.check(_lib.lib().orbit_bench_halfsweep(ctx.handle, op.handle, 0, _lib.ORBIT_PREC_MIXED, T, args.roof_reps,
-                                                C.byref(ms), C.byref(by)))
-    achieved = by.value / (ms.value * 1e-3) / 1e9
-    log(f"halfsweep mixed: {ms.value * 1e3:.2f} us/launch, {by.value / 1e6:.2f} MB, {achieved:.0f} GB/s")
-    ms64 = C.c_double()
-    by64 = C.c_double()
-    _lib.check(_lib.lib().orbit_bench_halfsweep(ctx.handle, op.handle, 0, _lib.ORBIT_PREC_FP64, 1, args.roof_reps,
-                                                C.byref(ms64), C.byref(by64)))
-    log(f"halfsweep fp64 1 slot: {ms64.value * 1e3:.2f} us/launch, "
-        f"{by64.value / (ms64.value * 1e-3) / 1e9:.0f} GB/s")
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(wl, int(plain[0][1]["gmres_iters"]))
-        log(f"cpu baseline: {cpu['value']:.4f} orbit solves/s (solve only {cpu['solve_only_value']:.3f})")
-
-    lat_ms = np.array(lat) * 1e3
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "orbit solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32/bf16 inner sweeps, f64 GMRES+refinement", "data": "synthetic (seeded config 1)",
-        "config": {"workload": "64x70x15 grid, 12 tracers, 8 bins, dense 15x15 convective columns; "
-                               "batched mixed-precision periodic forward solve to 1e-8 rel. residual",
+        "dtype": "f32+bf16 (inner sweeps), f64 (GMRES, residuals, refinement, FCT)",
+        "data": "synthetic (seeded; random-init operators of the production shape)",
+        "config": {"workload": "64x70x15 grid, 12 tracers, 8 bins, dense 15x15 convective columns, "
+                               f"FCT deferred correction ({N_PASSES} passes); batched mixed-precision "
+                               "periodic forward solve to 1e-8 relative residual",
                    "global_batch": T, "seq_len": 8, "parallelism": "single GPU, 12 orbits batched",
-                   "fct_passes": 0},
-        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+                   "fct_passes": N_PASSES},
+        "p50_ms": round(float(np.percentile(np.array(lat) * 1e3, 50)), 3),
+        "p99_ms": round(float(np.percentile(np.array(lat) * 1e3, 99)), 3),
         "plain_fp64_value": round(plain_value, 3),
         "speedup_vs_plain_fp64": round(value / plain_value, 2),
         "max_rel_err_c0_vs_fp64": max(errs),
-        "gmres_iters_mixed": [int(i["gmres_iters"]) for i in info_mixed],
-        "residual_mixed_max": max(float(i["residual"]) for i in info_mixed),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_halfsweep<float,float,bf16,float>",
-                     "us_per_launch": round(ms.value * 1e3, 3),
-                     "algorithmic_bytes_per_launch": by.value},
-        "cpu_baseline": cpu,
+        "gmres_iters_mixed_per_pass": iters_m[-1] if iters_m else None,
     }
+
+    if not args.quick:
+        # same without FCT (one periodic solve per orbit)
+        op.solve_forward(e_dev, tol=1e-8, precision="mixed")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            op.solve_forward(e_dev, tol=1e-8, precision="mixed")
+        nofct_mixed = 3 * T / (time.perf_counter() - t1)
+        t1 = time.perf_counter()
+        for s in range(npl):
+            op.solve_forward(e_dev[s:s + 1], tracers=[s], tol=1e-8, precision="fp64")
+        nofct_plain = npl / (time.perf_counter() - t1)
+        out["no_fct"] = {"mixed_value": round(nofct_mixed, 2), "plain_fp64_value": round(nofct_plain, 2),
+                         "speedup": round(nofct_mixed / nofct_plain, 2)}
+        log(f"no FCT: mixed {nofct_mixed:.1f}/s, plain {nofct_plain:.1f}/s")
+
+        # config 3: adjoint, receptor on 50 surface cells, batched mixed vs FP64 one at a time
+        rng = np.random.default_rng(3)
+        R = np.zeros((T, 8, wl.N))
+        for s in range(T):
+            cells = rng.choice(wl.nx * wl.ny, 50, replace=False)
+            R[s, 0, cells] = rng.random(50)
+        R_dev = torch.from_numpy(R).cuda()
+        op.solve_adjoint(R_dev, tol=1e-8, precision="mixed")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        lam_m, inf_m = op.solve_adjoint(R_dev, tol=1e-8, precision="mixed")
+        adj_mixed = T / (time.perf_counter() - t1)
+        t1 = time.perf_counter()
+        lam_p = []
+        for s in range(npl):
+            lp, _ = op.solve_adjoint(R_dev[s:s + 1], tracers=[s], tol=1e-8, precision="fp64")
+            lam_p.append(lp)
+        adj_plain = npl / (time.perf_counter() - t1)
+        aerr = [float((lam_m[s, 0] - lam_p[s][0, 0]).abs().max() / lam_p[s][0, 0].abs().max()) for s in range(npl)]
+        out["adjoint"] = {"mixed_value": round(adj_mixed, 2), "plain_fp64_value": round(adj_plain, 2),
+                          "max_rel_err_lambda0_vs_fp64": max(aerr)}
+        log(f"adjoint: mixed {adj_mixed:.1f}/s, plain {adj_plain:.1f}/s, err {max(aerr):.2e}")
+
+        # config 2: 144 uneven orbits = 12 months x 12 tracers, sizes 1:1:1
+        jobs, base_e = [], []
+        sizes = ["24x20", "48x40", "64x70"]
+        for month in range(12):
+            wlm = synth.config_workload(2, n_tracers=12, month=month, size=sizes[month % 3])
+            opm = OperatorSet.from_workload(wlm, ctx)
+            jobs.append((opm, torch.from_numpy(wlm.e).cuda(), list(range(12))))
+            base_e.append(wlm.e)
+        serve.solve_many(jobs[:3], n_workers=4)  # graphs and scratch warm
+        res, wall2 = serve.solve_many(jobs, n_workers=4)
+        out["batched_144"] = {"orbits": 144, "wall_ms": round(wall2 * 1e3, 2),
+                              "value": round(144 / wall2, 2), "unit": "orbit solves/s",
+                              "sizes": "24x20, 48x40, 64x70 (x15) in 1:1:1", "streams": 4}
+        log(f"config 2: 144 orbits in {wall2 * 1e3:.1f} ms = {144 / wall2:.1f}/s")
+
+        # config 4: Poisson serving sweep over resident month operator sets
+        ops = {(m, sizes[m % 3]): jobs[m][0] for m in range(12)}
+        sweeps = []
+        srv = serve.OrbitServer(ops, n_workers=4, max_slots=16, hbm_budget=64e9)
+        try:
+            for rate, nq in ((5.0, 25), (50.0, 60), (200.0, 120)):
+                rq = np.random.default_rng(int(rate))
+
+                def make(r):
+                    m = int(r.integers(0, 12))
+                    key = (m, sizes[m % 3])
+                    opk = ops[key]
+                    nt = int(r.integers(1, 13))
+                    tr = sorted(r.choice(12, nt, replace=False).tolist())
+                    mode = "forward" if r.random() < 0.5 else "adjoint"
+                    return serve.make_query(opk, m, key[1], tr, mode, r, base_e=base_e[m], tol=1e-6)
+
+                rep = serve.run_poisson(srv, rate, nq, make, rq)
+                sweeps.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in rep.items()})
+                log(f"serving at {rate}/s: p50 {rep['p50_ms']:.1f} ms, p99 {rep['p99_ms']:.1f} ms, "
+                    f"{rep['solves_per_s']:.1f} solves/s, mean batch {rep['mean_batch']:.1f}")
+        finally:
+            srv.close()
+        out["serving"] = sweeps
+
+    # roofline of the dominant kernel: mixed half-sweep on all slots, live HIP events
+    ms = C.c_double()
+    by = C.c_double()
+    _lib.check(_lib.lib().orbit_bench_halfsweep(ctx.handle, op.handle, 0, _lib.ORBIT_PREC_MIXED, T,
+                                                args.roof_reps, C.byref(ms), C.byref(by)))
+    achieved = by.value / (ms.value * 1e-3) / 1e9
+    log(f"halfsweep mixed: {ms.value * 1e3:.2f} us/launch, {by.value / 1e6:.2f} MB, {achieved:.0f} GB/s")
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
+                       "kernel": "k_halfsweep<float,float,bf16,float> (12 slots, 64x70x15)",
+                       "us_per_launch": round(ms.value * 1e3, 3), "algorithmic_bytes_per_launch": by.value}
+
+    cpu = None
+    if not args.no_cpu:
+        it = int(plain[0][1][0][0]["gmres_iters"]) if plain else 6
+        cpu = cpu_baseline(wl, it)
+        log(f"cpu baseline: {cpu['value']:.4f} orbit solves/s (solve only {cpu['solve_only_value']:.3f})")
+    out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

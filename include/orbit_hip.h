@@ -203,6 +203,12 @@ int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, cons
 int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,
                                orbit_fct** out);
 
+/* Health check of an opset's stored arrays: counts[14] receives the number of
+ * non-finite entries, summed over bins, of V64, V32, VT64, VT32, H64, H32,
+ * HT64, HT32, d64, d32, P64, P16, PT64, PT16 (no counterpart in the reference;
+ * the reference's factor backend reports failures through UMFPACK status). */
+int orbit_opset_check(orbit_ctx* ctx, orbit_opset* op, long long* counts);
+
 /* Diagnostic with no counterpart in the reference: average device time (ms) of
  * one red-black half-sweep launch of `bin` on nslots tracer slots (tracers
  * 0..nslots-1), precision ORBIT_PREC_FP64 (FP64 blocks) or ORBIT_PREC_MIXED (FP32

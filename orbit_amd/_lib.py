@@ -31,7 +31,7 @@ EXPORTS = [
     "orbit_fct_linop_create", "orbit_fct_linop_apply", "orbit_fct_linop_apply_dev",
     "orbit_fct_linop_csr", "orbit_fct_linop_destroy", "orbit_bench_halfsweep",
     "orbit_fct_grid_create", "orbit_fct_grid_destroy", "orbit_fct_source_batch_dev",
-    "orbit_fct_linop_create_dev",
+    "orbit_fct_linop_create_dev", "orbit_opset_check",
 ]
 
 
@@ -122,6 +122,7 @@ def _declare(lib):
         "orbit_fct_grid_create": (_I, [_P, C.POINTER(GridMetrics), C.POINTER(_P)]),
         "orbit_fct_grid_destroy": (_I, [_P]),
         "orbit_fct_linop_create_dev": (_I, [_P, _P, _P, _D, C.POINTER(_P)]),
+        "orbit_opset_check": (_I, [_P, _P, C.POINTER(C.c_longlong)]),
         "orbit_fct_source_batch_dev": (_I, [_P, _P, _I, _P, C.c_longlong, _P, _D, _I, _P, C.c_longlong,
                                             _P, C.c_longlong]),
     }

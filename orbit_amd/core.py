@@ -135,6 +135,15 @@ class OperatorSet:
         check(lib().orbit_opset_stats_get(self.handle, C.byref(s)))
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    CHECK_FIELDS = ("V64", "V32", "VT64", "VT32", "H64", "H32", "HT64", "HT32", "d64", "d32",
+                    "P64", "P16", "PT64", "PT16")
+
+    def check(self) -> dict:
+        """Non-finite entry counts of every stored array (0 everywhere when healthy)."""
+        c = (C.c_longlong * 14)()
+        check(lib().orbit_opset_check(self.ctx.handle, self.handle, c))
+        return dict(zip(self.CHECK_FIELDS, list(c)))
+
     def update_diag(self, D: np.ndarray):
         D = np.ascontiguousarray(D, dtype=np.float64)
         check(lib().orbit_opset_update_diag(self.ctx.handle, self.handle, ptr(D)))
@@ -148,7 +157,8 @@ class OperatorSet:
         return x
 
     # -- periodic solves ----------------------------------------------------
-    def _solve(self, fn, tracers, inp, warm, opts, device: bool):
+    def _solve(self, fn, tracers, inp, warm, opts, device: bool, ctx: Optional[Context] = None):
+        ctx = ctx or self.ctx
         tracers = np.ascontiguousarray(tracers, dtype=np.int32)
         nt = len(tracers)
         info = (_lib.Info * nt)()
@@ -166,7 +176,7 @@ class OperatorSet:
                 wt = warm.to(torch.float64).reshape(nt, self.N).contiguous()  # kept alive for the call
             # the library runs on its own stream: order it after torch's work
             torch.cuda.current_stream(inp.device).synchronize()
-            rc = fn(self.ctx.handle, self.handle, ptr(tracers), nt, inp.data_ptr(),
+            rc = fn(ctx.handle, self.handle, ptr(tracers), nt, inp.data_ptr(),
                     None if wt is None else wt.data_ptr(), C.byref(opts), out.data_ptr(), info)
         else:
             inp = np.ascontiguousarray(inp, dtype=np.float64)
@@ -175,14 +185,14 @@ class OperatorSet:
             if warm is not None:
                 warm = np.ascontiguousarray(warm, dtype=np.float64).reshape(nt, self.N)
                 wp = ptr(warm)
-            rc = fn(self.ctx.handle, self.handle, ptr(tracers), nt, ptr(inp), wp,
+            rc = fn(ctx.handle, self.handle, ptr(tracers), nt, ptr(inp), wp,
                     C.byref(opts), ptr(out), info)
         check(rc)
         return out, [i.as_dict() for i in info]
 
     def solve_forward(self, e, tracers=None, c_warm=None, tol=1e-8, maxiter=200, restart=30,
                       precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
-                      max_rounds=0, verbose=False, rhs="e"):
+                      max_rounds=0, verbose=False, rhs="e", ctx: Optional[Context] = None):
         """Periodic orbits of tracers for emissions e (nt, n_bins, N).
 
         rhs="s": e is s itself, (nt, N), for (I - M)c0 = s.
@@ -193,11 +203,11 @@ class OperatorSet:
                           use_graph, max_rounds, verbose, rhs)
         dev = hasattr(e, "is_cuda") and e.is_cuda
         fn = lib().orbit_solve_forward_dev if dev else lib().orbit_solve_forward
-        return self._solve(fn, tracers, e, c_warm, opts, dev)
+        return self._solve(fn, tracers, e, c_warm, opts, dev, ctx)
 
     def solve_adjoint(self, R, tracers=None, lam_warm=None, tol=1e-8, maxiter=200, restart=30,
                       precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
-                      max_rounds=0, verbose=False, rhs="e"):
+                      max_rounds=0, verbose=False, rhs="e", ctx: Optional[Context] = None):
         """Adjoint orbits for receptors R (nt, n_bins, N); returns (lam (nt, n_bins+1, N), info).
 
         rhs="s": R is s_adj itself, (nt, N), for (I - M^T) lam0 = s_adj."""
@@ -206,7 +216,7 @@ class OperatorSet:
                           use_graph, max_rounds, verbose, rhs)
         dev = hasattr(R, "is_cuda") and R.is_cuda
         fn = lib().orbit_solve_adjoint_dev if dev else lib().orbit_solve_adjoint
-        return self._solve(fn, tracers, R, lam_warm, opts, dev)
+        return self._solve(fn, tracers, R, lam_warm, opts, dev, ctx)
 
     @staticmethod
     def _opts(tol, maxiter, restart, precision, inner, acc, use_graph, max_rounds, verbose, rhs="e"):

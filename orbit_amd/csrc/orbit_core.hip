@@ -48,6 +48,15 @@ int Ctx::ensure(int slot, size_t bytes) {
     return ORBIT_OK;
 }
 
+// Host-to-device copy of a small host array that completes before returning:
+// a pageable source may be read by the runtime when the stream reaches the copy,
+// so the source must not be freed or rewritten before then.
+int h2d_sync(hipStream_t st, void* dst, const void* src, size_t bytes) {
+    ORBIT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    ORBIT_HIP_CHECK(hipStreamSynchronize(st));
+    return ORBIT_OK;
+}
+
 template <class T>
 T* Opset::alloc(size_t n) {
     void* p = nullptr;
@@ -231,14 +240,14 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     }
     CT hx = 0;
     if (!a.nbr_zero) {
+        // Missing neighbours (domain edges) are skipped, not read: the only
+        // in-bounds stand-in would be this column, which may hold stale data.
         const int j = col / a.nx, i = col - j * a.nx;
-        const int cW = i > 0 ? col - 1 : col;
-        const int cE = i < a.nx - 1 ? col + 1 : col;
-        const int cS = j > 0 ? col - a.nx : col;
-        const int cN = j < a.ny - 1 ? col + a.nx : col;
         const AT* Hc = H + (size_t)col * 64 + lane;
-        hx = (CT)Hc[0] * (CT)x[(size_t)cW * 16 + lane] + (CT)Hc[16] * (CT)x[(size_t)cE * 16 + lane] +
-             (CT)Hc[32] * (CT)x[(size_t)cS * 16 + lane] + (CT)Hc[48] * (CT)x[(size_t)cN * 16 + lane];
+        if (i > 0) hx += (CT)Hc[0] * (CT)x[(size_t)(col - 1) * 16 + lane];
+        if (i < a.nx - 1) hx += (CT)Hc[16] * (CT)x[(size_t)(col + 1) * 16 + lane];
+        if (j > 0) hx += (CT)Hc[32] * (CT)x[(size_t)(col - a.nx) * 16 + lane];
+        if (j < a.ny - 1) hx += (CT)Hc[48] * (CT)x[(size_t)(col + a.nx) * 16 + lane];
     }
     CT r = bv - hx;
     if (!a.own_zero) {
@@ -1063,7 +1072,7 @@ int calibrate_opset(orbit_ctx* ctx, orbit_opset* op) {
     for (int c = 0; c < op->ncol; ++c)
         for (int k = 0; k < op->nz; ++k) hb[(size_t)c * 16 + k] = 1.0 + 0.5 * std::sin(0.37 * c + 1.3 * k);
     const int zero = 0;
-    ORBIT_HIP_CHECK(hipMemcpyAsync(tmap, &zero, 4, hipMemcpyHostToDevice, st));
+    { int rc0 = h2d_sync(st, tmap, &zero, 4); if (rc0) return rc0; }
     ORBIT_HIP_CHECK(hipMemcpyAsync(b, hb.data(), n16 * 8, hipMemcpyHostToDevice, st));
     Batch B{ctx, op, 1, tmap, nullptr, nullptr};
     const long long vstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 256;
@@ -1269,6 +1278,7 @@ int apply_I_minus_M(Solver& S, const double* v, long long vslot) {
             key.tracers = {(int)(intptr_t)S.B.tmap_dev, (int)((intptr_t)S.B.tmap_dev >> 32),
                            (int)(intptr_t)S.B.active_dev, (int)((intptr_t)S.B.active_dev >> 32)};
             auto& graphs = S.B.op->graphs;
+            std::lock_guard<std::mutex> lock(S.B.op->graph_mu);
             auto it = graphs.find(key);
             if (it == graphs.end()) {
                 hipGraph_t graph;
@@ -1324,7 +1334,7 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     const int nb = S.nb;
     dim3 g(grid1(n16).x, ns);
     std::vector<int> act(part);
-    ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.active_dev, act.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
+    { int rc0 = h2d_sync(st, S.B.active_dev, act.data(), ns * sizeof(int)); if (rc0) return rc0; }
     // beta = ||r||, V0 = r / beta
     hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.r, vs, (int)n16, nb, S.partials);
     hipLaunchKernelGGL(k_sum_partials, grid1(ns), dim3(256), 0, st, S.partials, nb, S.scal, ns);
@@ -1369,10 +1379,10 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
         }
         if (!any) break;
         if (changed)
-            ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.active_dev, act.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
+            { int rc0 = h2d_sync(st, S.B.active_dev, act.data(), ns * sizeof(int)); if (rc0) return rc0; }
     }
     // c += V y for the participating slots
-    ORBIT_HIP_CHECK(hipMemcpyAsync(S.kcols, kc.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
+    { int rc0 = h2d_sync(st, S.kcols, kc.data(), ns * sizeof(int)); if (rc0) return rc0; }
     hipLaunchKernelGGL(k_gmres_solve, grid1(ns), dim3(256), 0, st, S.gst, S.kcols, ns, S.h1);
     hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, S.m, S.h1, -1.0, S.c, vs,
                        (int)n16, nb, (double*)nullptr, S.B.part_dev, (const int*)S.kcols);
@@ -1460,9 +1470,9 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     const long long vs = (long long)n16;
     dim3 g(grid1(n16).x, nt);
     std::vector<int> ones(nt, 1), tv(tracers, tracers + nt);
-    ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.tmap_dev, tv.data(), nt * 4, hipMemcpyHostToDevice, st));
-    ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.part_dev, ones.data(), nt * 4, hipMemcpyHostToDevice, st));
-    ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.active_dev, ones.data(), nt * 4, hipMemcpyHostToDevice, st));
+    { int rc0 = h2d_sync(st, S.B.tmap_dev, tv.data(), nt * 4); if (rc0) return rc0; }
+    { int rc0 = h2d_sync(st, S.B.part_dev, ones.data(), nt * 4); if (rc0) return rc0; }
+    { int rc0 = h2d_sync(st, S.B.active_dev, ones.data(), nt * 4); if (rc0) return rc0; }
     // inputs -> device layout (forward: dtau*e; adjoint: R)
     const double scale = dir == FWD ? op->dtau : 1.0;
     if (o.rhs_is_s) {
@@ -1532,7 +1542,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     bool have_orbit = false;
     int next_sweeps = S.acc;
     while (true) {
-        ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.part_dev, part.data(), nt * 4, hipMemcpyHostToDevice, st));
+        { int rc0 = h2d_sync(st, S.B.part_dev, part.data(), nt * 4); if (rc0) return rc0; }
         if (need_check) {
             const bool warm = have_orbit && S.prec == ORBIT_PREC_MIXED;
             rc = forced_sweep(S, false, warm ? next_sweeps : S.acc, warm);
@@ -1567,7 +1577,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             else
                 target[s] = o.tol * snorm[s];
         }
-        ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.part_dev, part.data(), nt * 4, hipMemcpyHostToDevice, st));
+        { int rc0 = h2d_sync(st, S.B.part_dev, part.data(), nt * 4); if (rc0) return rc0; }
         std::vector<int> steps;
         rc = gmres_cycle(S, part, target, steps);
         if (rc) return rc;
@@ -1597,7 +1607,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                 nsw[s] = std::min(S.acc, std::max(2, k));
                 next_sweeps = std::max(next_sweeps, nsw[s]);
             }
-            ORBIT_HIP_CHECK(hipMemcpyAsync(S.nsw_dev, nsw.data(), nt * 4, hipMemcpyHostToDevice, st));
+            { int rc0 = h2d_sync(st, S.nsw_dev, nsw.data(), nt * 4); if (rc0) return rc0; }
         }
     }
     // re-run the last forced sweep for all non-zero slots so every orbit matches its c
@@ -1612,7 +1622,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         }
     const long long os = (long long)(nbn + 1) * n16;
     if (anyzero) {
-        ORBIT_HIP_CHECK(hipMemcpyAsync(S.B.part_dev, zmask.data(), nt * 4, hipMemcpyHostToDevice, st));
+        { int rc0 = h2d_sync(st, S.B.part_dev, zmask.data(), nt * 4); if (rc0) return rc0; }
         for (int k = 0; k <= nbn; ++k)
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)k * n16, os, S.orbit + (size_t)k * n16,
                                os, -1.0, S.orbit + (size_t)k * n16, os, (int)n16, S.B.part_dev);
@@ -1735,7 +1745,7 @@ extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int trace
     double* red = base + 4 * n16;       // nb partials, then the sum at red[nb]
     double* ext = red + nb + 8;
     int* tmap = (int*)(ext + op->N);
-    ORBIT_HIP_CHECK(hipMemcpyAsync(tmap, &tracer, 4, hipMemcpyHostToDevice, st));
+    { int rc0 = h2d_sync(st, tmap, &tracer, 4); if (rc0) return rc0; }
     ORBIT_HIP_CHECK(hipMemcpyAsync(ext, b, (size_t)op->N * 8, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, 1), dim3(256), 0, st, ext, (long long)op->N, rhs,
                        (long long)n16, 1, op->nz, op->ncol, 1.0);
@@ -1797,7 +1807,7 @@ extern "C" int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, i
     int* tmap = (int*)(base + 2 * (size_t)nslots * n16 * xe);
     std::vector<int> tv(nslots);
     std::iota(tv.begin(), tv.end(), 0);
-    ORBIT_HIP_CHECK(hipMemcpyAsync(tmap, tv.data(), nslots * 4, hipMemcpyHostToDevice, st));
+    { int rc0 = h2d_sync(st, tmap, tv.data(), nslots * 4); if (rc0) return rc0; }
     ORBIT_HIP_CHECK(hipMemsetAsync(base, 0, 2 * (size_t)nslots * n16 * xe, st));
     Batch B{ctx, op, nslots, tmap, nullptr, nullptr};
     const int sp = mixed ? SP_F32 : SP_F64;
@@ -1827,5 +1837,58 @@ extern "C" int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, i
     const double per_slot = cols * (16.0 * ae /*d*/ + 256.0 * pe /*P*/ + 16.0 * xe /*b*/ + 32.0 * xe /*x rw*/ +
                                     16.0 * xe /*nbr x*/);
     *bytes_per_launch = shared + nslots * per_slot;
+    return ORBIT_OK;
+}
+
+
+namespace orbit {
+template <class T>
+__device__ __forceinline__ bool finite_of(T v) { return isfinite((double)v); }
+__device__ __forceinline__ bool finite_of(uint16_t v) { return isfinite(bf16_to_f32(v)); }
+
+template <class T>
+__global__ void k_count_nonfinite(const T* __restrict__ a, size_t n, unsigned long long* out) {
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    unsigned long long c = 0;
+    for (; g < n; g += stride)
+        if (!finite_of(a[g])) ++c;
+    if (c) atomicAdd(out, c);
+}
+}  // namespace orbit
+
+extern "C" int orbit_opset_check(orbit_ctx* ctx, orbit_opset* op, long long* counts) {
+    if (!ctx || !op || !counts) return fail(ORBIT_ERR_INVALID, "opset_check: NULL argument");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    unsigned long long* dc = nullptr;
+    ORBIT_HIP_CHECK(hipMalloc(&dc, 14 * sizeof(unsigned long long)));
+    ORBIT_HIP_CHECK(hipMemsetAsync(dc, 0, 14 * sizeof(unsigned long long), st));
+    const size_t ncol = op->ncol, G = op->ngroups, T = op->ntr;
+    const size_t nV = G * ncol * 256, nH = G * ncol * 64, nd = T * op->n16, nP = T * ncol * 256;
+    for (const BinOps& B : op->bins) {
+        dim3 g(1024), b(256);
+        hipLaunchKernelGGL(k_count_nonfinite<double>, g, b, 0, st, B.V64, nV, dc + 0);
+        hipLaunchKernelGGL(k_count_nonfinite<float>, g, b, 0, st, B.V32, nV, dc + 1);
+        hipLaunchKernelGGL(k_count_nonfinite<double>, g, b, 0, st, B.VT64, nV, dc + 2);
+        hipLaunchKernelGGL(k_count_nonfinite<float>, g, b, 0, st, B.VT32, nV, dc + 3);
+        hipLaunchKernelGGL(k_count_nonfinite<double>, g, b, 0, st, B.H64, nH, dc + 4);
+        hipLaunchKernelGGL(k_count_nonfinite<float>, g, b, 0, st, B.H32, nH, dc + 5);
+        hipLaunchKernelGGL(k_count_nonfinite<double>, g, b, 0, st, B.HT64, nH, dc + 6);
+        hipLaunchKernelGGL(k_count_nonfinite<float>, g, b, 0, st, B.HT32, nH, dc + 7);
+        hipLaunchKernelGGL(k_count_nonfinite<double>, g, b, 0, st, B.d64, nd, dc + 8);
+        hipLaunchKernelGGL(k_count_nonfinite<float>, g, b, 0, st, B.d32, nd, dc + 9);
+        hipLaunchKernelGGL(k_count_nonfinite<double>, g, b, 0, st, B.P64, nP, dc + 10);
+        hipLaunchKernelGGL(k_count_nonfinite<uint16_t>, g, b, 0, st, B.P16, nP, dc + 11);
+        hipLaunchKernelGGL(k_count_nonfinite<double>, g, b, 0, st, B.PT64, nP, dc + 12);
+        hipLaunchKernelGGL(k_count_nonfinite<uint16_t>, g, b, 0, st, B.PT16, nP, dc + 13);
+    }
+    unsigned long long hc[14];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hc, dc, sizeof(hc), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(dc);
+    if (e != hipSuccess) return fail(ORBIT_ERR_HIP, std::string("opset_check: ") + hipGetErrorString(e));
+    for (int i = 0; i < 14; ++i) counts[i] = (long long)hc[i];
     return ORBIT_OK;
 }

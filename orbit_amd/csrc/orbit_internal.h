@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -103,6 +104,7 @@ struct Opset {
     std::vector<void*> owned;
     double bytes = 0.0;
     std::map<GraphKey, hipGraphExec_t> graphs;
+    std::mutex graph_mu;  // contexts on several host threads share an opset
     template <class T>
     T* alloc(size_t n);
 };

@@ -1,0 +1,236 @@
+"""Serving loop: continuous batching of forward-marginal and adjoint orbit queries.
+
+The reference has no serving path (batch/CLI only); this is the request loop
+the periodic-orbit solve is wrapped in for interactive marginal-emission
+queries.  Every month's operator set stays resident on the device
+(OperatorSet: shared column blocks, stencils, factored BF16/FP64 column blocks).
+
+* A query names a month, a sub-domain size class, 1-12 tracers of that month's
+  set and a mode: "forward" (marginal: the linear response dc to a perturbation
+  de, a forward periodic solve) or "adjoint" (receptor weights R at c_0, an
+  adjoint periodic solve giving dJ/de for every bin).
+* Admission: a query is admitted when the device memory its batch needs
+  (scratch of the solve + its inputs/outputs) fits in the free part of the HBM
+  budget; otherwise it waits.
+* Continuous batching: each of n_workers host threads owns a HIP stream
+  (Context).  A free worker takes the oldest waiting query and every other
+  waiting query with the same (month, size, mode), up to max_slots tracers and
+  within the budget, and solves them as one batch; queries that arrive while
+  batches are on the device are picked up by the next free worker, so work of
+  different batches overlaps on the GPU.
+* Results are independent of batching: every kernel and reduction works per
+  tracer slot in a fixed order, so a query's answer is bit-identical alone and
+  in any mix (tests/test_gpu_serve.py).
+"""
+from __future__ import annotations
+
+import dataclasses
+import threading
+import time
+from collections import deque
+from concurrent.futures import Future
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from .core import Context, OperatorSet
+
+
+@dataclasses.dataclass
+class Query:
+    month: int
+    size: str
+    tracers: List[int]
+    mode: str                      # "forward" | "adjoint"
+    payload: object                # CUDA float64 (nt, n_bins, N): de (forward) or R (adjoint)
+    tol: float = 1e-6
+    qid: int = -1
+    t_arrival: float = 0.0
+    t_start: float = 0.0
+    t_done: float = 0.0
+    batch_size: int = 0
+    future: Optional[Future] = None
+
+
+def slot_bytes(op: OperatorSet, restart: int = 30) -> int:
+    """Device bytes one tracer slot of a solve needs (scratch + input + output)."""
+    n16 = op.ny * op.nx * 16
+    nb = op.n_bins
+    doubles = n16 * (nb + (nb + 1) + (restart + 1) + 6) + op.N * (nb + (nb + 1))
+    return int(doubles * 8 + n16 * 3 * 4 + 64 * 1024)
+
+
+class OrbitServer:
+    def __init__(self, opsets: Dict[Tuple[int, str], OperatorSet], n_workers: int = 4,
+                 max_slots: int = 16, hbm_budget: float = 32e9, precision: str = "mixed",
+                 device: int = 0):
+        self.opsets = opsets
+        self.max_slots = max_slots
+        self.budget = float(hbm_budget)
+        self.in_use = 0.0
+        self.precision = precision
+        self.queue: deque = deque()
+        self.cv = threading.Condition()
+        self.stop = False
+        self.done: List[Query] = []
+        self.next_id = 0
+        self.ctxs = [Context(device) for _ in range(n_workers)]
+        self.workers = [threading.Thread(target=self._work, args=(c,), daemon=True) for c in self.ctxs]
+        for w in self.workers:
+            w.start()
+
+    # -- client side ------------------------------------------------------
+    def submit(self, q: Query) -> Future:
+        key = (q.month, q.size)
+        if key not in self.opsets:
+            raise KeyError(f"no resident operator set for month {q.month}, size {q.size}")
+        op = self.opsets[key]
+        if q.mode not in ("forward", "adjoint"):
+            raise ValueError("mode must be 'forward' or 'adjoint'")
+        if any(t < 0 or t >= op.n_tracers for t in q.tracers):
+            raise ValueError("tracer index out of range")
+        if tuple(q.payload.shape) != (len(q.tracers), op.n_bins, op.N):
+            raise ValueError("payload must be (n_tracers, n_bins, N)")
+        q.future = Future()
+        with self.cv:
+            q.qid = self.next_id
+            self.next_id += 1
+            q.t_arrival = time.perf_counter()
+            self.queue.append(q)
+            self.cv.notify()
+        return q.future
+
+    def close(self):
+        with self.cv:
+            self.stop = True
+            self.cv.notify_all()
+        for w in self.workers:
+            w.join()
+        for c in self.ctxs:
+            c.close()
+
+    # -- worker side ------------------------------------------------------
+    def _take_batch(self):
+        """Oldest waiting query plus compatible ones that fit (called with cv held)."""
+        while self.queue:
+            head = self.queue[0]
+            op = self.opsets[(head.month, head.size)]
+            per = slot_bytes(op)
+            if self.in_use + per * len(head.tracers) > self.budget and self.in_use > 0:
+                return None, 0.0                      # wait for memory to free up
+            batch, slots = [], 0
+            for q in list(self.queue):
+                if (q.month, q.size, q.mode, q.tol) != (head.month, head.size, head.mode, head.tol):
+                    continue
+                n = len(q.tracers)
+                if batch and (slots + n > self.max_slots or self.in_use + per * (slots + n) > self.budget):
+                    continue
+                batch.append(q)
+                slots += n
+            for q in batch:
+                self.queue.remove(q)
+            need = per * slots
+            self.in_use += need
+            return batch, need
+        return None, 0.0
+
+    def _work(self, ctx: Context):
+        import torch
+        while True:
+            with self.cv:
+                batch = None
+                while not self.stop:
+                    batch, need = self._take_batch()
+                    if batch:
+                        break
+                    self.cv.wait(timeout=0.05)
+                if self.stop and not batch:
+                    return
+            t0 = time.perf_counter()
+            try:
+                head = batch[0]
+                op = self.opsets[(head.month, head.size)]
+                tracers = [t for q in batch for t in q.tracers]
+                inp = torch.cat([q.payload for q in batch], dim=0).contiguous()
+                if head.mode == "forward":
+                    out, info = op.solve_forward(inp, tracers, tol=head.tol, precision=self.precision, ctx=ctx)
+                else:
+                    out, info = op.solve_adjoint(inp, tracers, tol=head.tol, precision=self.precision, ctx=ctx)
+                t1 = time.perf_counter()
+                k = 0
+                for q in batch:
+                    n = len(q.tracers)
+                    q.t_start, q.t_done, q.batch_size = t0, t1, len(tracers)
+                    q.future.set_result((out[k:k + n], info[k:k + n]))
+                    k += n
+            except Exception as exc:  # surfaced to every query of the batch
+                for q in batch:
+                    q.future.set_exception(exc)
+            finally:
+                with self.cv:
+                    self.in_use -= need
+                    self.done.extend(batch)
+                    self.cv.notify_all()
+
+
+def make_query(op: OperatorSet, month: int, size: str, tracers, mode: str, rng, base_e=None,
+               tol: float = 1e-6):
+    """Synthetic query: forward = 1% lognormal perturbation of the month's
+    emissions of the chosen tracers; adjoint = a receptor on 10-100 random
+    surface cells with U(0,1) weights at c_0."""
+    import torch
+    nt = len(tracers)
+    if mode == "forward":
+        de = 0.01 * base_e[tracers] * rng.lognormal(0.0, 1.0, base_e[tracers].shape)
+        payload = de
+    else:
+        R = np.zeros((nt, op.n_bins, op.N))
+        ncol = op.nx * op.ny
+        for i in range(nt):
+            m = int(rng.integers(10, 101))
+            cells = rng.choice(ncol, m, replace=False)
+            R[i, 0, cells] = rng.random(m)
+        payload = R
+    return Query(month=month, size=size, tracers=list(tracers), mode=mode,
+                 payload=torch.from_numpy(np.ascontiguousarray(payload)).to("cuda:0"), tol=tol)
+
+
+def run_poisson(server: OrbitServer, rate: float, n_queries: int, make, rng) -> dict:
+    """Submit n_queries with exponential inter-arrival times at `rate` per second;
+    return latency percentiles and throughput."""
+    queries = [make(rng) for _ in range(n_queries)]
+    gaps = rng.exponential(1.0 / rate, n_queries)
+    t0 = time.perf_counter()
+    futs = []
+    t_next = t0
+    for q, g in zip(queries, gaps):
+        t_next += g
+        dt = t_next - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        futs.append(server.submit(q))
+    for f in futs:
+        f.result()
+    wall = time.perf_counter() - t0
+    lat = np.array([q.t_done - q.t_arrival for q in queries]) * 1e3
+    solves = sum(len(q.tracers) for q in queries)
+    return {"rate": rate, "queries": n_queries, "orbit_solves": solves, "wall_s": wall,
+            "solves_per_s": solves / wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_batch": float(np.mean([q.batch_size for q in queries]))}
+
+
+def solve_many(jobs, n_workers: int = 4, precision: str = "mixed", tol: float = 1e-8):
+    """Batched uneven orbits: jobs = [(OperatorSet, e (nt, n_bins, N) CUDA, tracers)],
+    run as concurrent batches on n_workers streams.  Returns (results, wall seconds)."""
+    opsets = {(i, "job"): j[0] for i, j in enumerate(jobs)}
+    srv = OrbitServer(opsets, n_workers=n_workers, max_slots=max(len(j[2]) for j in jobs),
+                      hbm_budget=1e12, precision=precision)
+    try:
+        t0 = time.perf_counter()
+        futs = [srv.submit(Query(month=i, size="job", tracers=list(j[2]), mode="forward", payload=j[1], tol=tol))
+                for i, j in enumerate(jobs)]
+        res = [f.result() for f in futs]
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    return res, wall

@@ -182,3 +182,22 @@ def test_mass_balance_matches_oracle():
         assert got[k] == pytest.approx(ref[k], rel=1e-9, abs=1e-12), k
     np.testing.assert_allclose(got["layer_p90"], ref["layer_p90"], rtol=1e-9, atol=1e-12)
     assert got["interior_max"] < 1e-9
+
+
+def test_shared_context_many_opsets_stay_finite():
+    """Operator sets of different sizes share one context's scratch: a solve must
+    never read scratch it did not write (stale values, even NaN, are harmless)."""
+    import torch
+    outs = []
+    for month, size, nt in ((0, "24x20", 3), (1, "48x40", 3), (0, "24x20", 4), (2, "48x40", 2)):
+        wl = synth.config_workload(2, n_tracers=nt, month=month, size=size)
+        op = OperatorSet.from_workload(wl)
+        assert not any(op.check().values())
+        e = torch.from_numpy(wl.e).cuda()
+        for prec in ("mixed", "fp64"):
+            orbit, info = op.solve_forward(e, tol=1e-8, precision=prec)
+            assert torch.isfinite(orbit).all(), (size, nt, prec)
+            assert all(i["converged"] and np.isfinite(i["s_norm"]) for i in info)
+            outs.append(orbit)
+        lam, ainfo = op.solve_adjoint(e, tol=1e-8, precision="mixed")
+        assert torch.isfinite(lam).all() and all(i["converged"] for i in ainfo)
