@@ -50,6 +50,11 @@ typedef struct {
     int rhs_is_s;        /* 1: the input is s itself ([nt][N]) for (I-M)c0 = s (or
                             (I-M^T)lam0 = s_adj); states 1..n_bins are then the
                             unforced propagation of c0 */
+    int warm_orbit;      /* 1: c_warm is a whole orbit [nt][n_bins+1][N] (an output
+                            of an earlier solve); mixed precision starts its first
+                            accurate sweep from those states */
+    int warm_sweeps;     /* sweeps of that first warm-started sweep (0: the cold count);
+                            enough for its states to reach 1e-12 relative */
 } orbit_opts;
 
 typedef struct {
@@ -68,6 +73,7 @@ typedef struct {
     int default_inner_sweeps, default_acc_sweeps;
     double dtau;
     double device_bytes;  /* HBM held by the opset (blocks, inverses, stencils) */
+    double sweep_contraction;  /* measured error contraction of one red-black sweep (with margin) */
 } orbit_opset_stats;
 
 /* Air-mass-balance statistics (reference orbit/core/operator.py:598-652). */

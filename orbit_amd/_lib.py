@@ -49,7 +49,7 @@ class Opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("restart", C.c_int),
                 ("precision", C.c_int), ("inner_sweeps", C.c_int), ("acc_sweeps", C.c_int),
                 ("use_graph", C.c_int), ("verbose", C.c_int), ("max_rounds", C.c_int),
-                ("rhs_is_s", C.c_int)]
+                ("rhs_is_s", C.c_int), ("warm_orbit", C.c_int), ("warm_sweeps", C.c_int)]
 
 
 class Info(C.Structure):
@@ -65,7 +65,7 @@ class OpsetStats(C.Structure):
     _fields_ = [("nz", C.c_int), ("ny", C.c_int), ("nx", C.c_int), ("n_bins", C.c_int),
                 ("n_tracers", C.c_int), ("n_groups", C.c_int),
                 ("default_inner_sweeps", C.c_int), ("default_acc_sweeps", C.c_int),
-                ("dtau", C.c_double), ("device_bytes", C.c_double)]
+                ("dtau", C.c_double), ("device_bytes", C.c_double), ("sweep_contraction", C.c_double)]
 
 
 class MbStats(C.Structure):

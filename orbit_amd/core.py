@@ -173,7 +173,8 @@ class OperatorSet:
             out = torch.empty((nt, self.n_bins + 1, self.N), dtype=torch.float64, device=inp.device)
             wt = None
             if warm is not None:
-                wt = warm.to(torch.float64).reshape(nt, self.N).contiguous()  # kept alive for the call
+                wshape = (nt, self.n_bins + 1, self.N) if opts.warm_orbit else (nt, self.N)
+                wt = warm.to(torch.float64).reshape(wshape).contiguous()  # kept alive for the call
             # the library runs on its own stream: order it after torch's work
             torch.cuda.current_stream(inp.device).synchronize()
             rc = fn(ctx.handle, self.handle, ptr(tracers), nt, inp.data_ptr(),
@@ -183,7 +184,8 @@ class OperatorSet:
             out = np.empty((nt, self.n_bins + 1, self.N))
             wp = None
             if warm is not None:
-                warm = np.ascontiguousarray(warm, dtype=np.float64).reshape(nt, self.N)
+                wshape = (nt, self.n_bins + 1, self.N) if opts.warm_orbit else (nt, self.N)
+                warm = np.ascontiguousarray(warm, dtype=np.float64).reshape(wshape)
                 wp = ptr(warm)
             rc = fn(ctx.handle, self.handle, ptr(tracers), nt, ptr(inp), wp,
                     C.byref(opts), ptr(out), info)
@@ -192,15 +194,21 @@ class OperatorSet:
 
     def solve_forward(self, e, tracers=None, c_warm=None, tol=1e-8, maxiter=200, restart=30,
                       precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
-                      max_rounds=0, verbose=False, rhs="e", ctx: Optional[Context] = None):
+                      max_rounds=0, verbose=False, rhs="e", ctx: Optional[Context] = None,
+                      warm_sweeps=0):
         """Periodic orbits of tracers for emissions e (nt, n_bins, N).
 
         rhs="s": e is s itself, (nt, N), for (I - M)c0 = s.
+        c_warm: (nt, N) start for c0, or a whole earlier orbit (nt, n_bins+1, N) whose
+        states also start the first accurate sweep (mixed) with warm_sweeps sweeps.
         Returns (orbit (nt, n_bins+1, N), info list).  e may be a numpy array
         (host) or a CUDA torch tensor (device path, no host copies)."""
         tracers = list(range(self.n_tracers)) if tracers is None else list(tracers)
         opts = self._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps,
                           use_graph, max_rounds, verbose, rhs)
+        if c_warm is not None and len(c_warm.shape) == 3:
+            opts.warm_orbit = 1
+            opts.warm_sweeps = int(warm_sweeps)
         dev = hasattr(e, "is_cuda") and e.is_cuda
         fn = lib().orbit_solve_forward_dev if dev else lib().orbit_solve_forward
         return self._solve(fn, tracers, e, c_warm, opts, dev, ctx)

@@ -18,6 +18,7 @@
 //   apply_monodromy_adjoint ... solve   orbit/core/orbit_adjoint.py:36-144
 //   mass_balance_diagnostics            orbit/core/operator.py:598-652
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -164,7 +165,6 @@ __global__ void k_transpose_stencil(const T* __restrict__ H, T* __restrict__ HT,
 // red-black column half-sweep, correction form
 // ------------------------------------------------------------------------- //
 struct SweepArgs {
-    const int* cols;
     int ncols;
     int nx, ny;
     const void* V;
@@ -184,9 +184,10 @@ struct SweepArgs {
     int own_zero;       // x of this colour is known to be zero (not read)
     int nbr_zero;       // x of the other colour is known to be zero (not read)
     int nslots;
-    int tiles_per_xcd;  // column tiles (16 columns) given to each of the 8 XCDs
+    int tiles_per_xcd;  // column tiles given to each of the 8 XCDs
     const int* counts;  // nullable: per-slot sweep count; slots past theirs skip
     int sweep_idx;
+    int colour;         // 0 red ((i + j) even), 1 black
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
@@ -200,70 +201,129 @@ __device__ __forceinline__ void xcd_tile_slot(int L, int nslots, int tiles_per_x
     tile = xcd * tiles_per_xcd + lt;
 }
 
-template <class XT, class AT, class PT, class CT>
+// Column id of the idx-th column of colour c in row-major order.  Rows 2p and
+// 2p+1 together hold exactly nx columns of each colour; row 2p's start at i = c.
+__device__ __forceinline__ void colour_col(int idx, int c, int nx, int& col, int& i, int& j) {
+    const int p = idx / nx, r = idx - p * nx;
+    const int n0 = (nx + 1 - c) >> 1;
+    if (r < n0) {
+        j = 2 * p;
+        i = c + 2 * r;
+    } else {
+        j = 2 * p + 1;
+        i = (1 - c) + 2 * (r - n0);
+    }
+    col = j * nx + i;
+}
+
+// Lanes exchanging through LDS here always belong to one wavefront (a column's
+// 16 levels are 16 consecutive lanes), so wave-level ordering is enough.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One colour's half-sweep: x_c += P_c (b - A x)_c for every column c of the colour
+// and every tracer slot.  Lane k of a 16-lane group owns level k of one column of
+// one slot.  CS = 0: a workgroup is 16 columns of one slot; CS = 1: 4 columns x 4
+// slots, so each wave reads the shared column block and stencil of one column
+// once for four slots.  OZ / NZ: x of this / the other colour is known to be zero
+// and is not read.
+// Every load is in bounds and issued up front (no early exits, no divergent
+// branches around loads): missing edge neighbours read this column's x and are
+// replaced by zero with a select, so stale values there never enter; inactive
+// slots and padding columns compute but do not store.  The per-lane arithmetic
+// does not depend on CS, so a slot's result is the same in any batch.
+template <class XT, class AT, class PT, class CT, int CS, bool OZ, bool NZ>
 __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     __shared__ CT sx[256];
     __shared__ CT sr[256];
-    int tile, slot;
-    xcd_tile_slot(blockIdx.x, a.nslots, a.tiles_per_xcd, tile, slot);
-    if (tile * 16 >= a.ncols) return;          // uniform over the block
-    if (a.active && !a.active[slot]) return;  // uniform over the block
-    if (a.counts && a.sweep_idx >= a.counts[slot]) return;
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    const int idx = tile * 16 + grp;
-    const bool valid = idx < a.ncols;
-    const int col = a.cols[valid ? idx : 0];
-    const int tr = a.tmap[slot];
-    const XT* __restrict__ b = (const XT*)a.b + (size_t)slot * a.bstride;
-    XT* __restrict__ x = (XT*)a.x + (size_t)slot * a.xstride;
+    int tile, sg, slot, idx;
+    if (CS) {
+        xcd_tile_slot(blockIdx.x, (a.nslots + 3) >> 2, a.tiles_per_xcd, tile, sg);
+        if (tile * 4 >= a.ncols) return;  // uniform over the block
+        slot = sg * 4 + (grp & 3);
+        idx = tile * 4 + (grp >> 2);
+    } else {
+        xcd_tile_slot(blockIdx.x, a.nslots, a.tiles_per_xcd, tile, sg);
+        if (tile * 16 >= a.ncols) return;  // uniform over the block
+        slot = sg;
+        idx = tile * 16 + grp;
+    }
+    const bool valid = idx < a.ncols && slot < a.nslots;
+    const int ls = slot < a.nslots ? slot : a.nslots - 1;
+    int col, i, j;
+    colour_col(idx < a.ncols ? idx : 0, a.colour, a.nx, col, i, j);
+    const size_t base = (size_t)col * 16 + lane;
+    const XT* __restrict__ b = (const XT*)a.b + (size_t)ls * a.bstride;
+    XT* __restrict__ x = (XT*)a.x + (size_t)ls * a.xstride;
+    const int tr = a.tmap[ls];
+    int act = 1, cnt = 0x7fffffff;
+    if (a.active) act = a.active[ls];
+    if (a.counts) cnt = a.counts[ls];
+
+    // tracer-independent loads: rhs, own x, neighbours' x
+    const CT bv = (CT)b[base];
+    const CT xo = OZ ? CT(0) : (CT)x[base];
+    CT xw = 0, xe = 0, xs = 0, xn = 0;
+    if (!NZ) {
+        xw = (CT)x[(size_t)(i > 0 ? col - 1 : col) * 16 + lane];
+        xe = (CT)x[(size_t)(i < a.nx - 1 ? col + 1 : col) * 16 + lane];
+        xs = (CT)x[(size_t)(j > 0 ? col - a.nx : col) * 16 + lane];
+        xn = (CT)x[(size_t)(j < a.ny - 1 ? col + a.nx : col) * 16 + lane];
+    }
+    // per-tracer operator rows
     const AT* __restrict__ V = (const AT*)a.V + (size_t)tr * a.vstride;
     const AT* __restrict__ H = (const AT*)a.H + (size_t)tr * a.hstride;
     const AT* __restrict__ d = (const AT*)a.d + (size_t)tr * a.dstride;
     const PT* __restrict__ P = (const PT*)a.P + (size_t)tr * a.pstride;
-    const size_t base = (size_t)col * 16 + lane;
-
-    // Issue every load that does not depend on the exchange first, so their
-    // latencies overlap: P row, V row, stencil, neighbours, rhs, diagonal, own x.
     const PT* Pr = P + (size_t)col * 256 + lane * 16;
     PT prow[16];
 #pragma unroll
     for (int m = 0; m < 16; ++m) prow[m] = Pr[m];
-    const CT bv = (CT)b[base];
-    const CT xo = a.own_zero ? CT(0) : (CT)x[base];
     AT vrow[16];
     CT dv = 0;
-    if (!a.own_zero) {
+    if (!OZ) {
         const AT* Vr = V + (size_t)col * 256 + lane * 16;
 #pragma unroll
         for (int m = 0; m < 16; ++m) vrow[m] = Vr[m];
         dv = (CT)d[base];
     }
-    CT hx = 0;
-    if (!a.nbr_zero) {
-        // Missing neighbours (domain edges) are skipped, not read: the only
-        // in-bounds stand-in would be this column, which may hold stale data.
-        const int j = col / a.nx, i = col - j * a.nx;
+    CT hw = 0, he = 0, hs = 0, hn = 0;
+    if (!NZ) {
         const AT* Hc = H + (size_t)col * 64 + lane;
-        if (i > 0) hx += (CT)Hc[0] * (CT)x[(size_t)(col - 1) * 16 + lane];
-        if (i < a.nx - 1) hx += (CT)Hc[16] * (CT)x[(size_t)(col + 1) * 16 + lane];
-        if (j > 0) hx += (CT)Hc[32] * (CT)x[(size_t)(col - a.nx) * 16 + lane];
-        if (j < a.ny - 1) hx += (CT)Hc[48] * (CT)x[(size_t)(col + a.nx) * 16 + lane];
+        hw = (CT)Hc[0];
+        he = (CT)Hc[16];
+        hs = (CT)Hc[32];
+        hn = (CT)Hc[48];
     }
-    CT r = bv - hx;
-    if (!a.own_zero) {
+    const bool on = valid & (act != 0) & (a.sweep_idx < cnt);
+
+    CT r = bv;
+    if (!NZ) {
+        CT hx = 0;
+        hx += hw * (i > 0 ? xw : CT(0));
+        hx += he * (i < a.nx - 1 ? xe : CT(0));
+        hx += hs * (j > 0 ? xs : CT(0));
+        hx += hn * (j < a.ny - 1 ? xn : CT(0));
+        r -= hx;
+    }
+    if (!OZ) {
         sx[threadIdx.x] = xo;
-        __syncthreads();
+        wave_sync();
         CT acc = dv * xo;
 #pragma unroll
         for (int m = 0; m < 16; ++m) acc += (CT)vrow[m] * sx[(grp << 4) + m];
         r -= acc;
     }
     sr[threadIdx.x] = r;
-    __syncthreads();
+    wave_sync();
     CT dx = 0;
 #pragma unroll
     for (int m = 0; m < 16; ++m) dx += (CT)ldv(prow[m]) * sr[(grp << 4) + m];
-    if (valid) x[base] = (XT)(xo + dx);
+    if (on) x[base] = (XT)(xo + dx);
 }
 
 // y = A x (one operator apply, FP64), used for residual checks of the inner solves.
@@ -612,6 +672,8 @@ extern "C" void orbit_opts_default(orbit_opts* o) {
     o->verbose = 0;
     o->max_rounds = 0;
     o->rhs_is_s = 0;
+    o->warm_orbit = 0;
+    o->warm_sweeps = 0;
 }
 
 extern "C" int orbit_ctx_create(int device, orbit_ctx** out) {
@@ -765,21 +827,10 @@ int finish_opset(orbit_ctx* ctx, orbit_opset* op, std::vector<HostBin>& hb) {
         hipLaunchKernelGGL(k_to_bf16, grid1(nP), dim3(256), 0, st, B.PT64, B.PT16, nP);
         ORBIT_HIP_CHECK(hipGetLastError());
     }
-    // colour lists
-    std::vector<int> cl[2];
-    for (int c = 0; c < ncol; ++c) {
-        int j = c / op->nx, i = c - j * op->nx;
-        cl[(i + j) & 1].push_back(c);
-    }
-    for (int c = 0; c < 2; ++c) {
-        op->ncols_c[c] = (int)cl[c].size();
-        op->cols_dev[c] = op->alloc<int>(std::max<size_t>(cl[c].size(), 1));
-        if (!op->cols_dev[c]) return fail(ORBIT_ERR_NOMEM, "opset: device allocation failed");
-        if (!cl[c].empty()) {
-            int rc = upload(st, op->cols_dev[c], cl[c].data(), cl[c].size() * sizeof(int));
-            if (rc) return rc;
-        }
-    }
+    // columns of each colour ((i + j) even / odd); the sweeps map a colour index
+    // to its column arithmetically
+    op->ncols_c[0] = (ncol + 1) / 2;
+    op->ncols_c[1] = ncol / 2;
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
     return calibrate_opset(ctx, op);
 }
@@ -915,7 +966,7 @@ extern "C" int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st) {
     st->nz = op->nz; st->ny = op->ny; st->nx = op->nx; st->n_bins = op->nbins;
     st->n_tracers = op->ntr; st->n_groups = op->ngroups;
     st->default_inner_sweeps = op->def_inner; st->default_acc_sweeps = op->def_acc;
-    st->dtau = op->dtau; st->device_bytes = op->bytes;
+    st->dtau = op->dtau; st->device_bytes = op->bytes; st->sweep_contraction = op->rho;
     return ORBIT_OK;
 }
 
@@ -991,13 +1042,30 @@ struct Batch {
     int* part_dev;     // slots taking part in the current round
 };
 
+template <class XT, class AT, class PT, class CT>
+int launch_sweep_kernel(const SweepArgs& a, int cs, bool own_zero, bool nbr_zero, dim3 grid, dim3 block,
+                        hipStream_t st) {
+    // (own_zero, nbr_zero) = (true, false) is the second colour of a cold first sweep
+    if (cs) {
+        if (own_zero && nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, true, true>), grid, block, 0, st, a);
+        else if (own_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, true, false>), grid, block, 0, st, a);
+        else if (nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, false, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, false, false>), grid, block, 0, st, a);
+    } else {
+        if (own_zero && nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, true, true>), grid, block, 0, st, a);
+        else if (own_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, true, false>), grid, block, 0, st, a);
+        else if (nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, false, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, false, false>), grid, block, 0, st, a);
+    }
+    return ORBIT_OK;
+}
+
 int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, const void* rhs,
                      long long bstride, void* x, long long xstride, bool own_zero, bool nbr_zero,
                      const int* mask, const int* counts = nullptr, int sweep_idx = 0) {
     orbit_opset* op = B.op;
     const BinOps& O = op->bins[bin];
     SweepArgs a;
-    a.cols = op->cols_dev[colour];
     a.ncols = op->ncols_c[colour];
     if (a.ncols == 0) return ORBIT_OK;
     a.nx = op->nx;
@@ -1016,23 +1084,27 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     a.nbr_zero = nbr_zero;
     a.counts = counts;
     a.sweep_idx = sweep_idx;
-    const bool T = dir == ADJ;
-    const int ntiles = (a.ncols + 15) / 16;
+    a.colour = colour;
     a.nslots = B.nslots;
+    const bool T = dir == ADJ;
+    // Reduced-precision sweeps of four or more slots share each column's loads
+    // over four slots of a wave (CS = 1).
+    const int cs = (prec != SP_F64 && B.nslots >= 4) ? 1 : 0;
+    const int ntiles = cs ? (a.ncols + 3) / 4 : (a.ncols + 15) / 16;
     a.tiles_per_xcd = (ntiles + 7) / 8;
-    dim3 grid(8 * a.tiles_per_xcd * B.nslots), block(256);
-    hipStream_t st = B.ctx->stream;
+    dim3 grid(8 * a.tiles_per_xcd * (cs ? (B.nslots + 3) / 4 : B.nslots)), block(256);
     if (prec == SP_F64) {
         a.V = T ? O.VT64 : O.V64; a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT64 : O.P64;
-        hipLaunchKernelGGL((k_halfsweep<double, double, double, double>), grid, block, 0, st, a);
+        return launch_sweep_kernel<double, double, double, double>(a, cs, own_zero, nbr_zero, grid, block,
+                                                                   B.ctx->stream);
     } else if (prec == SP_ACC_MIXED) {
         a.V = T ? O.VT64 : O.V64; a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT16 : O.P16;
-        hipLaunchKernelGGL((k_halfsweep<double, double, uint16_t, double>), grid, block, 0, st, a);
-    } else {
-        a.V = T ? O.VT32 : O.V32; a.H = T ? O.HT32 : O.H32; a.d = O.d32; a.P = T ? O.PT16 : O.P16;
-        hipLaunchKernelGGL((k_halfsweep<float, float, uint16_t, float>), grid, block, 0, st, a);
+        return launch_sweep_kernel<double, double, uint16_t, double>(a, cs, own_zero, nbr_zero, grid, block,
+                                                                     B.ctx->stream);
     }
-    return ORBIT_OK;
+    a.V = T ? O.VT32 : O.V32; a.H = T ? O.HT32 : O.H32; a.d = O.d32; a.P = T ? O.PT16 : O.P16;
+    return launch_sweep_kernel<float, float, uint16_t, float>(a, cs, own_zero, nbr_zero, grid, block,
+                                                              B.ctx->stream);
 }
 
 // x = P_bin(rhs) approximately: `sweeps` red-black sweeps starting from x = 0.
@@ -1490,6 +1562,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     std::vector<double> snorm;
     const int max_cycles = o.max_rounds > 0 ? o.max_rounds : o.maxiter;
     bool need_check = true;
+    int warm_first = 0;  // > 0: the first check starts from a caller-given orbit
     if (S.prec == ORBIT_PREC_MIXED) {
         // s~ by an FP32 forced sweep: only its norm and its direction are used; the
         // residuals that decide convergence are accurate FP64 sweeps.
@@ -1500,7 +1573,15 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         if (rc) return rc;
         rc = norms(S, S.r, vs, (int)n16, snorm);
         if (rc) return rc;
-        if (warm_dev) {
+        if (warm_dev && o.warm_orbit) {
+            // whole previous orbit: its states start the first accurate sweep
+            const long long os_ = (long long)(nbn + 1) * n16;
+            hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
+                               (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
+            const double* c0 = dir == FWD ? S.orbit : S.orbit + (size_t)nbn * n16;
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, c0, os_, c0, os_, 0.0, S.c, vs, (int)n16, nullptr);
+            warm_first = o.warm_sweeps > 0 ? std::min(o.warm_sweeps, S.acc) : S.acc;
+        } else if (warm_dev) {
             hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
                                S.c, vs, 1, op->nz, op->ncol, 1.0);
         } else {
@@ -1539,8 +1620,13 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             conv[s] = 1;
         }
     int cycles = 0;
-    bool have_orbit = false;
-    int next_sweeps = S.acc;
+    bool have_orbit = warm_first > 0;
+    int next_sweeps = warm_first > 0 ? warm_first : S.acc;
+    if (warm_first > 0) {
+        std::vector<int> nsw(nt, warm_first);
+        int rc0 = h2d_sync(st, S.nsw_dev, nsw.data(), nt * 4);
+        if (rc0) return rc0;
+    }
     while (true) {
         { int rc0 = h2d_sync(st, S.B.part_dev, part.data(), nt * 4); if (rc0) return rc0; }
         if (need_check) {
@@ -1669,7 +1755,7 @@ int host_solve(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, cons
     hipSetDevice(ctx->device);
     const size_t nin = (size_t)nt * ((opts && opts->rhs_is_s) ? 1 : op->nbins) * op->N,
                  nout = (size_t)nt * (op->nbins + 1) * op->N,
-                 nw = (size_t)nt * op->N;
+                 nw = (size_t)nt * ((opts && opts->warm_orbit) ? op->nbins + 1 : 1) * op->N;
     double *din = nullptr, *dout = nullptr, *dw = nullptr;
     ORBIT_HIP_CHECK(hipMalloc(&din, nin * 8));
     if (hipMalloc(&dout, nout * 8) != hipSuccess) {

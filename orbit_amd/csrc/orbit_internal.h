@@ -99,7 +99,6 @@ struct Opset {
     int def_inner = 6, def_acc = 14;
     double rho = 0.1;  // calibrated contraction of one red-black sweep (with margin)
     std::vector<BinOps> bins;
-    int* cols_dev[2] = {nullptr, nullptr};  // column ids of each colour
     int ncols_c[2] = {0, 0};
     std::vector<void*> owned;
     double bytes = 0.0;

@@ -30,20 +30,43 @@ def make_fct_grids(grids: Sequence, ctx=None) -> List[FctGrid]:
 
 
 def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, n_passes: int = 6,
-                tol: float = 1e-8, precision: str = "mixed", ghost=None, maxiter: int = 200, **kw):
+                tol: float = 1e-8, precision: str = "mixed", ghost=None, maxiter: int = 200,
+                warm: str = "c0", **kw):
     """e: (nt, n_bins, N) CUDA float64.  ghost: optional (nt, n_bins, N) lateral
-    inflow concentration per bin.  Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
+    inflow concentration per bin.  warm: "orbit" starts each pass from the whole
+    previous orbit (mixed precision), "c0" from its c0 only.
+    Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
     if any(g.ctx is not op.ctx for g in fct_grids):
         raise ValueError("FCT grids and the operator set must share one context (stream)")
     orbit, info = op.solve_forward(e, tracers, tol=tol, precision=precision, maxiter=maxiter, **kw)
     hist = [info]
-    e_eff = torch.empty_like(e)
+    e_prev = e
+    rho = op.stats()["sweep_contraction"]
+    eps = min(1e-12, 0.01 * tol)
     for _ in range(1, n_passes):
+        e_eff = torch.empty_like(e)
         for tau in range(op.n_bins):
             fct_grids[tau].source_batch(orbit[:, tau + 1], e_eff[:, tau], base=e[:, tau], dt=op.dtau,
                                         ghost=None if ghost is None else ghost[:, tau])
-        orbit, info = op.solve_forward(e_eff, tracers, c_warm=orbit[:, 0], tol=tol, precision=precision,
-                                       maxiter=maxiter, **kw)
+        if precision == "mixed" and warm == "orbit":
+            # The previous orbit starts the first accurate sweep.  Its states are
+            # off by at most dtau * sum_tau ||de_tau|| (the propagators contract),
+            # and each sweep shrinks that error by rho.
+            de = op.dtau * torch.linalg.vector_norm(e_eff - e_prev, dim=2).sum(dim=1)
+            cn = torch.linalg.vector_norm(orbit[:, 0], dim=1).clamp_min(1e-300)
+            rel = float((de / cn).max())
+            if rel == 0.0:
+                k = 2
+            elif rel < 1.0:
+                k = max(2, int(math.ceil(math.log(eps / rel) / math.log(rho))) + 1)
+            else:
+                k = 0  # large change (or nan): the cold sweep count
+            orbit, info = op.solve_forward(e_eff, tracers, c_warm=orbit, warm_sweeps=k,
+                                           tol=tol, precision=precision, maxiter=maxiter, **kw)
+        else:
+            orbit, info = op.solve_forward(e_eff, tracers, c_warm=orbit[:, 0], tol=tol, precision=precision,
+                                           maxiter=maxiter, **kw)
+        e_prev = e_eff
         hist.append(info)
     return orbit, hist
 

@@ -138,6 +138,15 @@ def test_alone_and_in_a_batch_give_identical_results(small):
     assert np.array_equal(both[1], alone[0])
     rev, _ = op.solve_forward(wl.e[::-1].copy(), tracers=[2, 1, 0], tol=1e-8, precision="mixed")
     assert np.array_equal(both[0], rev[2])
+    # five slots take the four-slots-per-column sweep mapping: same bits again
+    five, _ = op.solve_forward(np.concatenate([wl.e, wl.e[:2]]), tracers=[0, 1, 2, 0, 1], tol=1e-8,
+                               precision="mixed")
+    for k, t in enumerate((0, 1, 2, 0, 1)):
+        assert np.array_equal(five[k], both[t]), k
+    lam3, _ = op.solve_adjoint(wl.e, tol=1e-8, precision="mixed")
+    lam5, _ = op.solve_adjoint(np.concatenate([wl.e, wl.e[:2]]), tracers=[0, 1, 2, 0, 1], tol=1e-8,
+                               precision="mixed")
+    assert np.array_equal(lam5[4], lam3[1])
 
 
 def test_lu_protocol_call_surface(cfg0):
