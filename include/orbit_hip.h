@@ -217,10 +217,13 @@ int orbit_opset_check(orbit_ctx* ctx, orbit_opset* op, long long* counts);
 
 /* Diagnostic with no counterpart in the reference: average device time (ms) of
  * one red-black half-sweep launch of `bin` on nslots tracer slots (tracers
- * 0..nslots-1), precision ORBIT_PREC_FP64 (FP64 blocks) or ORBIT_PREC_MIXED (FP32
- * vectors, BF16 blocks), measured with HIP events over `reps` back-to-back
- * launches on the context's stream.  Also returns the algorithmic bytes one
- * launch moves (each operand read once, the written colour once). */
+ * 0..nslots-1), precision ORBIT_PREC_FP64 (FP64 blocks), ORBIT_PREC_MIXED (the
+ * inner sweep of the mixed solve: FP32 vectors, BF16 blocks) or
+ * ORBIT_SWEEP_ACC_MIXED (its accurate sweep: FP64 vectors and operator, BF16
+ * blocks), measured with HIP events over `reps` back-to-back launches on the
+ * context's stream.  Also returns the algorithmic bytes one launch moves (each
+ * operand read once, the written colour once). */
+#define ORBIT_SWEEP_ACC_MIXED 2
 int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, int precision, int nslots,
                           int reps, double* ms_per_launch, double* bytes_per_launch);
 

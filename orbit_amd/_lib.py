@@ -20,6 +20,7 @@ ORBIT_ERR_STRUCTURE = -3
 ORBIT_ERR_NOMEM = -4
 ORBIT_PREC_FP64 = 0
 ORBIT_PREC_MIXED = 1
+ORBIT_SWEEP_ACC_MIXED = 2
 
 # Every symbol include/orbit_hip.h declares.
 EXPORTS = [

@@ -169,6 +169,7 @@ struct SweepArgs {
     int nx, ny;
     const void* V;
     long long vstride;  // per tracer (0 when the block is shared)
+    const void* Vt;     // transpose of V (read by the CS mapping, which needs vstride 0)
     const void* H;
     long long hstride;
     const void* d;
@@ -239,6 +240,7 @@ template <class XT, class AT, class PT, class CT, int CS, bool OZ, bool NZ>
 __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     __shared__ CT sx[256];
     __shared__ CT sr[256];
+    __shared__ AT sV[CS ? 4 * 256 : 1];
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
     int tile, sg, slot, idx;
     if (CS) {
@@ -283,12 +285,20 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     PT prow[16];
 #pragma unroll
     for (int m = 0; m < 16; ++m) prow[m] = Pr[m];
-    AT vrow[16];
+    // CS: the wave's column block (shared by its four slots) is staged in LDS,
+    // transposed, 4 elements a lane: sV[m*16 + k] = V[k][m].  Else a row a lane.
+    AT vrow[CS ? 4 : 16];
     CT dv = 0;
     if (!OZ) {
-        const AT* Vr = V + (size_t)col * 256 + lane * 16;
+        if (CS) {
+            const AT* Vt = (const AT*)a.Vt + (size_t)col * 256 + (threadIdx.x & 63) * 4;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) vrow[m] = Vr[m];
+            for (int q = 0; q < 4; ++q) vrow[q] = Vt[q];
+        } else {
+            const AT* Vr = V + (size_t)col * 256 + lane * 16;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) vrow[m] = Vr[m];
+        }
         dv = (CT)d[base];
     }
     CT hw = 0, he = 0, hs = 0, hn = 0;
@@ -312,10 +322,19 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     }
     if (!OZ) {
         sx[threadIdx.x] = xo;
-        wave_sync();
         CT acc = dv * xo;
+        if (CS) {
+            AT* sv = sV + (threadIdx.x >> 6) * 256;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) acc += (CT)vrow[m] * sx[(grp << 4) + m];
+            for (int q = 0; q < 4; ++q) sv[(threadIdx.x & 63) * 4 + q] = vrow[q];
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; ++m) acc += (CT)sv[m * 16 + lane] * sx[(grp << 4) + m];
+        } else {
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; ++m) acc += (CT)vrow[m] * sx[(grp << 4) + m];
+        }
         r -= acc;
     }
     sr[threadIdx.x] = r;
@@ -1089,20 +1108,23 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     const bool T = dir == ADJ;
     // Reduced-precision sweeps of four or more slots share each column's loads
     // over four slots of a wave (CS = 1).
-    const int cs = (prec != SP_F64 && B.nslots >= 4) ? 1 : 0;
+    const int cs = (prec != SP_F64 && B.nslots >= 4 && op->ngroups == 1) ? 1 : 0;
     const int ntiles = cs ? (a.ncols + 3) / 4 : (a.ncols + 15) / 16;
     a.tiles_per_xcd = (ntiles + 7) / 8;
     dim3 grid(8 * a.tiles_per_xcd * (cs ? (B.nslots + 3) / 4 : B.nslots)), block(256);
     if (prec == SP_F64) {
-        a.V = T ? O.VT64 : O.V64; a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT64 : O.P64;
+        a.V = T ? O.VT64 : O.V64; a.Vt = T ? O.V64 : O.VT64;
+        a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT64 : O.P64;
         return launch_sweep_kernel<double, double, double, double>(a, cs, own_zero, nbr_zero, grid, block,
                                                                    B.ctx->stream);
     } else if (prec == SP_ACC_MIXED) {
-        a.V = T ? O.VT64 : O.V64; a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT16 : O.P16;
+        a.V = T ? O.VT64 : O.V64; a.Vt = T ? O.V64 : O.VT64;
+        a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT16 : O.P16;
         return launch_sweep_kernel<double, double, uint16_t, double>(a, cs, own_zero, nbr_zero, grid, block,
                                                                      B.ctx->stream);
     }
-    a.V = T ? O.VT32 : O.V32; a.H = T ? O.HT32 : O.H32; a.d = O.d32; a.P = T ? O.PT16 : O.P16;
+    a.V = T ? O.VT32 : O.V32; a.Vt = T ? O.V32 : O.VT32;
+    a.H = T ? O.HT32 : O.H32; a.d = O.d32; a.P = T ? O.PT16 : O.P16;
     return launch_sweep_kernel<float, float, uint16_t, float>(a, cs, own_zero, nbr_zero, grid, block,
                                                               B.ctx->stream);
 }
@@ -1882,8 +1904,11 @@ extern "C" int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, i
         return fail(ORBIT_ERR_INVALID, "bench: bad bin, slot count or reps");
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
+    if (precision != ORBIT_PREC_FP64 && precision != ORBIT_PREC_MIXED && precision != ORBIT_SWEEP_ACC_MIXED)
+        return fail(ORBIT_ERR_INVALID, "bench: unknown precision");
     const size_t n16 = op->n16;
     const bool mixed = precision == ORBIT_PREC_MIXED;
+    const bool bf16 = precision != ORBIT_PREC_FP64;
     const size_t xe = mixed ? 4 : 8;
     int rc = ctx->ensure(7, 2 * (size_t)nslots * n16 * xe + (size_t)nslots * 4 + 256);
     if (rc) return rc;
@@ -1896,7 +1921,7 @@ extern "C" int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, i
     { int rc0 = h2d_sync(st, tmap, tv.data(), nslots * 4); if (rc0) return rc0; }
     ORBIT_HIP_CHECK(hipMemsetAsync(base, 0, 2 * (size_t)nslots * n16 * xe, st));
     Batch B{ctx, op, nslots, tmap, nullptr, nullptr};
-    const int sp = mixed ? SP_F32 : SP_F64;
+    const int sp = mixed ? SP_F32 : (bf16 ? SP_ACC_MIXED : SP_F64);
     // warm up both colours once
     launch_halfsweep(B, bin, FWD, sp, 0, b, (long long)n16, x, (long long)n16, false, false, nullptr);
     launch_halfsweep(B, bin, FWD, sp, 1, b, (long long)n16, x, (long long)n16, false, false, nullptr);
@@ -1918,7 +1943,7 @@ extern "C" int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, i
     // colour's columns, per slot d, P, b, x (read+write) of the colour, and the
     // other colour's x once.
     const double cols = 0.5 * op->ncol;
-    const double ae = mixed ? 4.0 : 8.0, pe = mixed ? 2.0 : 8.0;
+    const double ae = mixed ? 4.0 : 8.0, pe = bf16 ? 2.0 : 8.0;
     const double shared = cols * (256.0 * ae + 64.0 * ae);
     const double per_slot = cols * (16.0 * ae /*d*/ + 256.0 * pe /*P*/ + 16.0 * xe /*b*/ + 32.0 * xe /*x rw*/ +
                                     16.0 * xe /*nbr x*/);
