@@ -34,7 +34,7 @@ METRIC = "orbit solves/s (64×70×15, 12 tracers) vs plain FP64 GPU; p50/p99 lat
 HBM_PEAK_GBS = 8000.0
 N_PASSES = 6
 # sweep kernel with the most device time per headline step (kernel-trace profile)
-DOMINANT = "accurate"
+DOMINANT = "inner"
 
 
 def log(*a):
