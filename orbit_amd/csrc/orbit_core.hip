@@ -1148,7 +1148,7 @@ int inner_solve(const Batch& B, int bin, int dir, int prec, int sweeps, const vo
 // Contraction of one red-black sweep, measured on every bin of tracer 0 with a
 // fixed smooth right-hand side: rho = (||r_8|| / ||r_3||)^(1/5).  The sweep
 // counts of the solves follow from it: 1e-12 for an accurate apply (20% margin
-// on the rate), 1e-6 for the inexact FP32 one (5% margin).
+// on the rate), 1e-5 for the inexact FP32 one.
 int calibrate_opset(orbit_ctx* ctx, orbit_opset* op) {
     hipStream_t st = ctx->stream;
     const size_t n16 = op->n16;
@@ -1200,9 +1200,11 @@ int calibrate_opset(orbit_ctx* ctx, orbit_opset* op) {
     if (!(rmax < 1.0)) return fail(ORBIT_ERR_INVALID, "opset: red-black sweeps do not contract on this operator");
     op->rho = std::min(0.97, std::max(0.02, 1.2 * rmax));
     op->def_acc = std::min(80, std::max(4, (int)std::ceil(std::log(1e-12) / std::log(op->rho))));
-    // the inexact FP32 apply only has to reach the accuracy the first refinement
-    // cycle asks for (1e-6 of the residual), with a 5% margin on the rate
-    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(1e-6) / std::log(std::min(0.97, 1.05 * rmax)))));
+    // The inexact FP32 apply sets how much one refinement round gains, not the
+    // accuracy of the answer (every round's residual is an accurate FP64 sweep):
+    // 1e-5 per apply keeps the round counts of 1e-6 at fewer sweeps (measured on
+    // the 64x70x15 FCT passes: 5 sweeps, same rounds, 12% less time than 7).
+    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(1e-5) / std::log(std::min(0.97, rmax)))));
     return ORBIT_OK;
 }
 
