@@ -209,6 +209,19 @@ int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, cons
 int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,
                                orbit_fct** out);
 
+/* Species coupling block K (N x N CSR over k-major indices) on the device, for
+ * the forward RHS term e_s -= K_(s,src) c_src[tau] (reference _solve_one_species,
+ * orbit/core/orbit.py:263-307) and the adjoint receptor term
+ * R_src -= dtau K_(r,src)^T (lam_r - R_r) (orbit/modes/adjoint.py:650-676). */
+typedef struct orbit_spmat orbit_spmat;
+int orbit_spmat_create(orbit_ctx* ctx, int n, const int32_t* rowptr, const int32_t* colidx,
+                       const double* vals, orbit_spmat** out);
+/* y += alpha * K x (trans = 0) or alpha * K^T x (trans = 1); x and y are device
+ * vectors of n doubles on the context's stream and must not alias. */
+int orbit_spmat_apply_dev(orbit_ctx* ctx, orbit_spmat* K, int trans, double alpha,
+                          const double* x_dev, double* y_dev);
+int orbit_spmat_destroy(orbit_spmat* K);
+
 /* Health check of an opset's stored arrays: counts[14] receives the number of
  * non-finite entries, summed over bins, of V64, V32, VT64, VT32, H64, H32,
  * HT64, HT32, d64, d32, P64, P16, PT64, PT16 (no counterpart in the reference;

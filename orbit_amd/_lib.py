@@ -32,7 +32,8 @@ EXPORTS = [
     "orbit_fct_linop_create", "orbit_fct_linop_apply", "orbit_fct_linop_apply_dev",
     "orbit_fct_linop_csr", "orbit_fct_linop_destroy", "orbit_bench_halfsweep",
     "orbit_fct_grid_create", "orbit_fct_grid_destroy", "orbit_fct_source_batch_dev",
-    "orbit_fct_linop_create_dev", "orbit_opset_check",
+    "orbit_fct_linop_create_dev", "orbit_opset_check", "orbit_spmat_create",
+    "orbit_spmat_apply_dev", "orbit_spmat_destroy",
 ]
 
 
@@ -120,6 +121,9 @@ def _declare(lib):
         "orbit_fct_linop_csr": (_I, [_P, _P, _P, _P, _P, C.POINTER(_I)]),
         "orbit_fct_linop_destroy": (_I, [_P]),
         "orbit_bench_halfsweep": (_I, [_P, _P, _I, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
+        "orbit_spmat_create": (_I, [_P, _I, _P, _P, _P, C.POINTER(_P)]),
+        "orbit_spmat_apply_dev": (_I, [_P, _P, _I, _D, _P, _P]),
+        "orbit_spmat_destroy": (_I, [_P]),
         "orbit_fct_grid_create": (_I, [_P, C.POINTER(GridMetrics), C.POINTER(_P)]),
         "orbit_fct_grid_destroy": (_I, [_P]),
         "orbit_fct_linop_create_dev": (_I, [_P, _P, _P, _D, C.POINTER(_P)]),
