@@ -58,6 +58,25 @@ int h2d_sync(hipStream_t st, void* dst, const void* src, size_t bytes) {
     return ORBIT_OK;
 }
 
+// Per-slot flags and counts (up to 64 slots) travel as kernel arguments: the
+// write is ordered on the stream like any launch and the host array can be
+// dropped at once, so the host does not wait for the device.
+struct SmallInts {
+    int v[64];
+};
+__global__ void k_set_ints(int* dst, SmallInts a, int n) {
+    const int i = threadIdx.x;
+    if (i < n) dst[i] = a.v[i];
+}
+int set_ints(hipStream_t st, int* dst, const int* src, int n) {
+    if (n > 64) return h2d_sync(st, dst, src, (size_t)n * sizeof(int));
+    SmallInts a;
+    for (int i = 0; i < n; ++i) a.v[i] = src[i];
+    hipLaunchKernelGGL(k_set_ints, dim3(1), dim3(64), 0, st, dst, a, n);
+    ORBIT_HIP_CHECK(hipGetLastError());
+    return ORBIT_OK;
+}
+
 template <class T>
 T* Opset::alloc(size_t n) {
     void* p = nullptr;
@@ -587,6 +606,11 @@ struct GState {
     double hnext;
     double ynorm;  // ||y|| = ||delta|| of the last cycle (orthonormal basis)
 };
+
+__global__ void k_gather_ynorm(const GState* st, int nslots, double* out) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot < nslots) out[slot] = st[slot].ynorm;
+}
 
 // beta = ||r||, g = beta e1; the caller then scales V_0 = r / beta.
 __global__ void k_gmres_start(GState* st, const double* norm2, int nslots, const int* active) {
@@ -1430,7 +1454,7 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     const int nb = S.nb;
     dim3 g(grid1(n16).x, ns);
     std::vector<int> act(part);
-    { int rc0 = h2d_sync(st, S.B.active_dev, act.data(), ns * sizeof(int)); if (rc0) return rc0; }
+    { int rc0 = set_ints(st, S.B.active_dev, act.data(), ns); if (rc0) return rc0; }
     // beta = ||r||, V0 = r / beta
     hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.r, vs, (int)n16, nb, S.partials);
     hipLaunchKernelGGL(k_sum_partials, grid1(ns), dim3(256), 0, st, S.partials, nb, S.scal, ns);
@@ -1475,10 +1499,10 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
         }
         if (!any) break;
         if (changed)
-            { int rc0 = h2d_sync(st, S.B.active_dev, act.data(), ns * sizeof(int)); if (rc0) return rc0; }
+            { int rc0 = set_ints(st, S.B.active_dev, act.data(), ns); if (rc0) return rc0; }
     }
     // c += V y for the participating slots
-    { int rc0 = h2d_sync(st, S.kcols, kc.data(), ns * sizeof(int)); if (rc0) return rc0; }
+    { int rc0 = set_ints(st, S.kcols, kc.data(), ns); if (rc0) return rc0; }
     hipLaunchKernelGGL(k_gmres_solve, grid1(ns), dim3(256), 0, st, S.gst, S.kcols, ns, S.h1);
     hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, S.m, S.h1, -1.0, S.c, vs,
                        (int)n16, nb, (double*)nullptr, S.B.part_dev, (const int*)S.kcols);
@@ -1566,9 +1590,9 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     const long long vs = (long long)n16;
     dim3 g(grid1(n16).x, nt);
     std::vector<int> ones(nt, 1), tv(tracers, tracers + nt);
-    { int rc0 = h2d_sync(st, S.B.tmap_dev, tv.data(), nt * 4); if (rc0) return rc0; }
-    { int rc0 = h2d_sync(st, S.B.part_dev, ones.data(), nt * 4); if (rc0) return rc0; }
-    { int rc0 = h2d_sync(st, S.B.active_dev, ones.data(), nt * 4); if (rc0) return rc0; }
+    { int rc0 = set_ints(st, S.B.tmap_dev, tv.data(), nt); if (rc0) return rc0; }
+    { int rc0 = set_ints(st, S.B.part_dev, ones.data(), nt); if (rc0) return rc0; }
+    { int rc0 = set_ints(st, S.B.active_dev, ones.data(), nt); if (rc0) return rc0; }
     // inputs -> device layout (forward: dtau*e; adjoint: R)
     const double scale = dir == FWD ? op->dtau : 1.0;
     if (o.rhs_is_s) {
@@ -1648,11 +1672,11 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     int next_sweeps = warm_first > 0 ? warm_first : S.acc;
     if (warm_first > 0) {
         std::vector<int> nsw(nt, warm_first);
-        int rc0 = h2d_sync(st, S.nsw_dev, nsw.data(), nt * 4);
+        int rc0 = set_ints(st, S.nsw_dev, nsw.data(), nt);
         if (rc0) return rc0;
     }
     while (true) {
-        { int rc0 = h2d_sync(st, S.B.part_dev, part.data(), nt * 4); if (rc0) return rc0; }
+        { int rc0 = set_ints(st, S.B.part_dev, part.data(), nt); if (rc0) return rc0; }
         if (need_check) {
             const bool warm = have_orbit && S.prec == ORBIT_PREC_MIXED;
             rc = forced_sweep(S, false, warm ? next_sweeps : S.acc, warm);
@@ -1687,7 +1711,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             else
                 target[s] = o.tol * snorm[s];
         }
-        { int rc0 = h2d_sync(st, S.B.part_dev, part.data(), nt * 4); if (rc0) return rc0; }
+        { int rc0 = set_ints(st, S.B.part_dev, part.data(), nt); if (rc0) return rc0; }
         std::vector<int> steps;
         rc = gmres_cycle(S, part, target, steps);
         if (rc) return rc;
@@ -1705,11 +1729,12 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             if (rc) return rc;
             next_sweeps = 2;
             std::vector<int> nsw(nt, 0);
+            hipLaunchKernelGGL(k_gather_ynorm, grid1(nt), dim3(256), 0, st, S.gst, nt, S.resid);
+            ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.resid, nt * 8, hipMemcpyDeviceToHost, st));
+            ORBIT_HIP_CHECK(hipStreamSynchronize(st));
             for (int s = 0; s < nt; ++s) {
                 if (!part[s]) continue;
-                double yn = 0.0;
-                ORBIT_HIP_CHECK(hipMemcpyAsync(&yn, &S.gst[s].ynorm, 8, hipMemcpyDeviceToHost, st));
-                ORBIT_HIP_CHECK(hipStreamSynchronize(st));
+                const double yn = S.host[s];
                 const double rel = cn[s] > 0.0 ? yn / cn[s] : 1.0;
                 int k = S.acc;
                 if (rel > 0.0 && rel < 1.0)
@@ -1717,7 +1742,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                 nsw[s] = std::min(S.acc, std::max(2, k));
                 next_sweeps = std::max(next_sweeps, nsw[s]);
             }
-            { int rc0 = h2d_sync(st, S.nsw_dev, nsw.data(), nt * 4); if (rc0) return rc0; }
+            { int rc0 = set_ints(st, S.nsw_dev, nsw.data(), nt); if (rc0) return rc0; }
         }
     }
     // re-run the last forced sweep for all non-zero slots so every orbit matches its c
@@ -1732,7 +1757,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         }
     const long long os = (long long)(nbn + 1) * n16;
     if (anyzero) {
-        { int rc0 = h2d_sync(st, S.B.part_dev, zmask.data(), nt * 4); if (rc0) return rc0; }
+        { int rc0 = set_ints(st, S.B.part_dev, zmask.data(), nt); if (rc0) return rc0; }
         for (int k = 0; k <= nbn; ++k)
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)k * n16, os, S.orbit + (size_t)k * n16,
                                os, -1.0, S.orbit + (size_t)k * n16, os, (int)n16, S.B.part_dev);
