@@ -6,9 +6,9 @@
 //   Zalesak bounds, P+-, Q+-, R+-, C        fct.py:381-447
 //   divergence into cells                   fct.py:360-378
 //   L_AD gains w = C*beta/dx                fct.py:495-572
-// One thread per cell or face; FP64.  Face quantities are recomputed where they
-// are needed (each needs x at most two cells away), so the only intermediates
-// written to HBM are R+ and R- per cell.
+// One thread per cell or face; FP64.  x = c/rho is formed once per cell; face
+// quantities are recomputed where they are needed (each needs x at most two
+// cells away), so the only other intermediates in HBM are R+ and R- per cell.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -57,23 +57,28 @@ struct Face {
     bool valid;
 };
 
-// x at a cell, from c / rho
-__device__ __forceinline__ double xat(const FctGrid& g, const double* c, size_t n) {
-    return c[n] / rho_at(g, n);
+// x = c / rho for nvec vectors (the transported variable), once per cell: the
+// face fluxes read x at up to two cells away, so forming it once saves the
+// stencil's repeated loads of dP and Dz and its divisions.
+__global__ void k_fct_x(FctGrid g, const double* __restrict__ c, long long cs, double* __restrict__ x,
+                        long long xs, int N) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    x[(size_t)blockIdx.y * xs + n] = c[(size_t)blockIdx.y * cs + n] / rho_at(g, n);
 }
 
-// x face between (k,j,i) [lo] and (k,j,i+1) [hi]
-__device__ Face face_x(const FctGrid& g, const double* c, const double* gh, int k, int j, int i) {
+// x face between (k,j,i) [lo] and (k,j,i+1) [hi]; xg: x of the ghost (nullable)
+__device__ __forceinline__ Face face_x(const FctGrid& g, const double* x, const double* xg, int k, int j, int i) {
     const int nx = g.nx;
     const size_t row = ((size_t)k * g.ny + j) * nx;
-    const double xl = xat(g, c, row + i), xh = xat(g, c, row + i + 1);
+    const double xl = x[row + i], xh = x[row + i + 1];
     double xll, xhh;
-    if (i > 0) xll = xat(g, c, row + i - 1);
-    else xll = gh ? xat(g, gh, row + 0) : xl;
-    if (i + 2 < nx) xhh = xat(g, c, row + i + 2);
-    else xhh = gh ? xat(g, gh, row + nx - 1) : xh;
+    if (i > 0) xll = x[row + i - 1];
+    else xll = xg ? xg[row + 0] : xl;
+    if (i + 2 < nx) xhh = x[row + i + 2];
+    else xhh = xg ? xg[row + nx - 1] : xh;
     double phl = vl_phi(xl, xll, xh), phh = vl_phi(xh, xhh, xl);
-    if (!gh) {
+    if (!xg) {
         if (i == 0) phl = 0.0;
         if (i == nx - 2) phh = 0.0;
     }
@@ -100,18 +105,18 @@ __device__ Face face_x(const FctGrid& g, const double* c, const double* gh, int 
 }
 
 // y face between (k,j,i) [lo = south] and (k,j+1,i) [hi = north]
-__device__ Face face_y(const FctGrid& g, const double* c, const double* gh, int k, int j, int i) {
+__device__ __forceinline__ Face face_y(const FctGrid& g, const double* x, const double* xg, int k, int j, int i) {
     const int nx = g.nx, ny = g.ny;
     const size_t lvl = (size_t)k * ny * nx;
-    auto at = [&](const double* a, int jj) { return xat(g, a, lvl + (size_t)jj * nx + i); };
-    const double xl = at(c, j), xh = at(c, j + 1);
+    auto at = [&](const double* a, int jj) { return a[lvl + (size_t)jj * nx + i]; };
+    const double xl = at(x, j), xh = at(x, j + 1);
     double xll, xhh;
-    if (j > 0) xll = at(c, j - 1);
-    else xll = gh ? at(gh, 0) : xl;
-    if (j + 2 < ny) xhh = at(c, j + 2);
-    else xhh = gh ? at(gh, ny - 1) : xh;
+    if (j > 0) xll = at(x, j - 1);
+    else xll = xg ? at(xg, 0) : xl;
+    if (j + 2 < ny) xhh = at(x, j + 2);
+    else xhh = xg ? at(xg, ny - 1) : xh;
     double phl = vl_phi(xl, xll, xh), phh = vl_phi(xh, xhh, xl);
-    if (!gh) {
+    if (!xg) {
         if (j == 0) phl = 0.0;
         if (j == ny - 2) phh = 0.0;
     }
@@ -140,27 +145,45 @@ __device__ Face face_y(const FctGrid& g, const double* c, const double* gh, int 
     return f;
 }
 
+// Cell kernels run on a (64 x 4)-thread tile of (i, j) per block and one block
+// row per (level, vector): blockIdx.z = k + nz * v.  No integer division.
+struct CellIdx {
+    int i, j, k, v;
+    size_t n;
+    bool ok;
+};
+__device__ __forceinline__ CellIdx cell_index(const FctGrid& g) {
+    CellIdx c;
+    c.i = blockIdx.x * 64 + (threadIdx.x & 63);
+    c.j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    c.k = blockIdx.z % g.nz;
+    c.v = blockIdx.z / g.nz;
+    c.ok = c.i < g.nx && c.j < g.ny;
+    c.n = ((size_t)c.k * g.ny + c.j) * g.nx + c.i;
+    return c;
+}
+
 // R+ and R- per cell (Zalesak 1979 in the steady-state form of fct.py:381-447)
-__global__ void k_fct_R(FctGrid g, const double* __restrict__ c, const double* __restrict__ gh, double dt,
-                        double* __restrict__ Rp, double* __restrict__ Rm, long long cs, long long rs) {
-    const size_t N = (size_t)g.nz * g.ny * g.nx;
-    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    c += (size_t)blockIdx.y * cs;
-    if (gh) gh += (size_t)blockIdx.y * cs;
-    Rp += (size_t)blockIdx.y * rs;
-    Rm += (size_t)blockIdx.y * rs;
-    const int nx = g.nx, ny = g.ny;
-    const int i = (int)(n % nx), j = (int)((n / nx) % ny), k = (int)(n / ((size_t)nx * ny));
-    const double x = xat(g, c, n);
+__global__ __launch_bounds__(256) void k_fct_R(FctGrid g, const double* __restrict__ x,
+                                               const double* __restrict__ xg, double dt, double* __restrict__ Rp,
+                                               double* __restrict__ Rm, long long xs, long long rs) {
+    const CellIdx q = cell_index(g);
+    if (!q.ok) return;
+    x += (size_t)q.v * xs;
+    if (xg) xg += (size_t)q.v * xs;
+    Rp += (size_t)q.v * rs;
+    Rm += (size_t)q.v * rs;
+    const int nx = g.nx, ny = g.ny, i = q.i, j = q.j, k = q.k;
+    const size_t n = q.n;
+    const double xc = x[n];
     // neighbourhood bounds; beyond the edge the ghost (or the edge cell itself)
-    const double* edge = gh ? gh : c;
-    const double xw = i > 0 ? xat(g, c, n - 1) : xat(g, edge, n);
-    const double xe = i < nx - 1 ? xat(g, c, n + 1) : xat(g, edge, n);
-    const double xs = j > 0 ? xat(g, c, n - nx) : xat(g, edge, n);
-    const double xn = j < ny - 1 ? xat(g, c, n + nx) : xat(g, edge, n);
-    const double xmax = fmax(fmax(fmax(x, xw), fmax(xe, xs)), xn);
-    const double xmin = fmin(fmin(fmin(x, xw), fmin(xe, xs)), xn);
+    const double* edge = xg ? xg : x;
+    const double xw = i > 0 ? x[n - 1] : edge[n];
+    const double xe = i < nx - 1 ? x[n + 1] : edge[n];
+    const double xs_ = j > 0 ? x[n - nx] : edge[n];
+    const double xn = j < ny - 1 ? x[n + nx] : edge[n];
+    const double xmax = fmax(fmax(fmax(xc, xw), fmax(xe, xs_)), xn);
+    const double xmin = fmin(fmin(fmin(xc, xw), fmin(xe, xs_)), xn);
     double pp = 0.0, pm = 0.0;
     auto add = [&](double v) {
         pp += fmax(v, 0.0);
@@ -168,25 +191,25 @@ __global__ void k_fct_R(FctGrid g, const double* __restrict__ c, const double* _
     };
     if (nx > 1) {
         if (i > 0) {  // west face: this cell is hi
-            Face f = face_x(g, c, gh, k, j, i - 1);
+            Face f = face_x(g, x, xg, k, j, i - 1);
             add(f.valid ? f.tr_hi * f.A * f.inv : 0.0);
         }
         if (i < nx - 1) {  // east face: this cell is lo
-            Face f = face_x(g, c, gh, k, j, i);
+            Face f = face_x(g, x, xg, k, j, i);
             add(f.valid ? -f.tr_lo * f.A * f.inv : 0.0);
         }
     }
     if (ny > 1) {
         if (j > 0) {
-            Face f = face_y(g, c, gh, k, j - 1, i);
+            Face f = face_y(g, x, xg, k, j - 1, i);
             add(f.valid ? f.tr_hi * f.A * f.inv : 0.0);
         }
         if (j < ny - 1) {
-            Face f = face_y(g, c, gh, k, j, i);
+            Face f = face_y(g, x, xg, k, j, i);
             add(f.valid ? -f.tr_lo * f.A * f.inv : 0.0);
         }
     }
-    const double qp = (xmax - x) / dt, qm = (x - xmin) / dt;
+    const double qp = (xmax - xc) / dt, qm = (xc - xmin) / dt;
     Rp[n] = pp > 0.0 ? fmin(1.0, qp / pp) : 0.0;
     Rm[n] = pm > 0.0 ? fmin(1.0, qm / pm) : 0.0;
 }
@@ -199,51 +222,51 @@ __device__ __forceinline__ double face_C(const Face& f, double Rp_hi, double Rm_
 }
 
 // d = rho * sum over faces of +-tr*C*A/dx   (C = 1 when not limited)
-__global__ void k_fct_div(FctGrid g, const double* __restrict__ c, const double* __restrict__ gh,
-                          const double* __restrict__ Rp, const double* __restrict__ Rm, int limited,
-                          double* __restrict__ d, long long cs, long long rs, long long ds,
-                          const double* __restrict__ base, long long bs) {
-    const size_t N = (size_t)g.nz * g.ny * g.nx;
-    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    c += (size_t)blockIdx.y * cs;
-    if (gh) gh += (size_t)blockIdx.y * cs;
-    Rp += (size_t)blockIdx.y * rs;
-    Rm += (size_t)blockIdx.y * rs;
-    d += (size_t)blockIdx.y * ds;
-    const int nx = g.nx, ny = g.ny;
-    const int i = (int)(n % nx), j = (int)((n / nx) % ny), k = (int)(n / ((size_t)nx * ny));
+__global__ __launch_bounds__(256) void k_fct_div(FctGrid g, const double* __restrict__ x,
+                                                 const double* __restrict__ xg, const double* __restrict__ Rp,
+                                                 const double* __restrict__ Rm, int limited, double* __restrict__ d,
+                                                 long long xs, long long rs, long long ds,
+                                                 const double* __restrict__ base, long long bs) {
+    const CellIdx q = cell_index(g);
+    if (!q.ok) return;
+    x += (size_t)q.v * xs;
+    if (xg) xg += (size_t)q.v * xs;
+    Rp += (size_t)q.v * rs;
+    Rm += (size_t)q.v * rs;
+    d += (size_t)q.v * ds;
+    const int nx = g.nx, ny = g.ny, i = q.i, j = q.j, k = q.k;
+    const size_t n = q.n;
     double acc = 0.0;
     if (nx > 1) {
         if (i > 0) {
-            Face f = face_x(g, c, gh, k, j, i - 1);
+            Face f = face_x(g, x, xg, k, j, i - 1);
             const double C = limited ? face_C(f, Rp[n], Rm[n], Rp[n - 1], Rm[n - 1]) : 1.0;
             if (f.valid) acc += f.tr_hi * C * f.A * f.inv;
         }
         if (i < nx - 1) {
-            Face f = face_x(g, c, gh, k, j, i);
+            Face f = face_x(g, x, xg, k, j, i);
             const double C = limited ? face_C(f, Rp[n + 1], Rm[n + 1], Rp[n], Rm[n]) : 1.0;
             if (f.valid) acc -= f.tr_lo * C * f.A * f.inv;
         }
     }
     if (ny > 1) {
         if (j > 0) {
-            Face f = face_y(g, c, gh, k, j - 1, i);
+            Face f = face_y(g, x, xg, k, j - 1, i);
             const double C = limited ? face_C(f, Rp[n], Rm[n], Rp[n - nx], Rm[n - nx]) : 1.0;
             if (f.valid) acc += f.tr_hi * C * f.A * f.inv;
         }
         if (j < ny - 1) {
-            Face f = face_y(g, c, gh, k, j, i);
+            Face f = face_y(g, x, xg, k, j, i);
             const double C = limited ? face_C(f, Rp[n + nx], Rm[n + nx], Rp[n], Rm[n]) : 1.0;
             if (f.valid) acc -= f.tr_lo * C * f.A * f.inv;
         }
     }
     const double v = acc * rho_at(g, n);
-    d[n] = base ? base[(size_t)blockIdx.y * bs + n] + v : v;
+    d[n] = base ? base[(size_t)q.v * bs + n] + v : v;
 }
 
 // Frozen gains of L_AD: per x face (nz,ny,nx-1) and y face (nz,ny-1,nx), w_lo and w_hi.
-__global__ void k_fct_gains(FctGrid g, const double* __restrict__ c, const double* __restrict__ Rp,
+__global__ void k_fct_gains(FctGrid g, const double* __restrict__ x, const double* __restrict__ Rp,
                             const double* __restrict__ Rm, double* __restrict__ wx, double* __restrict__ wy) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t nfx = nx > 1 ? (size_t)nz * ny * (nx - 1) : 0;
@@ -251,7 +274,7 @@ __global__ void k_fct_gains(FctGrid g, const double* __restrict__ c, const doubl
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nfx) {
         const int i = (int)(t % (nx - 1)), j = (int)((t / (nx - 1)) % ny), k = (int)(t / ((size_t)(nx - 1) * ny));
-        Face f = face_x(g, c, nullptr, k, j, i);
+        Face f = face_x(g, x, nullptr, k, j, i);
         const size_t lo = ((size_t)k * ny + j) * nx + i;
         const double C = face_C(f, Rp[lo + 1], Rm[lo + 1], Rp[lo], Rm[lo]);
         const double w = f.valid ? C * f.beta * f.inv : 0.0;
@@ -260,7 +283,7 @@ __global__ void k_fct_gains(FctGrid g, const double* __restrict__ c, const doubl
     } else if (t < nfx + nfy) {
         const size_t u = t - nfx;
         const int i = (int)(u % nx), j = (int)((u / nx) % (ny - 1)), k = (int)(u / ((size_t)nx * (ny - 1)));
-        Face f = face_y(g, c, nullptr, k, j, i);
+        Face f = face_y(g, x, nullptr, k, j, i);
         const size_t lo = ((size_t)k * ny + j) * nx + i;
         const double C = face_C(f, Rp[lo + nx], Rm[lo + nx], Rp[lo], Rm[lo]);
         const double w = f.valid ? C * f.beta * f.inv : 0.0;
@@ -397,6 +420,10 @@ struct DevMetrics {
 };
 
 inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+// launch shape of the cell kernels (cell_index): 64 x 4 cells per block
+inline dim3 cell_grid(const FctGrid& g, int nvec) {
+    return dim3((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)(g.nz * nvec));
+}
 
 int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, const double* c,
                     double dt, int limited, const double* ghost, double* d) {
@@ -420,7 +447,8 @@ int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, con
     const double *dc = c, *dg = ghost;
     double* dd = d;
     double* R = dalloc(2 * N);
-    if (!R) rc = fail(ORBIT_ERR_NOMEM, "fct: device allocation failed");
+    double* X = dalloc(ghost ? 2 * N : N);
+    if (!R || !X) rc = fail(ORBIT_ERR_NOMEM, "fct: device allocation failed");
     if (!rc && host) {
         double* a = dalloc(N);
         double* b = ghost ? dalloc(N) : nullptr;
@@ -434,9 +462,13 @@ int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, con
         dd = o;
     }
     if (!rc) {
-        if (limited) hipLaunchKernelGGL(k_fct_R, g1(N), dim3(256), 0, st, M.fg, dc, dg, dt, R, R + N, 0LL, 0LL);
-        hipLaunchKernelGGL(k_fct_div, g1(N), dim3(256), 0, st, M.fg, dc, dg, R, R + N, limited, dd, 0LL, 0LL, 0LL,
-                           (const double*)nullptr, 0LL);
+        double* xg = dg ? X + N : nullptr;
+        hipLaunchKernelGGL(k_fct_x, g1(N), dim3(256), 0, st, M.fg, dc, 0LL, X, 0LL, (int)N);
+        if (dg) hipLaunchKernelGGL(k_fct_x, g1(N), dim3(256), 0, st, M.fg, dg, 0LL, xg, 0LL, (int)N);
+        if (limited)
+            hipLaunchKernelGGL(k_fct_R, cell_grid(M.fg, 1), dim3(256), 0, st, M.fg, X, xg, dt, R, R + N, 0LL, 0LL);
+        hipLaunchKernelGGL(k_fct_div, cell_grid(M.fg, 1), dim3(256), 0, st, M.fg, X, xg, R, R + N, limited, dd, 0LL,
+                           0LL, 0LL, (const double*)nullptr, 0LL);
         if (hipGetLastError() != hipSuccess) rc = fail(ORBIT_ERR_HIP, "fct: kernel launch failed");
     }
     if (!rc && host && hipMemcpyAsync(d, dd, N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -481,7 +513,7 @@ extern "C" int orbit_fct_linop_create(orbit_ctx* ctx, const orbit_grid_metrics* 
               hipMalloc((void**)&L->tmp, 2 * N * 8) == hipSuccess;
     double* cb = nullptr;
     double* R = nullptr;
-    ok = ok && hipMalloc((void**)&cb, N * 8) == hipSuccess && hipMalloc((void**)&R, 2 * N * 8) == hipSuccess;
+    ok = ok && hipMalloc((void**)&cb, 2 * N * 8) == hipSuccess && hipMalloc((void**)&R, 2 * N * 8) == hipSuccess;
     if (!ok) {
         if (cb) hipFree(cb);
         if (R) hipFree(R);
@@ -492,9 +524,13 @@ extern "C" int orbit_fct_linop_create(orbit_ctx* ctx, const orbit_grid_metrics* 
     if (gm->concentration_state)
         hipMemcpyAsync(L->Dz, gm->Dz, N * 8, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(cb, c_base, N * 8, hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(k_fct_R, g1(N), dim3(256), 0, st, M.fg, cb, (const double*)nullptr, dt, R, R + N, 0LL, 0LL);
+    double* xb = cb + N;
+    hipLaunchKernelGGL(k_fct_x, g1(N), dim3(256), 0, st, M.fg, (const double*)cb, 0LL, xb, 0LL, (int)N);
+    hipLaunchKernelGGL(k_fct_R, cell_grid(M.fg, 1), dim3(256), 0, st, M.fg, (const double*)xb,
+                       (const double*)nullptr, dt, R, R + N, 0LL, 0LL);
     if (nfx + nfy > 0)
-        hipLaunchKernelGGL(k_fct_gains, g1(nfx + nfy), dim3(256), 0, st, M.fg, cb, R, R + N, L->wx, L->wy);
+        hipLaunchKernelGGL(k_fct_gains, g1(nfx + nfy), dim3(256), 0, st, M.fg, (const double*)xb, R, R + N, L->wx,
+                           L->wy);
     hipError_t e = hipGetLastError();
     hipStreamSynchronize(st);
     hipFree(cb);
@@ -717,6 +753,7 @@ struct orbit_fct_grid {
     std::vector<void*> bufs;
     FctGrid fg{};
     double* R = nullptr;   // 2 * N * cap scratch for R+ and R-
+    double* X = nullptr;   // 2 * N * cap scratch for x and the ghost's x
     int cap = 0;
 };
 
@@ -748,6 +785,7 @@ extern "C" int orbit_fct_grid_destroy(orbit_fct_grid* G) {
     }
     for (void* p : G->bufs) hipFree(p);
     if (G->R) hipFree(G->R);
+    if (G->X) hipFree(G->X);
     delete G;
     return ORBIT_OK;
 }
@@ -758,25 +796,35 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
                                           long long out_stride) {
     if (!ctx || !G || !c_dev || !out_dev || nvec < 1) return fail(ORBIT_ERR_INVALID, "fct_source_batch: bad argument");
     if (!(dt > 0.0)) return fail(ORBIT_ERR_INVALID, "fct_source_batch: dt must be positive");
-    if (nvec > 65535) return fail(ORBIT_ERR_INVALID, "fct_source_batch: nvec > 65535");
+    if (nvec > 65535 || (long long)nvec * G->fg.nz > 65535)
+        return fail(ORBIT_ERR_INVALID, "fct_source_batch: nvec * nz > 65535");
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     const size_t N = (size_t)G->fg.nz * G->fg.ny * G->fg.nx;
     if (G->cap < nvec) {
-        if (G->R) {
+        if (G->R || G->X) {
             hipStreamSynchronize(st);
-            hipFree(G->R);
-            G->R = nullptr;
+            if (G->R) hipFree(G->R);
+            if (G->X) hipFree(G->X);
+            G->R = G->X = nullptr;
+            G->cap = 0;
         }
         ORBIT_HIP_CHECK(hipMalloc((void**)&G->R, 2 * N * (size_t)nvec * 8));
+        ORBIT_HIP_CHECK(hipMalloc((void**)&G->X, 2 * N * (size_t)nvec * 8));
         G->cap = nvec;
     }
-    dim3 grid(g1(N).x, nvec);
+    double* x = G->X;
+    double* xg = ghost_dev ? G->X + N * nvec : nullptr;
+    dim3 gx(g1(N).x, nvec);
+    hipLaunchKernelGGL(k_fct_x, gx, dim3(256), 0, st, G->fg, c_dev, c_stride, x, (long long)N, (int)N);
+    if (ghost_dev)
+        hipLaunchKernelGGL(k_fct_x, gx, dim3(256), 0, st, G->fg, ghost_dev, c_stride, xg, (long long)N, (int)N);
+    const dim3 grid = cell_grid(G->fg, nvec);
     if (limited)
-        hipLaunchKernelGGL(k_fct_R, grid, dim3(256), 0, st, G->fg, c_dev, ghost_dev, dt, G->R, G->R + N * nvec,
-                           c_stride, (long long)N);
-    hipLaunchKernelGGL(k_fct_div, grid, dim3(256), 0, st, G->fg, c_dev, ghost_dev, G->R, G->R + N * nvec, limited,
-                       out_dev, c_stride, (long long)N, out_stride, base_dev, base_stride);
+        hipLaunchKernelGGL(k_fct_R, grid, dim3(256), 0, st, G->fg, (const double*)x, (const double*)xg, dt, G->R,
+                           G->R + N * nvec, (long long)N, (long long)N);
+    hipLaunchKernelGGL(k_fct_div, grid, dim3(256), 0, st, G->fg, (const double*)x, (const double*)xg, G->R,
+                       G->R + N * nvec, limited, out_dev, (long long)N, (long long)N, out_stride, base_dev, base_stride);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
@@ -799,7 +847,7 @@ extern "C" int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, con
     bool ok = hipMalloc((void**)&L->wx, std::max<size_t>(2 * nfx, 1) * 8) == hipSuccess &&
               hipMalloc((void**)&L->wy, std::max<size_t>(2 * nfy, 1) * 8) == hipSuccess &&
               hipMalloc((void**)&L->dP, N * 8) == hipSuccess && hipMalloc((void**)&L->Dz, N * 8) == hipSuccess &&
-              hipMalloc((void**)&L->tmp, 2 * N * 8) == hipSuccess && hipMalloc((void**)&R, 2 * N * 8) == hipSuccess;
+              hipMalloc((void**)&L->tmp, 2 * N * 8) == hipSuccess && hipMalloc((void**)&R, 3 * N * 8) == hipSuccess;
     if (!ok) {
         if (R) hipFree(R);
         orbit_fct_linop_destroy(L);
@@ -807,9 +855,13 @@ extern "C" int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, con
     }
     hipMemcpyAsync(L->dP, fg.dP, N * 8, hipMemcpyDeviceToDevice, st);
     if (fg.conc && fg.Dz) hipMemcpyAsync(L->Dz, fg.Dz, N * 8, hipMemcpyDeviceToDevice, st);
-    hipLaunchKernelGGL(k_fct_R, g1(N), dim3(256), 0, st, fg, c_base_dev, (const double*)nullptr, dt, R, R + N, 0LL, 0LL);
+    double* xb = R + 2 * N;
+    hipLaunchKernelGGL(k_fct_x, g1(N), dim3(256), 0, st, fg, c_base_dev, 0LL, xb, 0LL, (int)N);
+    hipLaunchKernelGGL(k_fct_R, cell_grid(fg, 1), dim3(256), 0, st, fg, (const double*)xb, (const double*)nullptr, dt,
+                       R, R + N, 0LL, 0LL);
     if (nfx + nfy > 0)
-        hipLaunchKernelGGL(k_fct_gains, g1(nfx + nfy), dim3(256), 0, st, fg, c_base_dev, R, R + N, L->wx, L->wy);
+        hipLaunchKernelGGL(k_fct_gains, g1(nfx + nfy), dim3(256), 0, st, fg, (const double*)xb, R, R + N, L->wx,
+                           L->wy);
     hipError_t e = hipGetLastError();
     hipStreamSynchronize(st);
     hipFree(R);
