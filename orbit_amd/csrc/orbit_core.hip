@@ -510,15 +510,25 @@ __global__ __launch_bounds__(RED_THREADS) void k_mdot(const double* __restrict__
 
 // coef[slot*MAXM1 + i] = sum over blocks of partials (fixed order)
 constexpr int MAXM1 = MAXM + 1;
-__global__ void k_finalize(const double* __restrict__ partials, int nvec, int nb,
-                           double* __restrict__ out, int nslots) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nslots * nvec) return;
-    int slot = g / nvec, i = g - slot * nvec;
-    const double* p = partials + ((size_t)slot * nvec + i) * nb;
+// Sum of nb partials by one wavefront: lane l adds partials l, l+64, ... in
+// order, then a fixed butterfly over the lanes (the result is in lane 0).  The
+// order depends only on nb, so every slot's sum is deterministic.
+__device__ __forceinline__ double wave_sum_partials(const double* __restrict__ p, int nb) {
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += p[b];
-    out[(size_t)slot * MAXM1 + i] = s;
+    for (int b = threadIdx.x & 63; b < nb; b += 64) s += p[b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    return s;
+}
+// blocks of up to 4 waves, one wave per output value
+inline dim3 grid_waves(size_t n) { return dim3((unsigned)((n + 3) / 4)); }
+
+__global__ __launch_bounds__(256) void k_finalize(const double* __restrict__ partials, int nvec, int nb,
+                                                  double* __restrict__ out, int nslots) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= nslots * nvec) return;  // uniform over the wave
+    const int slot = w / nvec, i = w - slot * nvec;
+    const double s = wave_sum_partials(partials + ((size_t)slot * nvec + i) * nb, nb);
+    if ((threadIdx.x & 63) == 0) out[(size_t)slot * MAXM1 + i] = s;
 }
 
 // w -= sum_i coef_i V_i ; optionally partials of ||w||^2
@@ -584,13 +594,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_negcount(const double* __restri
     if (threadIdx.x == 0) partials[(size_t)slot * nb + blk] = s;
 }
 
-__global__ void k_sum_partials(const double* __restrict__ partials, int nb, double* __restrict__ out,
-                               int nslots) {
-    int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= nslots) return;
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += partials[(size_t)slot * nb + b];
-    out[slot] = s;
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int nb,
+                                                      double* __restrict__ out, int nslots) {
+    const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (slot >= nslots) return;  // uniform over the wave
+    const double s = wave_sum_partials(partials + (size_t)slot * nb, nb);
+    if ((threadIdx.x & 63) == 0) out[slot] = s;
 }
 
 // ------------------------------------------------------------------------- //
@@ -1272,7 +1281,7 @@ int norms(Solver& S, const double* x, long long xslot, int n, std::vector<double
     hipStream_t st = S.B.ctx->stream;
     const int nb = (n + RED_CHUNK - 1) / RED_CHUNK;
     hipLaunchKernelGGL(k_norm2, dim3(nb, S.B.nslots), dim3(RED_THREADS), 0, st, x, xslot, n, nb, S.partials);
-    hipLaunchKernelGGL(k_sum_partials, grid1(S.B.nslots), dim3(256), 0, st, S.partials, nb, S.scal, S.B.nslots);
+    hipLaunchKernelGGL(k_sum_partials, grid_waves(S.B.nslots), dim3(256), 0, st, S.partials, nb, S.scal, S.B.nslots);
     ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.scal, S.B.nslots * 8, hipMemcpyDeviceToHost, st));
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
     out.assign(S.host, S.host + S.B.nslots);
@@ -1457,7 +1466,7 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     { int rc0 = set_ints(st, S.B.active_dev, act.data(), ns); if (rc0) return rc0; }
     // beta = ||r||, V0 = r / beta
     hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.r, vs, (int)n16, nb, S.partials);
-    hipLaunchKernelGGL(k_sum_partials, grid1(ns), dim3(256), 0, st, S.partials, nb, S.scal, ns);
+    hipLaunchKernelGGL(k_sum_partials, grid_waves(ns), dim3(256), 0, st, S.partials, nb, S.scal, ns);
     hipLaunchKernelGGL(k_gmres_start, grid1(ns), dim3(256), 0, st, S.gst, S.scal, ns, S.B.active_dev);
     hipLaunchKernelGGL(k_scale_by, g, dim3(256), 0, st, S.r, vs, S.Vk, kslot, S.gst, 0, (int)n16, S.B.active_dev);
     steps.assign(ns, 0);
@@ -1468,16 +1477,16 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
         // CGS2
         hipLaunchKernelGGL(k_mdot, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.w, vs,
                            (int)n16, nb, S.partials, S.B.active_dev);
-        hipLaunchKernelGGL(k_finalize, grid1((size_t)ns * (j + 1)), dim3(256), 0, st, S.partials, j + 1, nb, S.h1, ns);
+        hipLaunchKernelGGL(k_finalize, grid_waves((size_t)ns * (j + 1)), dim3(256), 0, st, S.partials, j + 1, nb, S.h1, ns);
         hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.h1, 1.0,
                            S.w, vs, (int)n16, nb, (double*)nullptr, S.B.active_dev, (const int*)nullptr);
         hipLaunchKernelGGL(k_mdot, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.w, vs,
                            (int)n16, nb, S.partials, S.B.active_dev);
-        hipLaunchKernelGGL(k_finalize, grid1((size_t)ns * (j + 1)), dim3(256), 0, st, S.partials, j + 1, nb, S.h2, ns);
+        hipLaunchKernelGGL(k_finalize, grid_waves((size_t)ns * (j + 1)), dim3(256), 0, st, S.partials, j + 1, nb, S.h2, ns);
         double* np = S.partials + (size_t)ns * MAXM1 * nb;
         hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.h2, 1.0,
                            S.w, vs, (int)n16, nb, np, S.B.active_dev, (const int*)nullptr);
-        hipLaunchKernelGGL(k_sum_partials, grid1(ns), dim3(256), 0, st, np, nb, S.scal, ns);
+        hipLaunchKernelGGL(k_sum_partials, grid_waves(ns), dim3(256), 0, st, np, nb, S.scal, ns);
         hipLaunchKernelGGL(k_gmres_column, grid1(ns), dim3(256), 0, st, S.gst, S.h1, S.h2, S.scal, j, ns,
                            S.B.active_dev, S.resid);
         hipLaunchKernelGGL(k_scale_by, g, dim3(256), 0, st, S.w, vs, S.Vk + (size_t)(j + 1) * n16, kslot,
@@ -1775,7 +1784,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         const int n = (int)((nbn + 1) * n16);
         const int nb = (n + RED_CHUNK - 1) / RED_CHUNK;
         hipLaunchKernelGGL(k_negcount, dim3(nb, nt), dim3(RED_THREADS), 0, st, S.orbit, os, n, nb, S.partials);
-        hipLaunchKernelGGL(k_sum_partials, grid1(nt), dim3(256), 0, st, S.partials, nb, S.scal, nt);
+        hipLaunchKernelGGL(k_sum_partials, grid_waves(nt), dim3(256), 0, st, S.partials, nb, S.scal, nt);
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.scal, nt * 8, hipMemcpyDeviceToHost, st));
     }
     hipLaunchKernelGGL(k_int_to_ext, dim3(grid1(op->N).x, nt * (nbn + 1)), dim3(256), 0, st, S.orbit, os, out_dev,
