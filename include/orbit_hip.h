@@ -240,6 +240,34 @@ int orbit_opset_check(orbit_ctx* ctx, orbit_opset* op, long long* counts);
 int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, int precision, int nslots,
                           int reps, double* ms_per_launch, double* bytes_per_launch);
 
+/* Launch shape of the red-black half-sweep (the column SpMV + block apply of
+ * every inner solve): slots_per_wave 1 (a wave is 4 columns of one tracer slot)
+ * or 4 (one column x 4 slots sharing the column's block loads), threads per
+ * workgroup 64, 128 or 256 (the column tile), xcd_map 1 (each of the 8 XCDs
+ * takes a contiguous range of column tiles) or 0 (linear block order).
+ * Every shape computes the same per-lane arithmetic: results are bit-identical
+ * whatever the shape.  Tables are [3][5]: rows FP64 sweeps, accurate mixed
+ * sweeps (FP64 vectors, BF16 blocks), FP32 inner sweeps; columns slot buckets
+ * 1, 2-3, 4-7, 8-15, 16+.  No counterpart in the reference (its factor backend
+ * has no launch shape); this is the autotuner of the batched solves, run once
+ * per sub-domain size class.  Not to be called while a solve on the opset runs. */
+typedef struct {
+    int slots_per_wave;
+    int threads;
+    int xcd_map;
+} orbit_sweep_shape;
+#define ORBIT_N_SWEEP_PRECS 3
+#define ORBIT_N_SLOT_BUCKETS 5
+/* Time every shape on bin 0 (slot counts 1, 2, 4, 8, 16; slot s runs tracer
+ * s mod n_tracers; `reps` launches each) and keep the fastest per precision and
+ * bucket.  table / us_table (nullable) receive the chosen shapes and their
+ * microseconds per launch. */
+int orbit_opset_autotune(orbit_ctx* ctx, orbit_opset* op, int reps, orbit_sweep_shape* table,
+                         double* us_table);
+int orbit_opset_get_shapes(orbit_opset* op, orbit_sweep_shape* table);
+/* Install a table (e.g. one tuned on another opset of the same size class). */
+int orbit_opset_set_shapes(orbit_opset* op, const orbit_sweep_shape* table);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,8 +33,11 @@ EXPORTS = [
     "orbit_fct_linop_csr", "orbit_fct_linop_destroy", "orbit_bench_halfsweep",
     "orbit_fct_grid_create", "orbit_fct_grid_destroy", "orbit_fct_source_batch_dev",
     "orbit_fct_linop_create_dev", "orbit_opset_check", "orbit_spmat_create",
-    "orbit_spmat_apply_dev", "orbit_spmat_destroy",
+    "orbit_spmat_apply_dev", "orbit_spmat_destroy", "orbit_opset_autotune",
+    "orbit_opset_get_shapes", "orbit_opset_set_shapes",
 ]
+N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
+N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
 
 
 class OrbitLibraryMissing(RuntimeError):
@@ -86,6 +89,12 @@ class GridMetrics(C.Structure):
                 ("dP", C.c_void_p), ("Dz", C.c_void_p)]
 
 
+class SweepShape(C.Structure):
+    _fields_ = [("slots_per_wave", C.c_int), ("threads", C.c_int), ("xcd_map", C.c_int)]
+
+
+ShapeTable = SweepShape * (N_SWEEP_PRECS * N_SLOT_BUCKETS)
+
 _lib = None
 _lock = threading.Lock()
 _P = C.c_void_p
@@ -128,6 +137,9 @@ def _declare(lib):
         "orbit_fct_grid_destroy": (_I, [_P]),
         "orbit_fct_linop_create_dev": (_I, [_P, _P, _P, _D, C.POINTER(_P)]),
         "orbit_opset_check": (_I, [_P, _P, C.POINTER(C.c_longlong)]),
+        "orbit_opset_autotune": (_I, [_P, _P, _I, C.POINTER(SweepShape), C.POINTER(_D)]),
+        "orbit_opset_get_shapes": (_I, [_P, C.POINTER(SweepShape)]),
+        "orbit_opset_set_shapes": (_I, [_P, C.POINTER(SweepShape)]),
         "orbit_fct_source_batch_dev": (_I, [_P, _P, _I, _P, C.c_longlong, _P, _D, _I, _P, C.c_longlong,
                                             _P, C.c_longlong]),
     }

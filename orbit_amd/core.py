@@ -144,6 +144,38 @@ class OperatorSet:
         check(lib().orbit_opset_check(self.ctx.handle, self.handle, c))
         return dict(zip(self.CHECK_FIELDS, list(c)))
 
+    # -- sweep launch shapes (autotuner) ----------------------------------
+    SWEEP_PRECS = ("fp64", "acc_mixed", "f32_inner")
+    SLOT_BUCKETS = ("1", "2-3", "4-7", "8-15", "16+")
+
+    @staticmethod
+    def _table_to_list(t) -> list:
+        return [(e.slots_per_wave, e.threads, e.xcd_map) for e in t]
+
+    def autotune(self, reps: int = 40):
+        """Time every half-sweep launch shape per sweep precision and slot bucket on
+        this operator set and keep the fastest.  Returns (table, us): table is a
+        flat list of (slots_per_wave, threads, xcd_map), 3 precisions x 5 buckets,
+        us the microseconds per launch of each chosen shape."""
+        t = _lib.ShapeTable()
+        us = (C.c_double * len(t))()
+        check(lib().orbit_opset_autotune(self.ctx.handle, self.handle, int(reps), t, us))
+        return self._table_to_list(t), list(us)
+
+    def get_shapes(self) -> list:
+        t = _lib.ShapeTable()
+        check(lib().orbit_opset_get_shapes(self.handle, t))
+        return self._table_to_list(t)
+
+    def set_shapes(self, table) -> None:
+        """Install a shape table (as returned by autotune/get_shapes), e.g. one tuned
+        on another operator set of the same size class."""
+        table = list(table)
+        if len(table) != _lib.N_SWEEP_PRECS * _lib.N_SLOT_BUCKETS:
+            raise ValueError("a shape table has 15 entries (3 precisions x 5 slot buckets)")
+        t = _lib.ShapeTable(*[_lib.SweepShape(*map(int, e)) for e in table])
+        check(lib().orbit_opset_set_shapes(self.handle, t))
+
     def update_diag(self, D: np.ndarray):
         D = np.ascontiguousarray(D, dtype=np.float64)
         check(lib().orbit_opset_update_diag(self.ctx.handle, self.handle, ptr(D)))

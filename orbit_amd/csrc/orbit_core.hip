@@ -204,7 +204,8 @@ struct SweepArgs {
     int own_zero;       // x of this colour is known to be zero (not read)
     int nbr_zero;       // x of the other colour is known to be zero (not read)
     int nslots;
-    int tiles_per_xcd;  // column tiles given to each of the 8 XCDs
+    int tiles_per_xcd;  // column tiles given to each of the 8 XCDs (xcd_map)
+    int xcd_map;        // 1: XCD-contiguous tile ranges; 0: linear block order
     const int* counts;  // nullable: per-slot sweep count; slots past theirs skip
     int sweep_idx;
     int colour;         // 0 red ((i + j) even), 1 black
@@ -219,6 +220,12 @@ __device__ __forceinline__ void xcd_tile_slot(int L, int nslots, int tiles_per_x
     const int lt = r / nslots;
     slot = r - lt * nslots;
     tile = xcd * tiles_per_xcd + lt;
+}
+
+// Linear order: consecutive blocks take the slot groups of one tile.
+__device__ __forceinline__ void linear_tile_slot(int L, int nslots, int& tile, int& slot) {
+    tile = L / nslots;
+    slot = L - tile * nslots;
 }
 
 // Column id of the idx-th column of colour c in row-major order.  Rows 2p and
@@ -246,10 +253,12 @@ __device__ __forceinline__ void wave_sync() {
 
 // One colour's half-sweep: x_c += P_c (b - A x)_c for every column c of the colour
 // and every tracer slot.  Lane k of a 16-lane group owns level k of one column of
-// one slot.  CS = 0: a workgroup is 16 columns of one slot; CS = 1: 4 columns x 4
-// slots, so each wave reads the shared column block and stencil of one column
-// once for four slots.  OZ / NZ: x of this / the other colour is known to be zero
-// and is not read.
+// one slot.  CS = 0: a workgroup of W threads is W/16 columns of one slot; CS = 1:
+// W/64 columns x 4 slots, so each wave reads the shared column block and stencil
+// of one column once for four slots.  W (64, 128 or 256) and the block order
+// (XCD-contiguous or linear) are launch choices; LDS is sized by the launch
+// (2*W values of CT, plus W/64 staged blocks of AT for CS = 1).  OZ / NZ: x of
+// this / the other colour is known to be zero and is not read.
 // Every load is in bounds and issued up front (no early exits, no divergent
 // branches around loads): missing edge neighbours read this column's x and are
 // replaced by zero with a select, so stale values there never enter; inactive
@@ -257,21 +266,25 @@ __device__ __forceinline__ void wave_sync() {
 // does not depend on CS, so a slot's result is the same in any batch.
 template <class XT, class AT, class PT, class CT, int CS, bool OZ, bool NZ>
 __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
-    __shared__ CT sx[256];
-    __shared__ CT sr[256];
-    __shared__ AT sV[CS ? 4 * 256 : 1];
+    extern __shared__ __align__(16) unsigned char smem[];
+    AT* sV = (AT*)smem;  // CS = 1 only: blockDim.x / 64 blocks of 256
+    CT* sx = (CT*)(smem + (CS ? (blockDim.x >> 6) * 256 * sizeof(AT) : 0));
+    CT* sr = sx + blockDim.x;
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int nsg = CS ? (a.nslots + 3) >> 2 : a.nslots;
+    const int tc = CS ? (int)(blockDim.x >> 6) : (int)(blockDim.x >> 4);  // columns per tile
     int tile, sg, slot, idx;
+    if (a.xcd_map)
+        xcd_tile_slot(blockIdx.x, nsg, a.tiles_per_xcd, tile, sg);
+    else
+        linear_tile_slot(blockIdx.x, nsg, tile, sg);
+    if (tile * tc >= a.ncols) return;  // uniform over the block
     if (CS) {
-        xcd_tile_slot(blockIdx.x, (a.nslots + 3) >> 2, a.tiles_per_xcd, tile, sg);
-        if (tile * 4 >= a.ncols) return;  // uniform over the block
         slot = sg * 4 + (grp & 3);
-        idx = tile * 4 + (grp >> 2);
+        idx = tile * tc + (grp >> 2);
     } else {
-        xcd_tile_slot(blockIdx.x, a.nslots, a.tiles_per_xcd, tile, sg);
-        if (tile * 16 >= a.ncols) return;  // uniform over the block
         slot = sg;
-        idx = tile * 16 + grp;
+        idx = tile * tc + grp;
     }
     const bool valid = idx < a.ncols && slot < a.nslots;
     const int ls = slot < a.nslots ? slot : a.nslots - 1;
@@ -829,6 +842,19 @@ int upload(hipStream_t st, void* dst, const void* src, size_t bytes) {
     return ORBIT_OK;
 }
 
+// Default sweep launch shapes: reduced-precision sweeps of four or more slots
+// share each column's loads over four slots of a wave (the shared block needs a
+// single transport group); 256 threads, XCD-contiguous tiles.
+void default_shapes(orbit_opset* op) {
+    for (int p = 0; p < N_SWEEP_PRECS; ++p)
+        for (int b = 0; b < N_SLOT_BUCKETS; ++b) {
+            SweepShape& sh = op->shapes[p][b];
+            sh.cs = (p != 0 /* FP64 */ && b >= 2 && op->ngroups == 1) ? 1 : 0;
+            sh.threads = 256;
+            sh.xcd = 1;
+        }
+}
+
 int finish_opset(orbit_ctx* ctx, orbit_opset* op, std::vector<HostBin>& hb) {
     hipStream_t st = ctx->stream;
     const int ncol = op->ncol, n16 = op->n16, G = op->ngroups, T = op->ntr;
@@ -883,6 +909,7 @@ int finish_opset(orbit_ctx* ctx, orbit_opset* op, std::vector<HostBin>& hb) {
     // to its column arithmetically
     op->ncols_c[0] = (ncol + 1) / 2;
     op->ncols_c[1] = ncol / 2;
+    default_shapes(op);
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
     return calibrate_opset(ctx, op);
 }
@@ -1095,19 +1122,26 @@ struct Batch {
 };
 
 template <class XT, class AT, class PT, class CT>
-int launch_sweep_kernel(const SweepArgs& a, int cs, bool own_zero, bool nbr_zero, dim3 grid, dim3 block,
-                        hipStream_t st) {
+int launch_sweep_kernel(SweepArgs a, const SweepShape& sh, bool own_zero, bool nbr_zero, hipStream_t st) {
+    const int cs = sh.cs, W = sh.threads;
+    const int tc = cs ? W / 64 : W / 16;  // columns per tile
+    const int ntiles = (a.ncols + tc - 1) / tc;
+    const int nsg = cs ? (a.nslots + 3) / 4 : a.nslots;
+    a.xcd_map = sh.xcd;
+    a.tiles_per_xcd = (ntiles + 7) / 8;
+    const dim3 grid(sh.xcd ? 8 * a.tiles_per_xcd * nsg : ntiles * nsg), block(W);
+    const size_t lds = (cs ? (size_t)(W / 64) * 256 * sizeof(AT) : 0) + 2 * (size_t)W * sizeof(CT);
     // (own_zero, nbr_zero) = (true, false) is the second colour of a cold first sweep
     if (cs) {
-        if (own_zero && nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, true, true>), grid, block, 0, st, a);
-        else if (own_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, true, false>), grid, block, 0, st, a);
-        else if (nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, false, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, false, false>), grid, block, 0, st, a);
+        if (own_zero && nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, true, true>), grid, block, lds, st, a);
+        else if (own_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, true, false>), grid, block, lds, st, a);
+        else if (nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, false, true>), grid, block, lds, st, a);
+        else hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, false, false>), grid, block, lds, st, a);
     } else {
-        if (own_zero && nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, true, true>), grid, block, 0, st, a);
-        else if (own_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, true, false>), grid, block, 0, st, a);
-        else if (nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, false, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, false, false>), grid, block, 0, st, a);
+        if (own_zero && nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, true, true>), grid, block, lds, st, a);
+        else if (own_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, true, false>), grid, block, lds, st, a);
+        else if (nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, false, true>), grid, block, lds, st, a);
+        else hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, false, false>), grid, block, lds, st, a);
     }
     return ORBIT_OK;
 }
@@ -1139,27 +1173,22 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     a.colour = colour;
     a.nslots = B.nslots;
     const bool T = dir == ADJ;
-    // Reduced-precision sweeps of four or more slots share each column's loads
-    // over four slots of a wave (CS = 1).
-    const int cs = (prec != SP_F64 && B.nslots >= 4 && op->ngroups == 1) ? 1 : 0;
-    const int ntiles = cs ? (a.ncols + 3) / 4 : (a.ncols + 15) / 16;
-    a.tiles_per_xcd = (ntiles + 7) / 8;
-    dim3 grid(8 * a.tiles_per_xcd * (cs ? (B.nslots + 3) / 4 : B.nslots)), block(256);
+    // launch shape of this precision and slot count (the four-slot mapping reads
+    // one shared column block, so it needs a single transport group)
+    SweepShape sh = op->shapes[prec][slot_bucket(B.nslots)];
+    if (op->ngroups != 1) sh.cs = 0;
     if (prec == SP_F64) {
         a.V = T ? O.VT64 : O.V64; a.Vt = T ? O.V64 : O.VT64;
         a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT64 : O.P64;
-        return launch_sweep_kernel<double, double, double, double>(a, cs, own_zero, nbr_zero, grid, block,
-                                                                   B.ctx->stream);
+        return launch_sweep_kernel<double, double, double, double>(a, sh, own_zero, nbr_zero, B.ctx->stream);
     } else if (prec == SP_ACC_MIXED) {
         a.V = T ? O.VT64 : O.V64; a.Vt = T ? O.V64 : O.VT64;
         a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT16 : O.P16;
-        return launch_sweep_kernel<double, double, uint16_t, double>(a, cs, own_zero, nbr_zero, grid, block,
-                                                                     B.ctx->stream);
+        return launch_sweep_kernel<double, double, uint16_t, double>(a, sh, own_zero, nbr_zero, B.ctx->stream);
     }
     a.V = T ? O.VT32 : O.V32; a.Vt = T ? O.V32 : O.VT32;
     a.H = T ? O.HT32 : O.H32; a.d = O.d32; a.P = T ? O.PT16 : O.P16;
-    return launch_sweep_kernel<float, float, uint16_t, float>(a, cs, own_zero, nbr_zero, grid, block,
-                                                              B.ctx->stream);
+    return launch_sweep_kernel<float, float, uint16_t, float>(a, sh, own_zero, nbr_zero, B.ctx->stream);
 }
 
 // x = P_bin(rhs) approximately: `sweeps` red-black sweeps starting from x = 0.
@@ -1987,6 +2016,131 @@ extern "C" int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, i
     return ORBIT_OK;
 }
 
+
+namespace {
+
+bool shape_ok(const orbit_sweep_shape& t) {
+    return (t.slots_per_wave == 1 || t.slots_per_wave == 4) &&
+           (t.threads == 64 || t.threads == 128 || t.threads == 256) && (t.xcd_map == 0 || t.xcd_map == 1);
+}
+
+void drop_graphs(orbit_opset* op) {
+    std::lock_guard<std::mutex> lock(op->graph_mu);
+    for (auto& kv : op->graphs) hipGraphExecDestroy(kv.second);
+    op->graphs.clear();
+}
+
+}  // namespace
+
+extern "C" int orbit_opset_get_shapes(orbit_opset* op, orbit_sweep_shape* table) {
+    if (!op || !table) return fail(ORBIT_ERR_INVALID, "get_shapes: NULL argument");
+    for (int p = 0; p < N_SWEEP_PRECS; ++p)
+        for (int b = 0; b < N_SLOT_BUCKETS; ++b) {
+            const SweepShape& sh = op->shapes[p][b];
+            table[p * N_SLOT_BUCKETS + b] = {sh.cs ? 4 : 1, sh.threads, sh.xcd};
+        }
+    return ORBIT_OK;
+}
+
+extern "C" int orbit_opset_set_shapes(orbit_opset* op, const orbit_sweep_shape* table) {
+    if (!op || !table) return fail(ORBIT_ERR_INVALID, "set_shapes: NULL argument");
+    for (int q = 0; q < N_SWEEP_PRECS * N_SLOT_BUCKETS; ++q)
+        if (!shape_ok(table[q]))
+            return fail(ORBIT_ERR_INVALID, "set_shapes: slots_per_wave must be 1 or 4, threads 64, 128 or 256, "
+                                           "xcd_map 0 or 1");
+    for (int p = 0; p < N_SWEEP_PRECS; ++p)
+        for (int b = 0; b < N_SLOT_BUCKETS; ++b) {
+            const orbit_sweep_shape& t = table[p * N_SLOT_BUCKETS + b];
+            SweepShape& sh = op->shapes[p][b];
+            sh.cs = (int8_t)(t.slots_per_wave == 4 && op->ngroups == 1);
+            sh.threads = (int16_t)t.threads;
+            sh.xcd = (int8_t)t.xcd_map;
+        }
+    drop_graphs(op);  // captured sweeps carry the old launch shapes
+    return ORBIT_OK;
+}
+
+// Time every launch shape of the half-sweep for each sweep precision and slot
+// bucket (representative slot counts 1, 2, 4, 8, 16; slot s runs tracer
+// s mod n_tracers) on bin 0 and keep the fastest.  Each candidate: two untimed
+// launches, then `reps` launches alternating colours between HIP events.
+extern "C" int orbit_opset_autotune(orbit_ctx* ctx, orbit_opset* op, int reps, orbit_sweep_shape* table,
+                                    double* us_table) {
+    if (!ctx || !op) return fail(ORBIT_ERR_INVALID, "autotune: NULL argument");
+    if (reps < 1 || reps > 100000) return fail(ORBIT_ERR_INVALID, "autotune: reps must be in [1, 100000]");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const size_t n16 = op->n16;
+    const int slots_of[N_SLOT_BUCKETS] = {1, 2, 4, 8, 16};
+    const int maxs = 16;
+    int rc = ctx->ensure(7, 2 * (size_t)maxs * n16 * 8 + (size_t)maxs * 4 + 256);
+    if (rc) return rc;
+    char* base = (char*)ctx->scratch[7].p;
+    void* b = base;
+    void* x = base + (size_t)maxs * n16 * 8;
+    int* tmap = (int*)(base + 2 * (size_t)maxs * n16 * 8);
+    std::vector<int> tv(maxs);
+    for (int q = 0; q < maxs; ++q) tv[q] = q % op->ntr;
+    { int rc0 = h2d_sync(st, tmap, tv.data(), maxs * 4); if (rc0) return rc0; }
+    ORBIT_HIP_CHECK(hipMemsetAsync(base, 0, 2 * (size_t)maxs * n16 * 8, st));
+    hipEvent_t e0, e1;
+    ORBIT_HIP_CHECK(hipEventCreate(&e0));
+    ORBIT_HIP_CHECK(hipEventCreate(&e1));
+    int err = ORBIT_OK;
+    for (int p = 0; p < N_SWEEP_PRECS && !err; ++p) {
+        const long long vs = (long long)n16;  // slot stride in elements of the precision's type
+        for (int bk = 0; bk < N_SLOT_BUCKETS && !err; ++bk) {
+            const int ns = slots_of[bk];
+            Batch B{ctx, op, ns, tmap, nullptr, nullptr};
+            SweepShape best = op->shapes[p][bk];
+            double best_us = 1e300;
+            for (int cs = 0; cs <= 1 && !err; ++cs) {
+                if (cs && (op->ngroups != 1 || ns < 2)) continue;
+                for (int W : {64, 128, 256}) {
+                    for (int xcd = 0; xcd <= 1; ++xcd) {
+                        SweepShape& sh = op->shapes[p][bk];
+                        sh.cs = (int8_t)cs;
+                        sh.threads = (int16_t)W;
+                        sh.xcd = (int8_t)xcd;
+                        for (int w = 0; w < 2 && !err; ++w)
+                            err = launch_halfsweep(B, 0, FWD, p, w & 1, b, vs, x, vs, false, false, nullptr);
+                        if (err) break;
+                        hipEventRecord(e0, st);
+                        for (int r = 0; r < reps && !err; ++r)
+                            err = launch_halfsweep(B, 0, FWD, p, r & 1, b, vs, x, vs, false, false, nullptr);
+                        hipEventRecord(e1, st);
+                        hipError_t he = hipEventSynchronize(e1);
+                        if (he != hipSuccess) {
+                            err = fail(ORBIT_ERR_HIP, std::string("autotune: ") + hipGetErrorString(he));
+                            break;
+                        }
+                        float ms = 0.f;
+                        hipEventElapsedTime(&ms, e0, e1);
+                        const double us = 1e3 * (double)ms / reps;
+                        if (us < best_us) {
+                            best_us = us;
+                            best = sh;
+                        }
+                    }
+                    if (err) break;
+                }
+            }
+            op->shapes[p][bk] = best;
+            if (table) table[p * N_SLOT_BUCKETS + bk] = {best.cs ? 4 : 1, best.threads, best.xcd};
+            if (us_table) us_table[p * N_SLOT_BUCKETS + bk] = best_us;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (err) {
+        default_shapes(op);
+        drop_graphs(op);
+        return err;
+    }
+    ORBIT_HIP_CHECK(hipGetLastError());
+    drop_graphs(op);
+    return ORBIT_OK;
+}
 
 namespace orbit {
 template <class T>

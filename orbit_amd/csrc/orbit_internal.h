@@ -79,6 +79,21 @@ struct BinOps {
     double* tdiag = nullptr;
 };
 
+// Launch shape of the half-sweep kernel: slots per wave (1 or 4), threads per
+// workgroup (64, 128, 256) and block order (XCD-contiguous or linear).  Chosen
+// per sweep precision and slot-count bucket; every shape computes the same
+// per-lane arithmetic, so results do not depend on it.
+struct SweepShape {
+    int8_t cs = 0;
+    int16_t threads = 256;
+    int8_t xcd = 1;
+};
+constexpr int N_SWEEP_PRECS = 3;   // FP64, accurate mixed (FP64 + BF16 blocks), FP32 inner
+constexpr int N_SLOT_BUCKETS = 5;  // 1, 2-3, 4-7, 8-15, 16+
+inline int slot_bucket(int nslots) {
+    return nslots <= 1 ? 0 : nslots < 4 ? 1 : nslots < 8 ? 2 : nslots < 16 ? 3 : 4;
+}
+
 struct GraphKey {
     int dir, prec, sweeps, nslots;
     std::vector<int> tracers;
@@ -102,6 +117,7 @@ struct Opset {
     int ncols_c[2] = {0, 0};
     std::vector<void*> owned;
     double bytes = 0.0;
+    SweepShape shapes[N_SWEEP_PRECS][N_SLOT_BUCKETS];  // set at creation, by autotune or by the caller
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::mutex graph_mu;  // contexts on several host threads share an opset
     template <class T>

@@ -60,10 +60,49 @@ def slot_bytes(op: OperatorSet, restart: int = 30) -> int:
     return int(doubles * 8 + n16 * 3 * 4 + 64 * 1024)
 
 
+class ShapeTuner:
+    """Autotuner of the sweep launch shape per sub-domain size class.
+
+    The first operator set of a size class (nz, ny, nx) is timed over every
+    half-sweep launch shape (slots per wave, threads per workgroup, XCD
+    mapping) for each sweep precision and slot bucket
+    (``OperatorSet.autotune``); every later set of that class gets the same
+    table (``set_shapes``).  Shapes change timing only: every shape computes
+    the same per-lane arithmetic, so results are bit-identical."""
+
+    def __init__(self, reps: int = 40):
+        self.reps = reps
+        self.tables: Dict[Tuple[int, int, int], list] = {}
+        self.us: Dict[Tuple[int, int, int], list] = {}
+
+    def apply(self, op: OperatorSet) -> list:
+        key = (op.nz, op.ny, op.nx)
+        if key not in self.tables:
+            self.tables[key], self.us[key] = op.autotune(self.reps)
+        else:
+            op.set_shapes(self.tables[key])
+        return self.tables[key]
+
+    def report(self) -> dict:
+        """{"NZxNYxNX": {precision: {bucket: [slots_per_wave, threads, xcd_map, us]}}}"""
+        out = {}
+        for key, tab in self.tables.items():
+            us = self.us[key]
+            d = {}
+            for p, pname in enumerate(OperatorSet.SWEEP_PRECS):
+                d[pname] = {b: list(tab[p * 5 + k]) + [round(us[p * 5 + k], 3)]
+                            for k, b in enumerate(OperatorSet.SLOT_BUCKETS)}
+            out["x".join(map(str, key))] = d
+        return out
+
+
 class OrbitServer:
     def __init__(self, opsets: Dict[Tuple[int, str], OperatorSet], n_workers: int = 4,
                  max_slots: int = 16, hbm_budget: float = 32e9, precision: str = "mixed",
-                 device: int = 0):
+                 device: int = 0, tuner: Optional[ShapeTuner] = None):
+        if tuner is not None:  # before any worker launches on these sets
+            for op in opsets.values():
+                tuner.apply(op)
         self.opsets = opsets
         self.max_slots = max_slots
         self.budget = float(hbm_budget)
@@ -219,12 +258,14 @@ def run_poisson(server: OrbitServer, rate: float, n_queries: int, make, rng) -> 
             "p99_ms": float(np.percentile(lat, 99)), "mean_batch": float(np.mean([q.batch_size for q in queries]))}
 
 
-def solve_many(jobs, n_workers: int = 4, precision: str = "mixed", tol: float = 1e-8):
+def solve_many(jobs, n_workers: int = 4, precision: str = "mixed", tol: float = 1e-8,
+               tuner: Optional[ShapeTuner] = None):
     """Batched uneven orbits: jobs = [(OperatorSet, e (nt, n_bins, N) CUDA, tracers)],
-    run as concurrent batches on n_workers streams.  Returns (results, wall seconds)."""
+    run as concurrent batches on n_workers streams (launch shapes per size class
+    from `tuner` when given).  Returns (results, wall seconds)."""
     opsets = {(i, "job"): j[0] for i, j in enumerate(jobs)}
     srv = OrbitServer(opsets, n_workers=n_workers, max_slots=max(len(j[2]) for j in jobs),
-                      hbm_budget=1e12, precision=precision)
+                      hbm_budget=1e12, precision=precision, tuner=tuner)
     try:
         t0 = time.perf_counter()
         futs = [srv.submit(Query(month=i, size="job", tracers=list(j[2]), mode="forward", payload=j[1], tol=tol))
