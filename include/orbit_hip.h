@@ -163,6 +163,59 @@ int orbit_solve_adjoint_dev(orbit_ctx* ctx, orbit_opset* op, const int* tracers,
                             const double* R_dev, const double* lam_warm_dev,
                             const orbit_opts* opts, double* lam_out_dev, orbit_info* info);
 
+/* Grid fields of one bin for the device-side transport assembly (the GridData
+ * fields read by orbit/core/operator.py:281-353 and the assemblers it calls).
+ * (nz,ny,nx) arrays are k-major; NULL marks an absent optional field. */
+typedef struct {
+    int nz, ny, nx;
+    int split_fluxes;          /* has_split_fluxes: UAvg/VAvg_plus/minus drive advection */
+    int concentration_state;   /* 1: return diag(rho) T diag(1/rho), rho = dP/(g Dz) */
+    int periodic_lon;          /* must be 0: the wrap face is not supported */
+    double dy, dlon, dlat;     /* m, degrees, degrees */
+    const double* lat;         /* (ny,) degrees, cell centres */
+    const double* dx;          /* (ny,) m */
+    const double* dP;          /* Pa */
+    const double* Dz;          /* m */
+    const double* UAvg;        /* west-face wind, m/s */
+    const double* VAvg;        /* south-face wind, m/s */
+    const double* UAvg_plus;   /* split fluxes only */
+    const double* UAvg_minus;
+    const double* VAvg_plus;
+    const double* VAvg_minus;
+    const double* Kxxyy;       /* m^2/s */
+    const double* K_meander_u; /* net fluxes only (face-centred) */
+    const double* K_meander_v;
+    const double* Kzz;         /* m^2/s */
+    const double* omega_edge_plus;   /* (nz+1,ny,nx) interface flux, downward, or NULL */
+    const double* omega_edge_minus;  /* (nz+1,ny,nx) upward */
+    const double* omega_plus;        /* (nz,ny,nx) cell-centred fallback, or NULL */
+    const double* omega_minus;
+    const double* CMFMC;             /* (nz+1,ny,nx) kg/m^2/s, or NULL */
+    const double* U_face_e;          /* (nz,ny) true east outer face, or NULL */
+    const double* U_plus_face_e;
+    const double* U_minus_face_e;
+    const double* V_face_n;          /* (nz,nx) true north outer face, or NULL */
+    const double* V_plus_face_n;
+    const double* V_minus_face_n;
+} orbit_grid_fields;
+
+/* Transport block of one bin assembled on the device from its grid fields
+ * (reference assemble_transport_block, orbit/core/operator.py:281-347: the
+ * Patankar horizontal convection-diffusion, omega advection, Kzz diffusion and
+ * CMFMC convection, then the concentration form).  stencil (host, 7*N doubles)
+ * receives, k-major, the diagonal and the W, E, S, N, down (k-1), up (k+1)
+ * entries of every row (0 where the neighbour does not exist). */
+int orbit_transport_assemble(orbit_ctx* ctx, const orbit_grid_fields* g, double* stencil);
+
+/* Opset of n_bins bins whose shared transport is assembled on the device from
+ * grids[n_bins] and whose per-tracer diagonal D[n_bins][n_tracers][N] (host,
+ * k-major) is added to it: L_{bin,t} = T_bin + diag(D_{bin,t})
+ * (reference assemble_single_species_L, orbit/core/operator.py:356-368), with
+ * no host CSR.  The result is the same opset orbit_opset_create_shared builds
+ * from the CSR of T. */
+int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracers, double dtau,
+                             const orbit_grid_fields* grids, const double* D, orbit_opset** out);
+
 /* Air-mass-balance pre-flight over the shared transport blocks of an opset
  * made with orbit_opset_create_shared (reference orbit/core/operator.py:598-652);
  * rho is (N,).  The caller applies the abort rule of orbit/cli.py:1604-1614. */

@@ -127,6 +127,22 @@ class OperatorSet:
         return cls(ctx, h, nz, ny, nx, nb, nt, dtau)
 
     @classmethod
+    def from_grids(cls, grids: Sequence, D: np.ndarray, dtau: float = DTAU,
+                   ctx: Optional[Context] = None) -> "OperatorSet":
+        """grids[bin]: GridData-like fields of each bin; D: (n_bins, n_tracers, N)
+        per-tracer diagonal.  The transport blocks are assembled on the device
+        (no host CSR); L_{bin,t} = T_bin + diag(D_{bin,t})."""
+        from . import assembly
+        ctx = ctx or default_context()
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        g0 = grids[0]
+        N = int(g0.nz) * int(g0.ny) * int(g0.nx)
+        if D.ndim != 3 or D.shape[0] != len(grids) or D.shape[2] != N:
+            raise ValueError("D must be (n_bins, n_tracers, N)")
+        h = assembly.create_opset_handle(ctx, grids, D, dtau)
+        return cls(ctx, h, int(g0.nz), int(g0.ny), int(g0.nx), len(grids), D.shape[1], dtau)
+
+    @classmethod
     def from_workload(cls, wl, ctx: Optional[Context] = None) -> "OperatorSet":
         return cls.from_shared(wl.T, wl.D, wl.nz, wl.ny, wl.nx, wl.dtau, ctx)
 

@@ -784,6 +784,92 @@ extern "C" void* orbit_ctx_stream(orbit_ctx* c) { return c ? (void*)c->stream : 
 // ------------------------------------------------------------------------- //
 int calibrate_opset(orbit_ctx* ctx, orbit_opset* op);
 
+namespace orbit {
+// Default sweep launch shapes: reduced-precision sweeps of four or more slots
+// share each column's loads over four slots of a wave (the shared block needs a
+// single transport group); 256 threads, XCD-contiguous tiles.
+void default_shapes(orbit_opset* op) {
+    for (int p = 0; p < N_SWEEP_PRECS; ++p)
+        for (int b = 0; b < N_SLOT_BUCKETS; ++b) {
+            SweepShape& sh = op->shapes[p][b];
+            sh.cs = (p != 0 /* FP64 */ && b >= 2 && op->ngroups == 1) ? 1 : 0;
+            sh.threads = 256;
+            sh.xcd = 1;
+        }
+}
+
+}  // namespace orbit
+
+namespace orbit {
+
+int alloc_bins(orbit_opset* op) {
+    const int ncol = op->ncol, n16 = op->n16, G = op->ngroups, T = op->ntr;
+    const size_t nV = (size_t)G * ncol * 256, nH = (size_t)G * ncol * 64, nd = (size_t)T * n16,
+                 nP = (size_t)T * ncol * 256;
+    op->bins.resize(op->nbins);
+    for (int b = 0; b < op->nbins; ++b) {
+        BinOps& B = op->bins[b];
+        B.V64 = op->alloc<double>(nV); B.V32 = op->alloc<float>(nV);
+        B.VT64 = op->alloc<double>(nV); B.VT32 = op->alloc<float>(nV);
+        B.H64 = op->alloc<double>(nH); B.H32 = op->alloc<float>(nH);
+        B.HT64 = op->alloc<double>(nH); B.HT32 = op->alloc<float>(nH);
+        B.d64 = op->alloc<double>(nd); B.d32 = op->alloc<float>(nd);
+        B.P64 = op->alloc<double>(nP); B.P16 = op->alloc<uint16_t>(nP);
+        B.PT64 = op->alloc<double>(nP); B.PT16 = op->alloc<uint16_t>(nP);
+        if (!B.V64 || !B.V32 || !B.VT64 || !B.VT32 || !B.H64 || !B.H32 || !B.HT64 || !B.HT32 ||
+            !B.d64 || !B.d32 || !B.P64 || !B.P16 || !B.PT64 || !B.PT16)
+            return fail(ORBIT_ERR_NOMEM, "opset: device allocation failed");
+        if (op->has_tdiag) {
+            B.tdiag = op->alloc<double>((size_t)n16);
+            if (!B.tdiag) return fail(ORBIT_ERR_NOMEM, "opset: device allocation failed");
+        }
+    }
+    return ORBIT_OK;
+}
+
+// FP32 copies, transposes, factored column blocks (FP64 and BF16) of one bin
+// whose V64, H64 and d64 are in place.
+int derive_bin(orbit_ctx* ctx, orbit_opset* op, int b) {
+    hipStream_t st = ctx->stream;
+    const int ncol = op->ncol, n16 = op->n16, G = op->ngroups, T = op->ntr;
+    const size_t nV = (size_t)G * ncol * 256, nH = (size_t)G * ncol * 64, nd = (size_t)T * n16,
+                 nP = (size_t)T * ncol * 256;
+    BinOps& B = op->bins[b];
+    hipLaunchKernelGGL((k_convert<double, float>), grid1(nV), dim3(256), 0, st, B.V64, B.V32, nV);
+    hipLaunchKernelGGL((k_convert<double, float>), grid1(nH), dim3(256), 0, st, B.H64, B.H32, nH);
+    hipLaunchKernelGGL((k_convert<double, float>), grid1(nd), dim3(256), 0, st, B.d64, B.d32, nd);
+    hipLaunchKernelGGL(k_transpose_blocks<double>, grid1(nV), dim3(256), 0, st, B.V64, B.VT64, (size_t)G * ncol);
+    hipLaunchKernelGGL((k_convert<double, float>), grid1(nV), dim3(256), 0, st, B.VT64, B.VT32, nV);
+    for (int g = 0; g < G; ++g) {
+        hipLaunchKernelGGL(k_transpose_stencil<double>, grid1((size_t)ncol * 16), dim3(256), 0, st,
+                           B.H64 + (size_t)g * ncol * 64, B.HT64 + (size_t)g * ncol * 64, op->nx, op->ny);
+    }
+    hipLaunchKernelGGL((k_convert<double, float>), grid1(nH), dim3(256), 0, st, B.HT64, B.HT32, nH);
+    const long long vstride = G == 1 ? 0 : (long long)ncol * 256;
+    hipLaunchKernelGGL(k_factor, dim3((ncol + 3) / 4, T), dim3(64), 0, st, B.V64, vstride, B.d64, (long long)n16,
+                       B.P64, (long long)ncol * 256, ncol);
+    hipLaunchKernelGGL(k_to_bf16, grid1(nP), dim3(256), 0, st, B.P64, B.P16, nP);
+    hipLaunchKernelGGL(k_transpose_blocks<double>, grid1(nP), dim3(256), 0, st, B.P64, B.PT64, (size_t)T * ncol);
+    hipLaunchKernelGGL(k_to_bf16, grid1(nP), dim3(256), 0, st, B.PT64, B.PT16, nP);
+    ORBIT_HIP_CHECK(hipGetLastError());
+    return ORBIT_OK;
+}
+
+void default_shapes(orbit_opset* op);
+
+// Colour counts, default launch shapes, then the sweep calibration.
+int finalize_opset(orbit_ctx* ctx, orbit_opset* op) {
+    // columns of each colour ((i + j) even / odd); the sweeps map a colour index
+    // to its column arithmetically
+    op->ncols_c[0] = (op->ncol + 1) / 2;
+    op->ncols_c[1] = op->ncol / 2;
+    default_shapes(op);
+    ORBIT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return calibrate_opset(ctx, op);
+}
+
+}  // namespace orbit
+
 namespace {
 
 struct HostBin {
@@ -842,76 +928,25 @@ int upload(hipStream_t st, void* dst, const void* src, size_t bytes) {
     return ORBIT_OK;
 }
 
-// Default sweep launch shapes: reduced-precision sweeps of four or more slots
-// share each column's loads over four slots of a wave (the shared block needs a
-// single transport group); 256 threads, XCD-contiguous tiles.
-void default_shapes(orbit_opset* op) {
-    for (int p = 0; p < N_SWEEP_PRECS; ++p)
-        for (int b = 0; b < N_SLOT_BUCKETS; ++b) {
-            SweepShape& sh = op->shapes[p][b];
-            sh.cs = (p != 0 /* FP64 */ && b >= 2 && op->ngroups == 1) ? 1 : 0;
-            sh.threads = 256;
-            sh.xcd = 1;
-        }
-}
-
 int finish_opset(orbit_ctx* ctx, orbit_opset* op, std::vector<HostBin>& hb) {
     hipStream_t st = ctx->stream;
     const int ncol = op->ncol, n16 = op->n16, G = op->ngroups, T = op->ntr;
-    const size_t nV = (size_t)G * ncol * 256, nH = (size_t)G * ncol * 64, nd = (size_t)T * n16,
-                 nP = (size_t)T * ncol * 256;
-    op->bins.resize(op->nbins);
+    const size_t nV = (size_t)G * ncol * 256, nH = (size_t)G * ncol * 64, nd = (size_t)T * n16;
+    int rc = alloc_bins(op);
+    if (rc) return rc;
     for (int b = 0; b < op->nbins; ++b) {
         BinOps& B = op->bins[b];
-        B.V64 = op->alloc<double>(nV); B.V32 = op->alloc<float>(nV);
-        B.VT64 = op->alloc<double>(nV); B.VT32 = op->alloc<float>(nV);
-        B.H64 = op->alloc<double>(nH); B.H32 = op->alloc<float>(nH);
-        B.HT64 = op->alloc<double>(nH); B.HT32 = op->alloc<float>(nH);
-        B.d64 = op->alloc<double>(nd); B.d32 = op->alloc<float>(nd);
-        B.P64 = op->alloc<double>(nP); B.P16 = op->alloc<uint16_t>(nP);
-        B.PT64 = op->alloc<double>(nP); B.PT16 = op->alloc<uint16_t>(nP);
-        if (!B.V64 || !B.V32 || !B.VT64 || !B.VT32 || !B.H64 || !B.H32 || !B.HT64 || !B.HT32 ||
-            !B.d64 || !B.d32 || !B.P64 || !B.P16 || !B.PT64 || !B.PT16)
-            return fail(ORBIT_ERR_NOMEM, "opset: device allocation failed");
         if (op->has_tdiag) {
-            B.tdiag = op->alloc<double>((size_t)n16);
-            if (!B.tdiag) return fail(ORBIT_ERR_NOMEM, "opset: device allocation failed");
-            int rc = upload(st, B.tdiag, hb[b].tdiag.data(), (size_t)n16 * 8);
+            rc = upload(st, B.tdiag, hb[b].tdiag.data(), (size_t)n16 * 8);
             if (rc) return rc;
         }
-        int rc = upload(st, B.V64, hb[b].V.data(), nV * 8);
+        rc = upload(st, B.V64, hb[b].V.data(), nV * 8);
         if (!rc) rc = upload(st, B.H64, hb[b].H.data(), nH * 8);
         if (!rc) rc = upload(st, B.d64, hb[b].d.data(), nd * 8);
+        if (!rc) rc = derive_bin(ctx, op, b);
         if (rc) return rc;
-        // derived arrays
-        hipLaunchKernelGGL((k_convert<double, float>), grid1(nV), dim3(256), 0, st, B.V64, B.V32, nV);
-        hipLaunchKernelGGL((k_convert<double, float>), grid1(nH), dim3(256), 0, st, B.H64, B.H32, nH);
-        hipLaunchKernelGGL((k_convert<double, float>), grid1(nd), dim3(256), 0, st, B.d64, B.d32, nd);
-        hipLaunchKernelGGL(k_transpose_blocks<double>, grid1(nV), dim3(256), 0, st, B.V64, B.VT64,
-                           (size_t)G * ncol);
-        hipLaunchKernelGGL((k_convert<double, float>), grid1(nV), dim3(256), 0, st, B.VT64, B.VT32, nV);
-        for (int g = 0; g < G; ++g) {
-            hipLaunchKernelGGL(k_transpose_stencil<double>, grid1((size_t)ncol * 16), dim3(256), 0, st,
-                               B.H64 + (size_t)g * ncol * 64, B.HT64 + (size_t)g * ncol * 64, op->nx,
-                               op->ny);
-        }
-        hipLaunchKernelGGL((k_convert<double, float>), grid1(nH), dim3(256), 0, st, B.HT64, B.HT32, nH);
-        const long long vstride = G == 1 ? 0 : (long long)ncol * 256;
-        hipLaunchKernelGGL(k_factor, dim3((ncol + 3) / 4, T), dim3(64), 0, st, B.V64, vstride, B.d64,
-                           (long long)n16, B.P64, (long long)ncol * 256, ncol);
-        hipLaunchKernelGGL(k_to_bf16, grid1(nP), dim3(256), 0, st, B.P64, B.P16, nP);
-        hipLaunchKernelGGL(k_transpose_blocks<double>, grid1(nP), dim3(256), 0, st, B.P64, B.PT64,
-                           (size_t)T * ncol);
-        hipLaunchKernelGGL(k_to_bf16, grid1(nP), dim3(256), 0, st, B.PT64, B.PT16, nP);
-        ORBIT_HIP_CHECK(hipGetLastError());
     }
-    // columns of each colour ((i + j) even / odd); the sweeps map a colour index
-    // to its column arithmetically
-    op->ncols_c[0] = (ncol + 1) / 2;
-    op->ncols_c[1] = ncol / 2;
-    default_shapes(op);
-    ORBIT_HIP_CHECK(hipStreamSynchronize(st));
-    return calibrate_opset(ctx, op);
+    return finalize_opset(ctx, op);
 }
 
 int check_dims(int nz, int ny, int nx, int nbins, int ntr) {

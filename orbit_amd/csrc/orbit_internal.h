@@ -128,3 +128,13 @@ struct Opset {
 
 struct orbit_ctx : orbit::Ctx {};
 struct orbit_opset : orbit::Opset {};
+
+namespace orbit {
+// Opset construction steps shared by the CSR and the grid-field builders:
+// allocate every bin's arrays; derive the FP32 copies, transposes and factored
+// column blocks of a bin whose V64, H64 and d64 are filled; then the colour
+// counts, default launch shapes and sweep calibration.
+int alloc_bins(orbit_opset* op);
+int derive_bin(orbit_ctx* ctx, orbit_opset* op, int b);
+int finalize_opset(orbit_ctx* ctx, orbit_opset* op);
+}  // namespace orbit

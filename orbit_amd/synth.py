@@ -392,3 +392,112 @@ def config_workload(cfg: int, n_tracers: Optional[int] = None, month: int = 0,
         conv = True if convection is None else convection
     size_id = list(SIZE_CLASSES).index(f"{nx}x{ny}")
     return make_workload(nx, ny, nz, nt, seed=[cfg, month, size_id], convection=conv)
+
+
+# --------------------------------------------------------------------------- #
+# GridData-like fields for the device-side transport assembly
+# --------------------------------------------------------------------------- #
+def _mass_divergence(g, U, V, Ue, Vn) -> np.ndarray:
+    """Horizontal air-mass divergence per cell (Pa/s) of the Patankar block's
+    advective part: dP_n (T_hcd 1)_n = sum over faces of the outgoing
+    dP_face u / h, with the y faces weighted by dx_face/dx_cell (the terrain
+    weights of orbit/core/convdiff.py:119-130, 285-298)."""
+    nz, ny, nx = g.nz, g.ny, g.nx
+    dP = g.dP
+    dx3 = np.broadcast_to(g.dx[None, :, None], (nz, ny, nx))
+    Dv = np.zeros((nz, ny, nx))
+    fx = 0.5 * (dP[:, :, :-1] + dP[:, :, 1:]) * U[:, :, 1:] / dx3[:, :, 1:]    # interior x faces
+    Dv[:, :, :-1] += fx
+    Dv[:, :, 1:] -= fx
+    Dv[:, :, 0] -= dP[:, :, 0] * U[:, :, 0] / dx3[:, :, 0]                     # outer faces
+    Dv[:, :, -1] += dP[:, :, -1] * Ue / dx3[:, :, -1]
+    dxf = 0.5 * (dx3[:, :-1, :] + dx3[:, 1:, :])
+    fy = dxf * 0.5 * (dP[:, :-1, :] + dP[:, 1:, :]) * V[:, 1:, :] / g.dy
+    Dv[:, :-1, :] += fy / dx3[:, :-1, :]
+    Dv[:, 1:, :] -= fy / dx3[:, 1:, :]
+    Dv[:, 0, :] -= dP[:, 0, :] * V[:, 0, :] / g.dy
+    Dv[:, -1, :] += dP[:, -1, :] * Vn / g.dy
+    return Dv
+
+
+def make_reference_grids(nx: int, ny: int, nz: int = 15, seed=0, wind_std: float = 5.0,
+                         cmfmc: bool = True, interface_omega: bool = True):
+    """Seeded GridData-like fields of the 8 bins, with the reference's attribute
+    names (orbit/core/grid_data.py:20-240), for the device-side assembly of the
+    transport block: winds on west/south faces plus the true east/north outer
+    faces, cell-centred Kxxyy and Kzz, face-centred meander diffusivities,
+    an interface mass flux omega_edge diagnosed from the horizontal
+    convergence (closed ground, so every interior row of the transport block
+    sums to zero: tests/test_interface_omega_transport.py:117-170 of the
+    reference), and a CMFMC updraft in a third of the columns.
+
+    interface_omega=False gives the pre-2026-09-26 reading instead: a
+    cell-centred omega that does not close the air budget (the air-mass
+    pre-flight flags it)."""
+    import types
+    rng = np.random.default_rng(seed)
+    grids = []
+    dP_base = None
+    for tau in range(N_BINS):
+        b = make_grid(nx, ny, nz, rng, tau, wind_std=wind_std, dP_base=dP_base)
+        if dP_base is None:
+            ps = _box3(_box3(rng.normal(0.0, 1.0, (ny, nx)), 0), 1)
+            ps = 1.0 + 0.05 * ps / max(np.abs(ps).max(), 1e-12)
+            b.dP = b.dP * ps[None]
+            dP_base = b.dP.copy()
+        g = types.SimpleNamespace(nz=nz, ny=ny, nx=nx, dlon=0.625, dlat=0.5)
+        g.lat = 4.0 + (np.arange(ny) + 0.5) * g.dlat
+        g.dx, g.dy, g.Dz, g.dP = b.dx, b.dy, b.Dz, b.dP
+        g.UAvg, g.VAvg = b.UAvg, b.VAvg
+        g.U_face_e, g.V_face_n = b.U_east, b.V_north
+        g.has_split_fluxes = False
+        g.concentration_state = True
+        g.periodic_lon = False
+        g.Kxxyy = 1.0e3 * np.exp(rng.normal(0.0, 0.3, (nz, ny, nx)))
+        g.K_meander_u = 50.0 * rng.random((nz, ny, nx))
+        g.K_meander_v = 50.0 * rng.random((nz, ny, nx))
+        kz = np.concatenate([b.Kzz[:, :, :], b.Kzz[-1:, :, :]], axis=0)   # interfaces -> cells
+        g.Kzz = kz
+        div = _mass_divergence(g, g.UAvg, g.VAvg, g.U_face_e, g.V_face_n)
+        om = np.zeros((nz + 1, ny, nx))
+        om[1:] = np.cumsum(div, axis=0)        # net downward flux through each bottom face
+        if interface_omega:
+            g.omega_edge = om
+            g.omega_edge_plus = np.maximum(om, 0.0)
+            g.omega_edge_minus = np.maximum(-om, 0.0)
+        else:
+            oc = om[:-1] + rng.normal(0.0, 0.5 * np.abs(om).mean() + 1e-6, (nz, ny, nx))
+            g.omega_plus = np.maximum(oc, 0.0)
+            g.omega_minus = np.maximum(-oc, 0.0)
+        if cmfmc:
+            F = np.zeros((nz + 1, ny, nx))
+            active = rng.random((ny, nx)) < 1.0 / 3.0
+            prof = np.sin(np.pi * np.arange(nz + 1) / nz) ** 2        # 0 at ground and top
+            F[:] = 0.02 * prof[:, None, None] * active[None] * rng.random((1, ny, nx))
+            g.CMFMC = F
+        grids.append(g)
+    return grids
+
+
+def grid_workload(nx: int, ny: int, nz: int = 15, n_tracers: int = 1, seed=0, **kw):
+    """Grids of make_reference_grids plus per-tracer diagonals D (n_bins,
+    n_tracers, N) and emissions e (n_tracers, n_bins, N) drawn as in
+    make_workload."""
+    grids = make_reference_grids(nx, ny, nz, seed=seed, **kw)
+    rng = np.random.default_rng([7919, *np.atleast_1d(seed).tolist()])
+    N = nz * ny * nx
+    ncol = nx * ny
+    D = np.zeros((N_BINS, n_tracers, N))
+    e = np.zeros((n_tracers, N_BINS, N))
+    diurnal = 1.0 + 0.5 * np.sin(2.0 * np.pi * np.arange(N_BINS) / N_BINS)
+    emis_diurnal = 1.0 + 0.4 * np.cos(2.0 * np.pi * (np.arange(N_BINS) - 4) / N_BINS)
+    dz0 = grids[0].Dz[0].ravel()
+    for t in range(n_tracers):
+        k = np.exp(rng.uniform(np.log(1e-6), np.log(1e-4), N))
+        vd = rng.uniform(0.001, 0.01, ncol)
+        mag = rng.lognormal(0.0, 1.0, ncol) * (rng.random(ncol) < 0.3) * 1.0e-5
+        for tau in range(N_BINS):
+            D[tau, t] = k * diurnal[tau]
+            D[tau, t, :ncol] += vd / dz0
+            e[t, tau, :ncol] = mag * emis_diurnal[tau]
+    return grids, D, e
