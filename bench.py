@@ -221,6 +221,30 @@ def main():
                           "max_rel_err_lambda0_vs_fp64": max(aerr)}
         log(f"adjoint: mixed {adj_mixed:.1f}/s, plain {adj_plain:.1f}/s, err {max(aerr):.2e}")
 
+        # operator sets built on the device from grid fields (transport assembly,
+        # factors, calibration) against the host-CSR path, and the air-mass pre-flight
+        from orbit_amd import assembly
+        grids_r, D_r, _ = synth.grid_workload(wl.nx, wl.ny, wl.nz, T, seed=[1, 0, 2])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        op_r = OperatorSet.from_grids(grids_r, D_r, ctx=ctx)
+        t_grid = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        mb = assembly.air_mass_preflight(op_r, grids_r)
+        t_mb = time.perf_counter() - t1
+        Ts_r = [assembly.assemble_transport_block(g, ctx) for g in grids_r]
+        t1 = time.perf_counter()
+        op_r2 = OperatorSet.from_shared(Ts_r, D_r, wl.nz, wl.ny, wl.nx, ctx=ctx)
+        t_csr = time.perf_counter() - t1
+        out["grid_assembly"] = {"opset_from_grids_s": round(t_grid, 3), "opset_from_host_csr_s": round(t_csr, 3),
+                                "air_mass_preflight_s": round(t_mb, 3),
+                                "interior_p90_per_day": mb["interior_p90"], "top_p90_per_day": mb["top_p90"],
+                                "shape": f"{wl.nx}x{wl.ny}x{wl.nz}, {T} tracers, 8 bins"}
+        log(f"opset from grids {t_grid:.3f} s (host CSR {t_csr:.3f} s), pre-flight {t_mb:.3f} s, "
+            f"interior p90 {mb['interior_p90']:.2e}/day")
+        op_r.close()
+        op_r2.close()
+
         # config 2: 144 uneven orbits = 12 months x 12 tracers, sizes 1:1:1
         jobs, base_e = [], []
         sizes = ["24x20", "48x40", "64x70"]

@@ -216,6 +216,23 @@ int orbit_transport_assemble(orbit_ctx* ctx, const orbit_grid_fields* g, double*
 int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracers, double dtau,
                              const orbit_grid_fields* grids, const double* D, orbit_opset** out);
 
+/* Regular-grid multilinear lookup table on the device: the 7-D ISORROPIA
+ * partitioning table of the closure (reference IsorropiaLUT.query,
+ * orbit/core/isorropia_lut.py:123-147, i.e. scipy RegularGridInterpolator
+ * with method "linear", bounds_error=False, fill_value=None: linear
+ * extrapolation outside the axes).  Axes strictly ascending, >= 2 values
+ * each, <= 1024 values in all, ndim <= 8; fields are FP32 tables in C order
+ * (host), nfields <= 8. */
+typedef struct orbit_lut orbit_lut;
+int orbit_lut_create(orbit_ctx* ctx, int ndim, const int* axis_len, const double* const* axes, int nfields,
+                     const float* const* fields, orbit_lut** out);
+/* out_dev[f][p] = clamp(interp_f(coords_dev[0][p], ..., coords_dev[ndim-1][p]),
+ * clamp_lo[f], clamp_hi[f]) for p < npts (device arrays of doubles on the
+ * context's stream; clamp arrays are host, nullable = no clamp). */
+int orbit_lut_query_dev(orbit_ctx* ctx, orbit_lut* lut, long long npts, const double* const* coords_dev,
+                        const double* clamp_lo, const double* clamp_hi, double* const* out_dev);
+int orbit_lut_destroy(orbit_lut* lut);
+
 /* Air-mass-balance pre-flight over the shared transport blocks of an opset
  * made with orbit_opset_create_shared (reference orbit/core/operator.py:598-652);
  * rho is (N,).  The caller applies the abort rule of orbit/cli.py:1604-1614. */

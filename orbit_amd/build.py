@@ -14,7 +14,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["orbit_core.hip", "orbit_fct.hip", "orbit_species.hip", "orbit_assemble.hip"]
+SOURCES = ["orbit_core.hip", "orbit_fct.hip", "orbit_species.hip", "orbit_assemble.hip", "orbit_lut.hip"]
 LIB = os.path.join(HERE, "liborbit_hip.so")
 
 

@@ -31,13 +31,16 @@ def make_fct_grids(grids: Sequence, ctx=None) -> List[FctGrid]:
 
 def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, n_passes: int = 6,
                 tol: float = 1e-8, precision: str = "mixed", ghost=None, maxiter: int = 200,
-                warm: str = "c0", **kw):
+                warm: str = "c0", ctx=None, **kw):
     """e: (nt, n_bins, N) CUDA float64.  ghost: optional (nt, n_bins, N) lateral
     inflow concentration per bin.  warm: "orbit" starts each pass from the whole
-    previous orbit (mixed precision), "c0" from its c0 only.
+    previous orbit (mixed precision), "c0" from its c0 only.  ctx: the context
+    (stream) the solves run on, the FCT grids' own (default: the operator set's).
     Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
-    if any(g.ctx is not op.ctx for g in fct_grids):
-        raise ValueError("FCT grids and the operator set must share one context (stream)")
+    ctx = ctx or op.ctx
+    if any(g.ctx is not ctx for g in fct_grids):
+        raise ValueError("FCT grids and the solves must share one context (stream)")
+    kw["ctx"] = ctx
     orbit, info = op.solve_forward(e, tracers, tol=tol, precision=precision, maxiter=maxiter, **kw)
     hist = [info]
     e_prev = e
@@ -46,8 +49,9 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
     for _ in range(1, n_passes):
         e_eff = torch.empty_like(e)
         for tau in range(op.n_bins):
+            # queued on the solves' stream: the next solve reads e_eff in order
             fct_grids[tau].source_batch(orbit[:, tau + 1], e_eff[:, tau], base=e[:, tau], dt=op.dtau,
-                                        ghost=None if ghost is None else ghost[:, tau])
+                                        ghost=None if ghost is None else ghost[:, tau], sync=False)
         if precision == "mixed" and warm == "orbit":
             # The previous orbit starts the first accurate sweep.  Its states are
             # off by at most dtau * sum_tau ||de_tau|| (the propagators contract),

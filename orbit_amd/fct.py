@@ -147,9 +147,11 @@ class FctGrid:
         check(lib().orbit_fct_grid_create(self.ctx.handle, C.byref(g), C.byref(h)))
         self.handle = h
 
-    def source_batch(self, c, out, base=None, dt=10800.0, limited=True, ghost=None):
+    def source_batch(self, c, out, base=None, dt=10800.0, limited=True, ghost=None, sync=True):
         """out[v] = base[v] + d_AD(c[v]) for the leading index v of 2-D views
-        (CUDA float64 tensors whose rows may be strided, e.g. orbit[:, tau+1])."""
+        (CUDA float64 tensors whose rows may be strided, e.g. orbit[:, tau+1]).
+        sync=False leaves the launch queued on the context's stream (for a
+        consumer on that same stream)."""
         nvec = c.shape[0]
         for t in (c, out) + ((base,) if base is not None else ()) + ((ghost,) if ghost is not None else ()):
             if t.stride(-1) != 1 or t.shape[-1] != self.N:
@@ -162,7 +164,8 @@ class FctGrid:
             None if ghost is None else ghost.data_ptr(), float(dt), int(bool(limited)),
             None if base is None else base.data_ptr(), 0 if base is None else base.stride(0),
             out.data_ptr(), out.stride(0)))
-        self.ctx.sync()
+        if sync:
+            self.ctx.sync()
 
     def linop(self, c_base, dt=10800.0):
         """Frozen L_AD at the device state c_base (CUDA float64, length N)."""
