@@ -216,6 +216,48 @@ int orbit_transport_assemble(orbit_ctx* ctx, const orbit_grid_fields* g, double*
 int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracers, double dtau,
                              const orbit_grid_fields* grids, const double* D, orbit_opset** out);
 
+/* Deposition and chemistry fields of one bin for the per-species diagonals
+ * (the GridData fields read by orbit/core/deposition.py:101-362 and the rates
+ * passed to assemble_species_operators, orbit/core/operator.py:371-595).
+ * (nz,ny,nx) arrays k-major; the dry velocities are read on level 0 only;
+ * is_land and F_p_vbs_surface are (ny,nx) / (5,ny,nx); F_p_vbs is (5,nz,ny,nx)
+ * with the VBS bins ordered C01, C1, C10, C100, C1000.  nh_part / no3_part are
+ * the partition fractions already chosen (marginal or equilibrium, with the
+ * reference's fallbacks); NULL marks an absent optional field. */
+typedef struct {
+    const double* particle_dry_dep;
+    const double* SO2_dry_dep;
+    const double* NOx_dry_dep;
+    const double* NH3_dry_dep;
+    const double* VOC_dry_dep;    /* NULL: 0.1 cm/s */
+    const double* HNO3_dry_dep;   /* NULL: 3 cm/s at the surface */
+    const double* O3_dry_dep;     /* NULL: 0.4 cm/s over land, 0.05 over water */
+    const double* is_land;        /* NULL: all land */
+    const double* particle_wet_dep;
+    const double* SO2_wet_dep;
+    const double* other_gas_wet_dep;
+    const double* nh_part;
+    const double* no3_part;       /* NULL: 0 (TotalNO3 deposits as HNO3) */
+    const double* F_p_vbs;        /* NULL: pure particle */
+    const double* F_p_vbs_surface;
+    const double* j_no2_soa;      /* NULL: no SOA photolytic sink */
+    double a_photo;
+    const double* so2_ox;         /* SO2 -> pSO4 rate (1/s) */
+    const double* nox_to_no3;     /* NULL: no NOx -> TotalNO3 coupling */
+    const double* o3_loss;
+    const double* co_loss;
+    const double* archive_OH;     /* NULL: no VBS aging */
+    double k_age, aging_gain;
+} orbit_species_fields;
+
+/* The 14 species' diagonals d_out[14][N] of L_s = T + diag(d_s) and the
+ * diagonal coupling blocks k_out[6][N] (pSO4 <- SO2, TotalNO3 <- NOx,
+ * C100 <- C1000, C10 <- C100, C1 <- C10, C01 <- C1; zero when inactive),
+ * computed on the device (reference assemble_species_operators,
+ * orbit/core/operator.py:430-583).  Dz is (nz,ny,nx); outputs are host buffers. */
+int orbit_species_diagonals(orbit_ctx* ctx, int nz, int ny, int nx, const double* Dz,
+                            const orbit_species_fields* fields, double* d_out, double* k_out);
+
 /* Regular-grid multilinear lookup table on the device: the 7-D ISORROPIA
  * partitioning table of the closure (reference IsorropiaLUT.query,
  * orbit/core/isorropia_lut.py:123-147, i.e. scipy RegularGridInterpolator

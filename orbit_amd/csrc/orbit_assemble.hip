@@ -493,3 +493,169 @@ extern "C" int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracer
     *out = op;
     return ORBIT_OK;
 }
+
+// ------------------------------------------------------------------------- //
+// per-species diagonals: deposition and chemistry losses
+// ------------------------------------------------------------------------- //
+// One thread per cell computes the diagonal of every species' operator
+// L_s = T + D_s (reference assemble_species_operators, operator.py:430-583,
+// with assemble_deposition, deposition.py:101-362): the wet rate on every
+// level, the dry velocity over the surface layer thickness, the
+// partition-weighted gas/particle blends of TotalNH, TotalNO3 and the VBS bins,
+// the SOA photolytic sink, then the SO2/NOx/O3/CO chemistry losses and the VBS
+// aging loss; and the diagonal coupling blocks (pSO4 <- SO2, TotalNO3 <- NOx,
+// the four aging steps).  Species indices are the reference's (deposition.py:55-73).
+namespace {
+
+constexpr int NSP = 14;
+constexpr int S_C100 = 0, S_PM25 = 1, S_NH = 2, S_SO2 = 3, S_NOX = 4, S_PSO4 = 5, S_NO3 = 6, S_O3 = 7, S_CO = 8,
+              S_C10 = 9, S_C1 = 10, S_C01 = 11, S_C1000 = 12, S_POA = 13;
+// VBS bins by position (low to high C*), and the aging steps source -> destination
+__device__ __constant__ int VBS_BIN[5] = {S_C01, S_C1, S_C10, S_C100, S_C1000};
+constexpr int N_COUPLE = 6;  // pSO4<-SO2, NO3<-NOx, C100<-C1000, C10<-C100, C1<-C10, C01<-C1
+
+__device__ __forceinline__ int vbs_pos(int s) {
+    return s == S_C01 ? 0 : s == S_C1 ? 1 : s == S_C10 ? 2 : s == S_C100 ? 3 : s == S_C1000 ? 4 : -1;
+}
+
+__global__ void k_species_diag(orbit_species_fields f, int nz, int ny, int nx, const double* __restrict__ Dz,
+                               double* __restrict__ d, double* __restrict__ K) {
+    const size_t N = (size_t)nz * ny * nx, plane = (size_t)ny * nx;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int k = (int)(n / plane);
+    const size_t c = n - (size_t)k * plane;  // surface index of the column
+    const double pwd = f.particle_wet_dep[n], ogw = f.other_gas_wet_dep[n];
+    const double pdd0 = f.particle_dry_dep[c];
+    const double dz0 = Dz[c];
+    const double p_nh = f.nh_part[n], p_nh0 = f.nh_part[c];
+    const double p_no3 = f.no3_part ? f.no3_part[n] : 0.0, p_no30 = f.no3_part ? f.no3_part[c] : 0.0;
+    const double hno3_vd0 = f.HNO3_dry_dep ? f.HNO3_dry_dep[c] : 0.03;
+    double o3_vd0;
+    if (f.O3_dry_dep) o3_vd0 = f.O3_dry_dep[c];
+    else o3_vd0 = (f.is_land ? f.is_land[c] : 1.0) > 0.0 ? 0.004 : 0.0005;
+    const double voc0 = f.VOC_dry_dep ? f.VOC_dry_dep[c] : 1.0e-3;
+    double age[5] = {0, 0, 0, 0, 0};
+    if (f.archive_OH) {
+        const double roh = f.k_age * f.archive_OH[n];
+        for (int b = 1; b < 5; ++b) {  // sources C1 .. C1000 (position 0, C01, does not age)
+            if (f.F_p_vbs) {
+                double F = f.F_p_vbs[(size_t)b * N + n];
+                F = F < 0.0 ? 0.0 : (F > 1.0 ? 1.0 : F);
+                age[b] = roh * (1.0 - F);
+            } else {
+                age[b] = roh;
+            }
+        }
+    }
+    for (int s = 0; s < NSP; ++s) {
+        const int b = vbs_pos(s);
+        double wet = 0.0, vd = 0.0;
+        if (b >= 0) {
+            wet = f.F_p_vbs ? f.F_p_vbs[(size_t)b * N + n] * pwd : pwd;
+            if (f.F_p_vbs_surface) {
+                const double F = f.F_p_vbs_surface[(size_t)b * plane + c];
+                vd = F * pdd0 + (1.0 - F) * voc0;
+            } else {
+                vd = pdd0;
+            }
+        } else if (s == S_PM25 || s == S_POA || s == S_PSO4) {
+            wet = pwd;
+            vd = pdd0;
+        } else if (s == S_NH) {
+            wet = (1.0 - p_nh) * ogw + p_nh * pwd;
+            vd = (1.0 - p_nh0) * f.NH3_dry_dep[c] + p_nh0 * pdd0;
+        } else if (s == S_SO2) {
+            wet = f.SO2_wet_dep[n];
+            vd = f.SO2_dry_dep[c];
+        } else if (s == S_NOX) {
+            vd = f.NOx_dry_dep[c];
+        } else if (s == S_NO3) {
+            wet = (1.0 - p_no3) * ogw + p_no3 * pwd;
+            vd = (1.0 - p_no30) * hno3_vd0 + p_no30 * pdd0;
+        } else if (s == S_O3) {
+            vd = o3_vd0;
+        }
+        double v = wet;
+        if (k == 0) v += dz0 > 0.0 ? vd / dz0 : 0.0;
+        if (b >= 0 && f.j_no2_soa && f.a_photo > 0.0)
+            v = v + f.a_photo * f.j_no2_soa[n] * (f.F_p_vbs ? f.F_p_vbs[(size_t)b * N + n] : 1.0);
+        if (s == S_SO2) v = v + f.so2_ox[n];
+        if (s == S_NOX && f.nox_to_no3) v = v + f.nox_to_no3[n];
+        if (s == S_O3 && f.o3_loss) v = v + f.o3_loss[n];
+        if (s == S_CO && f.co_loss) v = v + f.co_loss[n];
+        if (b >= 1 && f.archive_OH) v = v + age[b];
+        d[(size_t)s * N + n] = v;
+    }
+    K[n] = -f.so2_ox[n];
+    K[N + n] = f.nox_to_no3 ? -f.nox_to_no3[n] : 0.0;
+    for (int q = 0; q < 4; ++q)  // C1000 -> C100, C100 -> C10, C10 -> C1, C1 -> C01
+        K[(size_t)(2 + q) * N + n] = f.archive_OH ? -f.aging_gain * age[4 - q] : 0.0;
+}
+
+}  // namespace
+
+extern "C" int orbit_species_diagonals(orbit_ctx* ctx, int nz, int ny, int nx, const double* Dz,
+                                       const orbit_species_fields* f, double* d_out, double* k_out) {
+    if (!ctx || !Dz || !f || !d_out || !k_out) return fail(ORBIT_ERR_INVALID, "species_diagonals: NULL argument");
+    if (nz < 1 || ny < 1 || nx < 1) return fail(ORBIT_ERR_INVALID, "species_diagonals: bad dimensions");
+    if (!f->particle_dry_dep || !f->SO2_dry_dep || !f->NOx_dry_dep || !f->NH3_dry_dep || !f->particle_wet_dep ||
+        !f->SO2_wet_dep || !f->other_gas_wet_dep || !f->nh_part || !f->so2_ox)
+        return fail(ORBIT_ERR_INVALID, "species_diagonals: the dry/wet deposition fields, the TotalNH "
+                                       "partitioning and the SO2 oxidation rate are required");
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    const size_t N = (size_t)nz * ny * nx, plane = (size_t)ny * nx;
+    // device copies: (nz,ny,nx) fields, surface (ny,nx) fields, the 5-bin VBS fields
+    struct Item {
+        const double* src;
+        size_t n;
+        const double** dst;
+    };
+    orbit_species_fields g = *f;
+    const Item items[] = {
+        {f->particle_dry_dep, plane, &g.particle_dry_dep}, {f->SO2_dry_dep, plane, &g.SO2_dry_dep},
+        {f->NOx_dry_dep, plane, &g.NOx_dry_dep}, {f->NH3_dry_dep, plane, &g.NH3_dry_dep},
+        {f->VOC_dry_dep, plane, &g.VOC_dry_dep}, {f->HNO3_dry_dep, plane, &g.HNO3_dry_dep},
+        {f->O3_dry_dep, plane, &g.O3_dry_dep}, {f->is_land, plane, &g.is_land},
+        {f->particle_wet_dep, N, &g.particle_wet_dep}, {f->SO2_wet_dep, N, &g.SO2_wet_dep},
+        {f->other_gas_wet_dep, N, &g.other_gas_wet_dep}, {f->nh_part, N, &g.nh_part},
+        {f->no3_part, N, &g.no3_part}, {f->F_p_vbs, 5 * N, &g.F_p_vbs},
+        {f->F_p_vbs_surface, 5 * plane, &g.F_p_vbs_surface}, {f->j_no2_soa, N, &g.j_no2_soa},
+        {f->so2_ox, N, &g.so2_ox}, {f->nox_to_no3, N, &g.nox_to_no3}, {f->o3_loss, N, &g.o3_loss},
+        {f->co_loss, N, &g.co_loss}, {f->archive_OH, N, &g.archive_OH}, {Dz, plane, nullptr},
+    };
+    size_t total = 0;
+    for (const Item& it : items)
+        if (it.src) total += (it.n + 31) / 32 * 32;
+    const size_t out_n = (size_t)(NSP + N_COUPLE) * N;
+    double* buf = nullptr;
+    if (hipMalloc(&buf, (total + out_n) * 8) != hipSuccess)
+        return fail(ORBIT_ERR_NOMEM, "species_diagonals: device allocation failed");
+    double* p = buf;
+    const double* dDz = nullptr;
+    hipError_t e = hipSuccess;
+    for (const Item& it : items) {
+        if (!it.src) {
+            if (it.dst) *it.dst = nullptr;
+            continue;
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(p, it.src, it.n * 8, hipMemcpyHostToDevice, st);
+        if (it.dst) *it.dst = p;
+        else dDz = p;
+        p += (it.n + 31) / 32 * 32;
+    }
+    double* dd = p;
+    double* dk = p + (size_t)NSP * N;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_species_diag, blocks_for(N), dim3(256), 0, st, g, nz, ny, nx, dDz, dd, dk);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(d_out, dd, (size_t)NSP * N * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(k_out, dk, (size_t)N_COUPLE * N * 8, hipMemcpyDeviceToHost, st);
+    hipError_t e2 = hipStreamSynchronize(st);
+    hipFree(buf);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return fail(ORBIT_ERR_HIP, std::string("species_diagonals: ") + hipGetErrorString(e));
+    return ORBIT_OK;
+}
