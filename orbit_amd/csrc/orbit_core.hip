@@ -413,32 +413,58 @@ __global__ __launch_bounds__(256) void k_apply_A(const double* __restrict__ V, l
 // layout permutes and batched vector kernels
 // ------------------------------------------------------------------------- //
 // k-major (N,) per (slot, state) -> padded device layout, times a scale.
-__global__ void k_ext_to_int(const double* __restrict__ src, long long sstride,
-                             double* __restrict__ dst, long long dstride, int nstates, int nz,
-                             int ncol, double scale) {
-    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int n16 = ncol * 16;
+// The two layout permutes go through an LDS tile of 64 columns x 16 levels so
+// both the k-major side (64 consecutive columns of one level) and the padded
+// side (64 consecutive 16-level columns) are read and written in contiguous runs.
+constexpr int PT_COLS = 64;
+constexpr int PT_LD = 17;  // padded row of the tile (no LDS bank conflicts on the transpose)
+
+// k-major (N,) per (slot, state) -> padded device layout, times a scale.
+// grid: (ceil(ncol / 64), nslots * nstates), 256 threads.
+__global__ __launch_bounds__(256) void k_ext_to_int(const double* __restrict__ src, long long sstride,
+                                                    double* __restrict__ dst, long long dstride, int nstates,
+                                                    int nz, int ncol, double scale) {
+    __shared__ double tile[PT_COLS * PT_LD];
     const int s = blockIdx.y;  // slot*nstates + state
-    if (g >= (size_t)n16) return;
     const int slot = s / nstates, st = s - slot * nstates;
-    const int col = (int)(g >> 4), k = (int)(g & 15);
-    double v = 0.0;
-    if (k < nz) v = scale * src[(size_t)slot * sstride + (size_t)st * (size_t)(nz * ncol) + (size_t)k * ncol + col];
-    dst[(size_t)slot * dstride + (size_t)st * n16 + g] = v;
+    const int col0 = blockIdx.x * PT_COLS;
+    const double* in = src + (size_t)slot * sstride + (size_t)st * (size_t)nz * ncol;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int k = e / PT_COLS, c = e - k * PT_COLS;
+        double v = 0.0;
+        if (k < nz && col0 + c < ncol) v = scale * in[(size_t)k * ncol + col0 + c];
+        tile[c * PT_LD + k] = v;
+    }
+    __syncthreads();
+    double* out = dst + (size_t)slot * dstride + (size_t)st * (size_t)ncol * 16 + (size_t)col0 * 16;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int c = e >> 4, k = e & 15;
+        if (col0 + c < ncol) out[e] = tile[c * PT_LD + k];
+    }
 }
 
-__global__ void k_int_to_ext(const double* __restrict__ src, long long sstride,
-                             double* __restrict__ dst, long long dstride, int nstates, int nz,
-                             int ncol) {
-    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int N = nz * ncol;
+// padded device layout -> k-major (N,) per (slot, state); same grid.
+__global__ __launch_bounds__(256) void k_int_to_ext(const double* __restrict__ src, long long sstride,
+                                                    double* __restrict__ dst, long long dstride, int nstates,
+                                                    int nz, int ncol) {
+    __shared__ double tile[PT_COLS * PT_LD];
     const int s = blockIdx.y;
-    if (g >= (size_t)N) return;
     const int slot = s / nstates, st = s - slot * nstates;
-    const int k = (int)(g / ncol), col = (int)(g - (size_t)k * ncol);
-    dst[(size_t)slot * dstride + (size_t)st * N + g] =
-        src[(size_t)slot * sstride + (size_t)st * (size_t)(ncol * 16) + (size_t)col * 16 + k];
+    const int col0 = blockIdx.x * PT_COLS;
+    const double* in = src + (size_t)slot * sstride + (size_t)st * (size_t)ncol * 16 + (size_t)col0 * 16;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int c = e >> 4, k = e & 15;
+        if (col0 + c < ncol) tile[c * PT_LD + k] = in[e];
+    }
+    __syncthreads();
+    double* out = dst + (size_t)slot * dstride + (size_t)st * (size_t)nz * ncol;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int k = e / PT_COLS, c = e - k * PT_COLS;
+        if (k < nz && col0 + c < ncol) out[(size_t)k * ncol + col0 + c] = tile[c * PT_LD + k];
+    }
 }
+
+inline dim3 grid_permute(int ncol, int nstreams) { return dim3((unsigned)((ncol + PT_COLS - 1) / PT_COLS), nstreams); }
 
 // out[slot] = a[slot] + alpha * b[slot]   (strides in elements)
 __global__ void k_axpby(const double* __restrict__ a, long long as, const double* __restrict__ b,
@@ -1678,10 +1704,10 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         // homogeneous sweeps: no forcing; s itself enters every residual
         hipLaunchKernelGGL(k_zero, dim3(grid1((size_t)nbn * n16).x, nt), dim3(256), 0, st, S.E, (long long)nbn * n16,
                            (int)(nbn * n16));
-        hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, in_dev, (long long)op->N, S.sv,
+        hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, in_dev, (long long)op->N, S.sv,
                            (long long)n16, 1, op->nz, op->ncol, 1.0);
     } else {
-        hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt * nbn), dim3(256), 0, st, in_dev,
+        hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
                            (long long)nbn * op->N, S.E, (long long)nbn * n16, nbn, op->nz, op->ncol, scale);
     }
     ORBIT_HIP_CHECK(hipGetLastError());
@@ -1703,13 +1729,13 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         if (warm_dev && o.warm_orbit) {
             // whole previous orbit: its states start the first accurate sweep
             const long long os_ = (long long)(nbn + 1) * n16;
-            hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
+            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
                                (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
             const double* c0 = dir == FWD ? S.orbit : S.orbit + (size_t)nbn * n16;
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, c0, os_, c0, os_, 0.0, S.c, vs, (int)n16, nullptr);
             warm_first = o.warm_sweeps > 0 ? std::min(o.warm_sweeps, S.acc) : S.acc;
         } else if (warm_dev) {
-            hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
+            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
                                S.c, vs, 1, op->nz, op->ncol, 1.0);
         } else {
             hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.c, vs, (int)n16);  // r = s~ - (I-M)0
@@ -1731,7 +1757,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         rc = norms(S, sptr, sstride, (int)n16, snorm);
         if (rc) return rc;
         if (warm_dev) {
-            hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
+            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
                                S.c, vs, 1, op->nz, op->ncol, 1.0);
         } else {
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, sptr, sstride, sptr, sstride, 0.0, S.c, vs, (int)n16,
@@ -1857,7 +1883,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         hipLaunchKernelGGL(k_sum_partials, grid_waves(nt), dim3(256), 0, st, S.partials, nb, S.scal, nt);
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.scal, nt * 8, hipMemcpyDeviceToHost, st));
     }
-    hipLaunchKernelGGL(k_int_to_ext, dim3(grid1(op->N).x, nt * (nbn + 1)), dim3(256), 0, st, S.orbit, os, out_dev,
+    hipLaunchKernelGGL(k_int_to_ext, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, S.orbit, os, out_dev,
                        (long long)(nbn + 1) * op->N, nbn + 1, op->nz, op->ncol);
     ORBIT_HIP_CHECK(hipGetLastError());
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
@@ -1961,7 +1987,7 @@ extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int trace
     int* tmap = (int*)(ext + op->N);
     { int rc0 = h2d_sync(st, tmap, &tracer, 4); if (rc0) return rc0; }
     ORBIT_HIP_CHECK(hipMemcpyAsync(ext, b, (size_t)op->N * 8, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_ext_to_int, dim3(grid1(n16).x, 1), dim3(256), 0, st, ext, (long long)op->N, rhs,
+    hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, 1), dim3(256), 0, st, ext, (long long)op->N, rhs,
                        (long long)n16, 1, op->nz, op->ncol, 1.0);
     Batch B{ctx, op, 1, tmap, nullptr, nullptr};
     const int dir = trans ? ADJ : FWD;
@@ -1996,7 +2022,7 @@ extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int trace
         ORBIT_HIP_CHECK(hipStreamSynchronize(st));
         if (!(std::sqrt(hb[1]) > 1e-14 * bn)) break;
     }
-    hipLaunchKernelGGL(k_int_to_ext, dim3(grid1(op->N).x, 1), dim3(256), 0, st, xs, (long long)n16, ext,
+    hipLaunchKernelGGL(k_int_to_ext, grid_permute(op->ncol, 1), dim3(256), 0, st, xs, (long long)n16, ext,
                        (long long)op->N, 1, op->nz, op->ncol);
     ORBIT_HIP_CHECK(hipMemcpyAsync(x, ext, (size_t)op->N * 8, hipMemcpyDeviceToHost, st));
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
