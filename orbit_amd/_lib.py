@@ -158,7 +158,11 @@ def _declare(lib):
 
 
 def lib():
-    """Load liborbit_hip.so from the package directory (raises if absent)."""
+    """Load liborbit_hip.so from the package directory (raises if absent).
+
+    torch is imported first: it carries its own HIP runtime, and the process
+    must load that one before the library resolves its HIP dependency, or
+    torch's later device initialisation finds no GPU."""
     global _lib
     with _lock:
         if _lib is None:
@@ -166,6 +170,7 @@ def lib():
                 raise OrbitLibraryMissing(
                     f"{LIB_PATH} is not built; run `python -m orbit_amd.build` "
                     "(the HIP path has no CPU fallback)")
+            import torch  # noqa: F401  (load order of the HIP runtime, see above)
             h = C.CDLL(LIB_PATH)
             _declare(h)
             _lib = h
