@@ -371,9 +371,10 @@ typedef struct {
 #define ORBIT_N_SWEEP_PRECS 3
 #define ORBIT_N_SLOT_BUCKETS 5
 /* Time every shape on bin 0 (slot counts 1, 2, 4, 8, 16; slot s runs tracer
- * s mod n_tracers; `reps` launches each) and keep the fastest per precision and
- * bucket.  table / us_table (nullable) receive the chosen shapes and their
- * microseconds per launch. */
+ * s mod n_tracers; `reps` launches each, best of two rounds) and keep the
+ * fastest per precision and bucket; the shape in place before the call stays
+ * unless another is more than 3 % faster.  table / us_table (nullable) receive
+ * the chosen shapes and their microseconds per launch. */
 int orbit_opset_autotune(orbit_ctx* ctx, orbit_opset* op, int reps, orbit_sweep_shape* table,
                          double* us_table);
 int orbit_opset_get_shapes(orbit_opset* op, orbit_sweep_shape* table);

@@ -2129,8 +2129,14 @@ extern "C" int orbit_opset_set_shapes(orbit_opset* op, const orbit_sweep_shape* 
 
 // Time every launch shape of the half-sweep for each sweep precision and slot
 // bucket (representative slot counts 1, 2, 4, 8, 16; slot s runs tracer
-// s mod n_tracers) on bin 0 and keep the fastest.  Each candidate: two untimed
-// launches, then `reps` launches alternating colours between HIP events.
+// s mod n_tracers) on bin 0.  Each candidate: two untimed launches, then `reps`
+// launches alternating colours between HIP events; every candidate is timed in
+// two rounds and keeps its faster round.  The shape in place before the call
+// (the incumbent) is kept unless another is faster by more than
+// AUTOTUNE_MARGIN: shapes differ by a few percent, and a pick within the timing
+// noise can be slower in the real workload.
+static constexpr double AUTOTUNE_MARGIN = 0.03;
+
 extern "C" int orbit_opset_autotune(orbit_ctx* ctx, orbit_opset* op, int reps, orbit_sweep_shape* table,
                                     double* us_table) {
     if (!ctx || !op) return fail(ORBIT_ERR_INVALID, "autotune: NULL argument");
@@ -2159,42 +2165,41 @@ extern "C" int orbit_opset_autotune(orbit_ctx* ctx, orbit_opset* op, int reps, o
         for (int bk = 0; bk < N_SLOT_BUCKETS && !err; ++bk) {
             const int ns = slots_of[bk];
             Batch B{ctx, op, ns, tmap, nullptr, nullptr};
-            SweepShape best = op->shapes[p][bk];
-            double best_us = 1e300;
-            for (int cs = 0; cs <= 1 && !err; ++cs) {
+            // candidate 0 is the incumbent
+            std::vector<SweepShape> cand{op->shapes[p][bk]};
+            for (int cs = 0; cs <= 1; ++cs) {
                 if (cs && (op->ngroups != 1 || ns < 2)) continue;
-                for (int W : {64, 128, 256}) {
-                    for (int xcd = 0; xcd <= 1; ++xcd) {
-                        SweepShape& sh = op->shapes[p][bk];
-                        sh.cs = (int8_t)cs;
-                        sh.threads = (int16_t)W;
-                        sh.xcd = (int8_t)xcd;
-                        for (int w = 0; w < 2 && !err; ++w)
-                            err = launch_halfsweep(B, 0, FWD, p, w & 1, b, vs, x, vs, false, false, nullptr);
-                        if (err) break;
-                        hipEventRecord(e0, st);
-                        for (int r = 0; r < reps && !err; ++r)
-                            err = launch_halfsweep(B, 0, FWD, p, r & 1, b, vs, x, vs, false, false, nullptr);
-                        hipEventRecord(e1, st);
-                        hipError_t he = hipEventSynchronize(e1);
-                        if (he != hipSuccess) {
-                            err = fail(ORBIT_ERR_HIP, std::string("autotune: ") + hipGetErrorString(he));
-                            break;
-                        }
-                        float ms = 0.f;
-                        hipEventElapsedTime(&ms, e0, e1);
-                        const double us = 1e3 * (double)ms / reps;
-                        if (us < best_us) {
-                            best_us = us;
-                            best = sh;
-                        }
-                    }
-                    if (err) break;
-                }
+                for (int W : {64, 128, 256})
+                    for (int xcd = 0; xcd <= 1; ++xcd) cand.push_back(SweepShape{(int8_t)cs, (int16_t)W, (int8_t)xcd});
             }
-            op->shapes[p][bk] = best;
-            if (table) table[p * N_SLOT_BUCKETS + bk] = {best.cs ? 4 : 1, best.threads, best.xcd};
-            if (us_table) us_table[p * N_SLOT_BUCKETS + bk] = best_us;
+            std::vector<double> us(cand.size(), 1e300);
+            for (int round = 0; round < 2 && !err; ++round)
+                for (size_t c = 0; c < cand.size() && !err; ++c) {
+                    op->shapes[p][bk] = cand[c];
+                    for (int w = 0; w < 2 && !err; ++w)
+                        err = launch_halfsweep(B, 0, FWD, p, w & 1, b, vs, x, vs, false, false, nullptr);
+                    if (err) break;
+                    hipEventRecord(e0, st);
+                    for (int r = 0; r < reps && !err; ++r)
+                        err = launch_halfsweep(B, 0, FWD, p, r & 1, b, vs, x, vs, false, false, nullptr);
+                    hipEventRecord(e1, st);
+                    hipError_t he = hipEventSynchronize(e1);
+                    if (he != hipSuccess) {
+                        err = fail(ORBIT_ERR_HIP, std::string("autotune: ") + hipGetErrorString(he));
+                        break;
+                    }
+                    float ms = 0.f;
+                    hipEventElapsedTime(&ms, e0, e1);
+                    us[c] = std::min(us[c], 1e3 * (double)ms / reps);
+                }
+            if (err) break;
+            size_t best = 0;
+            for (size_t c = 1; c < cand.size(); ++c)
+                if (us[c] < us[best]) best = c;
+            if (us[best] > (1.0 - AUTOTUNE_MARGIN) * us[0]) best = 0;
+            op->shapes[p][bk] = cand[best];
+            if (table) table[p * N_SLOT_BUCKETS + bk] = {cand[best].cs ? 4 : 1, cand[best].threads, cand[best].xcd};
+            if (us_table) us_table[p * N_SLOT_BUCKETS + bk] = us[best];
         }
     }
     hipEventDestroy(e0);
