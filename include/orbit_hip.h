@@ -282,7 +282,8 @@ int orbit_mass_balance(orbit_ctx* ctx, orbit_opset* op, const double* rho, orbit
 
 /* Horizontal FCT anti-diffusive source d_AD = rho*(-div(C*A)) for one species
  * and bin (reference compute_horizontal_fct_source, orbit/core/fct.py:459-492).
- * c, ghost (nullable) and d_out are host buffers of N doubles. */
+ * c, ghost (nullable) and d_out are host buffers of N doubles (device buffers
+ * for the _dev variant, where d_out must not overlap c or ghost). */
 int orbit_fct_source(orbit_ctx* ctx, const orbit_grid_metrics* g, const double* c, double dt,
                      int limited, const double* ghost, double* d_out);
 int orbit_fct_source_dev(orbit_ctx* ctx, const orbit_grid_metrics* g, const double* c_dev,
@@ -311,7 +312,10 @@ int orbit_fct_grid_destroy(orbit_fct_grid* G);
 /* Batched compute_horizontal_fct_source (reference orbit/core/fct.py:459-492) on
  * nvec device vectors: vector v is c_dev + v*c_stride (ghost likewise, nullable);
  * out + v*out_stride receives base + v*base_stride + d_AD (base nullable: d_AD).
- * All pointers are device pointers on the context's stream. */
+ * All pointers are device pointers on the context's stream; strides are >= 0.
+ * The span of out ([out, out + (nvec-1)*out_stride + N)) must not overlap the
+ * spans of c or ghost (ORBIT_ERR_INVALID), and base may overlap out only as out
+ * itself with the same stride. */
 int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, const double* c_dev,
                                long long c_stride, const double* ghost_dev, double dt, int limited,
                                const double* base_dev, long long base_stride, double* out_dev,

@@ -67,18 +67,22 @@ __global__ void k_fct_x(FctGrid g, const double* __restrict__ c, long long cs, d
     x[(size_t)blockIdx.y * xs + n] = c[(size_t)blockIdx.y * cs + n] / rho_at(g, n);
 }
 
-// x face between (k,j,i) [lo] and (k,j,i+1) [hi]; xg: x of the ghost (nullable)
-__device__ __forceinline__ Face face_x(const FctGrid& g, const double* x, const double* xg, int k, int j, int i) {
+// x face between (k,j,i) [lo] and (k,j,i+1) [hi].  xat(j, i): x of an
+// in-domain cell of level k; gat(j, i): x of the ghost there (read only when
+// has_g).  The global-memory kernels and the LDS-tiled one share this code, so
+// a face's arithmetic is the same expression whichever kernel evaluates it.
+template <class XA, class GA>
+__device__ __forceinline__ Face face_x_t(const FctGrid& g, XA xat, GA gat, bool has_g, int k, int j, int i) {
     const int nx = g.nx;
     const size_t row = ((size_t)k * g.ny + j) * nx;
-    const double xl = x[row + i], xh = x[row + i + 1];
+    const double xl = xat(j, i), xh = xat(j, i + 1);
     double xll, xhh;
-    if (i > 0) xll = x[row + i - 1];
-    else xll = xg ? xg[row + 0] : xl;
-    if (i + 2 < nx) xhh = x[row + i + 2];
-    else xhh = xg ? xg[row + nx - 1] : xh;
+    if (i > 0) xll = xat(j, i - 1);
+    else xll = has_g ? gat(j, 0) : xl;
+    if (i + 2 < nx) xhh = xat(j, i + 2);
+    else xhh = has_g ? gat(j, nx - 1) : xh;
     double phl = vl_phi(xl, xll, xh), phh = vl_phi(xh, xhh, xl);
-    if (!xg) {
+    if (!has_g) {
         if (i == 0) phl = 0.0;
         if (i == nx - 2) phh = 0.0;
     }
@@ -105,18 +109,18 @@ __device__ __forceinline__ Face face_x(const FctGrid& g, const double* x, const 
 }
 
 // y face between (k,j,i) [lo = south] and (k,j+1,i) [hi = north]
-__device__ __forceinline__ Face face_y(const FctGrid& g, const double* x, const double* xg, int k, int j, int i) {
+template <class XA, class GA>
+__device__ __forceinline__ Face face_y_t(const FctGrid& g, XA xat, GA gat, bool has_g, int k, int j, int i) {
     const int nx = g.nx, ny = g.ny;
     const size_t lvl = (size_t)k * ny * nx;
-    auto at = [&](const double* a, int jj) { return a[lvl + (size_t)jj * nx + i]; };
-    const double xl = at(x, j), xh = at(x, j + 1);
+    const double xl = xat(j, i), xh = xat(j + 1, i);
     double xll, xhh;
-    if (j > 0) xll = at(x, j - 1);
-    else xll = xg ? at(xg, 0) : xl;
-    if (j + 2 < ny) xhh = at(x, j + 2);
-    else xhh = xg ? at(xg, ny - 1) : xh;
+    if (j > 0) xll = xat(j - 1, i);
+    else xll = has_g ? gat(0, i) : xl;
+    if (j + 2 < ny) xhh = xat(j + 2, i);
+    else xhh = has_g ? gat(ny - 1, i) : xh;
     double phl = vl_phi(xl, xll, xh), phh = vl_phi(xh, xhh, xl);
-    if (!xg) {
+    if (!has_g) {
         if (j == 0) phl = 0.0;
         if (j == ny - 2) phh = 0.0;
     }
@@ -143,6 +147,21 @@ __device__ __forceinline__ Face face_y(const FctGrid& g, const double* x, const 
     f.tr_hi = f.valid ? wh : 1.0;
     f.inv = g.dy > 0.0 ? 1.0 / g.dy : 1.0;
     return f;
+}
+
+// The same faces on x (and the ghost's x) stored per cell in global memory.
+__device__ __forceinline__ Face face_x(const FctGrid& g, const double* x, const double* xg, int k, int j, int i) {
+    const size_t lvl = (size_t)k * g.ny * g.nx;
+    const int nx = g.nx;
+    return face_x_t(g, [&](int jj, int ii) { return x[lvl + (size_t)jj * nx + ii]; },
+                    [&](int jj, int ii) { return xg[lvl + (size_t)jj * nx + ii]; }, xg != nullptr, k, j, i);
+}
+
+__device__ __forceinline__ Face face_y(const FctGrid& g, const double* x, const double* xg, int k, int j, int i) {
+    const size_t lvl = (size_t)k * g.ny * g.nx;
+    const int nx = g.nx;
+    return face_y_t(g, [&](int jj, int ii) { return x[lvl + (size_t)jj * nx + ii]; },
+                    [&](int jj, int ii) { return xg[lvl + (size_t)jj * nx + ii]; }, xg != nullptr, k, j, i);
 }
 
 // Cell kernels run on a (64 x 4)-thread tile of (i, j) per block and one block
@@ -221,48 +240,187 @@ __device__ __forceinline__ double face_C(const Face& f, double Rp_hi, double Rm_
     return fmin(fmax(C, 0.0), 1.0);
 }
 
-// d = rho * sum over faces of +-tr*C*A/dx   (C = 1 when not limited)
-__global__ __launch_bounds__(256) void k_fct_div(FctGrid g, const double* __restrict__ x,
-                                                 const double* __restrict__ xg, const double* __restrict__ Rp,
-                                                 const double* __restrict__ Rm, int limited, double* __restrict__ d,
-                                                 long long xs, long long rs, long long ds,
-                                                 const double* __restrict__ base, long long bs) {
-    const CellIdx q = cell_index(g);
-    if (!q.ok) return;
-    x += (size_t)q.v * xs;
-    if (xg) xg += (size_t)q.v * xs;
-    Rp += (size_t)q.v * rs;
-    Rm += (size_t)q.v * rs;
-    d += (size_t)q.v * ds;
-    const int nx = g.nx, ny = g.ny, i = q.i, j = q.j, k = q.k;
-    const size_t n = q.n;
-    double acc = 0.0;
-    if (nx > 1) {
-        if (i > 0) {
-            Face f = face_x(g, x, xg, k, j, i - 1);
-            const double C = limited ? face_C(f, Rp[n], Rm[n], Rp[n - 1], Rm[n - 1]) : 1.0;
-            if (f.valid) acc += f.tr_hi * C * f.A * f.inv;
-        }
-        if (i < nx - 1) {
-            Face f = face_x(g, x, xg, k, j, i);
-            const double C = limited ? face_C(f, Rp[n + 1], Rm[n + 1], Rp[n], Rm[n]) : 1.0;
-            if (f.valid) acc -= f.tr_lo * C * f.A * f.inv;
-        }
-    }
-    if (ny > 1) {
-        if (j > 0) {
-            Face f = face_y(g, x, xg, k, j - 1, i);
-            const double C = limited ? face_C(f, Rp[n], Rm[n], Rp[n - nx], Rm[n - nx]) : 1.0;
-            if (f.valid) acc += f.tr_hi * C * f.A * f.inv;
-        }
-        if (j < ny - 1) {
-            Face f = face_y(g, x, xg, k, j, i);
-            const double C = limited ? face_C(f, Rp[n + nx], Rm[n + nx], Rp[n], Rm[n]) : 1.0;
-            if (f.valid) acc -= f.tr_lo * C * f.A * f.inv;
+// The source d = base + rho * sum over faces of +-tr*C*A/dx (C = 1 when not
+// limited) in one pass per (64 x 8 tile of (i, j), level, vector): x = c/rho on
+// the tile and a 3-cell halo, every face the tile and its 1-cell ring touch
+// (each once), R+- on the tile and that ring, then the divergence of the
+// tile's cells -- all through LDS, with nothing between the steps in HBM.
+// Each face, R and cell value is the same expression as in face_x_t /
+// face_y_t / k_fct_R, summed in the same order.
+constexpr int FT_I = 64, FT_J = 8, FT_H = 3;
+constexpr int FX_W = FT_I + 2 * FT_H, FX_H = FT_J + 2 * FT_H;  // x tile with halo
+constexpr int XF_W = FT_I + 3, XF_H = FT_J + 2;  // x faces: lo i in [i0-2, i0+FT_I], j in [j0-1, j0+FT_J]
+constexpr int YF_W = FT_I + 2, YF_H = FT_J + 3;  // y faces: lo j in [j0-2, j0+FT_J], i in [i0-1, i0+FT_I]
+constexpr int RR_W = FT_I + 2, RR_H = FT_J + 2;  // R: the tile and its 1-cell ring
+
+__global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, const double* __restrict__ c, long long cs,
+                                                  const double* __restrict__ cg, double dt, int limited,
+                                                  double* __restrict__ d, long long ds,
+                                                  const double* __restrict__ base, long long bs) {
+    __shared__ double sx[FX_H][FX_W];
+    __shared__ double xfA[XF_H][XF_W], xfL[XF_H][XF_W], xfH[XF_H][XF_W];
+    __shared__ double yfA[YF_H][YF_W], yfL[YF_H][YF_W], yfH[YF_H][YF_W];
+    __shared__ double sRp[RR_H][RR_W], sRm[RR_H][RR_W];
+    __shared__ double sinvx[XF_H];
+    __shared__ unsigned char xfV[XF_H][XF_W], yfV[YF_H][YF_W];
+    const int nx = g.nx, ny = g.ny;
+    const int i0 = blockIdx.x * FT_I, j0 = blockIdx.y * FT_J;
+    const int k = blockIdx.z % g.nz, v = blockIdx.z / g.nz;
+    const size_t lvl = (size_t)k * ny * nx;
+    const double* cv = c + (size_t)v * cs;
+    const double* gv = cg ? cg + (size_t)v * cs : nullptr;
+    const bool has_g = gv != nullptr;
+    for (int t = threadIdx.x; t < FX_H * FX_W; t += blockDim.x) {
+        const int jj = t / FX_W, ii = t - jj * FX_W;
+        const int j = j0 - FT_H + jj, i = i0 - FT_H + ii;
+        if (j >= 0 && j < ny && i >= 0 && i < nx) {
+            const size_t n = lvl + (size_t)j * nx + i;
+            sx[jj][ii] = cv[n] / rho_at(g, n);
         }
     }
-    const double v = acc * rho_at(g, n);
-    d[n] = base ? base[(size_t)q.v * bs + n] + v : v;
+    if (threadIdx.x < XF_H) {
+        const int j = j0 - 1 + threadIdx.x;
+        if (j >= 0 && j < ny) {
+            const double dxf = g.dx[j];
+            sinvx[threadIdx.x] = 1.0 / (dxf > 0.0 ? dxf : 1.0);
+        }
+    }
+    __syncthreads();
+    auto xat = [&](int j, int i) { return sx[j - j0 + FT_H][i - i0 + FT_H]; };
+    auto gat = [&](int j, int i) {
+        const size_t n = lvl + (size_t)j * nx + i;
+        return gv[n] / rho_at(g, n);
+    };
+    if (nx > 1)
+        for (int t = threadIdx.x; t < XF_H * XF_W; t += blockDim.x) {
+            const int fj = t / XF_W, fi = t - fj * XF_W;
+            const int j = j0 - 1 + fj, i = i0 - 2 + fi;
+            if (j >= 0 && j < ny && i >= 0 && i <= nx - 2) {
+                const Face f = face_x_t(g, xat, gat, has_g, k, j, i);
+                xfA[fj][fi] = f.A;
+                xfL[fj][fi] = f.tr_lo;
+                xfH[fj][fi] = f.tr_hi;
+                xfV[fj][fi] = f.valid;
+            }
+        }
+    if (ny > 1)
+        for (int t = threadIdx.x; t < YF_H * YF_W; t += blockDim.x) {
+            const int fj = t / YF_W, fi = t - fj * YF_W;
+            const int j = j0 - 2 + fj, i = i0 - 1 + fi;
+            if (i >= 0 && i < nx && j >= 0 && j <= ny - 2) {
+                const Face f = face_y_t(g, xat, gat, has_g, k, j, i);
+                yfA[fj][fi] = f.A;
+                yfL[fj][fi] = f.tr_lo;
+                yfH[fj][fi] = f.tr_hi;
+                yfV[fj][fi] = f.valid;
+            }
+        }
+    __syncthreads();
+    const double invy = g.dy > 0.0 ? 1.0 / g.dy : 1.0;
+    // faces with lo cell (j, i), rebuilt from LDS (beta is not needed past A)
+    auto fx = [&](int j, int i) {
+        const int fj = j - j0 + 1, fi = i - i0 + 2;
+        Face f;
+        f.beta = 0.0;
+        f.A = xfA[fj][fi];
+        f.tr_lo = xfL[fj][fi];
+        f.tr_hi = xfH[fj][fi];
+        f.valid = xfV[fj][fi] != 0;
+        f.inv = sinvx[fj];
+        return f;
+    };
+    auto fy = [&](int j, int i) {
+        const int fj = j - j0 + 2, fi = i - i0 + 1;
+        Face f;
+        f.beta = 0.0;
+        f.A = yfA[fj][fi];
+        f.tr_lo = yfL[fj][fi];
+        f.tr_hi = yfH[fj][fi];
+        f.valid = yfV[fj][fi] != 0;
+        f.inv = invy;
+        return f;
+    };
+    if (limited) {
+        for (int t = threadIdx.x; t < RR_H * RR_W; t += blockDim.x) {
+            const int rj = t / RR_W, ri = t - rj * RR_W;
+            const int j = j0 - 1 + rj, i = i0 - 1 + ri;
+            if (j < 0 || j >= ny || i < 0 || i >= nx) continue;
+            const double xc = xat(j, i);
+            // beyond the edge: the ghost (or the cell itself); needed on the edge only
+            const bool edge = i == 0 || i == nx - 1 || j == 0 || j == ny - 1;
+            const double xe_ = has_g && edge ? gat(j, i) : xc;
+            const double xw = i > 0 ? xat(j, i - 1) : xe_;
+            const double xe = i < nx - 1 ? xat(j, i + 1) : xe_;
+            const double xs_ = j > 0 ? xat(j - 1, i) : xe_;
+            const double xn = j < ny - 1 ? xat(j + 1, i) : xe_;
+            const double xmax = fmax(fmax(fmax(xc, xw), fmax(xe, xs_)), xn);
+            const double xmin = fmin(fmin(fmin(xc, xw), fmin(xe, xs_)), xn);
+            double pp = 0.0, pm = 0.0;
+            auto add = [&](double a) {
+                pp += fmax(a, 0.0);
+                pm += fmax(-a, 0.0);
+            };
+            if (nx > 1) {
+                if (i > 0) {
+                    const Face f = fx(j, i - 1);
+                    add(f.valid ? f.tr_hi * f.A * f.inv : 0.0);
+                }
+                if (i < nx - 1) {
+                    const Face f = fx(j, i);
+                    add(f.valid ? -f.tr_lo * f.A * f.inv : 0.0);
+                }
+            }
+            if (ny > 1) {
+                if (j > 0) {
+                    const Face f = fy(j - 1, i);
+                    add(f.valid ? f.tr_hi * f.A * f.inv : 0.0);
+                }
+                if (j < ny - 1) {
+                    const Face f = fy(j, i);
+                    add(f.valid ? -f.tr_lo * f.A * f.inv : 0.0);
+                }
+            }
+            const double qp = (xmax - xc) / dt, qm = (xc - xmin) / dt;
+            sRp[rj][ri] = pp > 0.0 ? fmin(1.0, qp / pp) : 0.0;
+            sRm[rj][ri] = pm > 0.0 ? fmin(1.0, qm / pm) : 0.0;
+        }
+        __syncthreads();
+    }
+    auto Rp = [&](int j, int i) { return sRp[j - j0 + 1][i - i0 + 1]; };
+    auto Rm = [&](int j, int i) { return sRm[j - j0 + 1][i - i0 + 1]; };
+    for (int t = threadIdx.x; t < FT_J * FT_I; t += blockDim.x) {
+        const int tj = t / FT_I, ti = t - tj * FT_I;
+        const int j = j0 + tj, i = i0 + ti;
+        if (j >= ny || i >= nx) continue;
+        double acc = 0.0;
+        if (nx > 1) {
+            if (i > 0) {
+                const Face f = fx(j, i - 1);
+                const double C = limited ? face_C(f, Rp(j, i), Rm(j, i), Rp(j, i - 1), Rm(j, i - 1)) : 1.0;
+                if (f.valid) acc += f.tr_hi * C * f.A * f.inv;
+            }
+            if (i < nx - 1) {
+                const Face f = fx(j, i);
+                const double C = limited ? face_C(f, Rp(j, i + 1), Rm(j, i + 1), Rp(j, i), Rm(j, i)) : 1.0;
+                if (f.valid) acc -= f.tr_lo * C * f.A * f.inv;
+            }
+        }
+        if (ny > 1) {
+            if (j > 0) {
+                const Face f = fy(j - 1, i);
+                const double C = limited ? face_C(f, Rp(j, i), Rm(j, i), Rp(j - 1, i), Rm(j - 1, i)) : 1.0;
+                if (f.valid) acc += f.tr_hi * C * f.A * f.inv;
+            }
+            if (j < ny - 1) {
+                const Face f = fy(j, i);
+                const double C = limited ? face_C(f, Rp(j + 1, i), Rm(j + 1, i), Rp(j, i), Rm(j, i)) : 1.0;
+                if (f.valid) acc -= f.tr_lo * C * f.A * f.inv;
+            }
+        }
+        const size_t n = lvl + (size_t)j * nx + i;
+        const double val = acc * rho_at(g, n);
+        d[(size_t)v * ds + n] = base ? base[(size_t)v * bs + n] + val : val;
+    }
 }
 
 // Frozen gains of L_AD: per x face (nz,ny,nx-1) and y face (nz,ny-1,nx), w_lo and w_hi.
@@ -424,6 +582,14 @@ inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 inline dim3 cell_grid(const FctGrid& g, int nvec) {
     return dim3((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)(g.nz * nvec));
 }
+// launch shape of k_fct_tile: one FT_I x FT_J tile of a level of a vector per block
+inline dim3 tile_grid(const FctGrid& g, int nvec) {
+    return dim3((unsigned)((g.nx + FT_I - 1) / FT_I), (unsigned)((g.ny + FT_J - 1) / FT_J), (unsigned)(g.nz * nvec));
+}
+// [a, a + na) and [b, b + nb) (in doubles) share memory
+inline bool overlaps(const double* a, size_t na, const double* b, size_t nb) {
+    return a < b + nb && b < a + na;
+}
 
 int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, const double* c,
                     double dt, int limited, const double* ghost, double* d) {
@@ -446,9 +612,8 @@ int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, con
     };
     const double *dc = c, *dg = ghost;
     double* dd = d;
-    double* R = dalloc(2 * N);
-    double* X = dalloc(ghost ? 2 * N : N);
-    if (!R || !X) rc = fail(ORBIT_ERR_NOMEM, "fct: device allocation failed");
+    if (!host && (overlaps(d, N, c, N) || (ghost && overlaps(d, N, ghost, N))))
+        rc = fail(ORBIT_ERR_INVALID, "fct: d must not overlap c or ghost");
     if (!rc && host) {
         double* a = dalloc(N);
         double* b = ghost ? dalloc(N) : nullptr;
@@ -462,13 +627,8 @@ int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, con
         dd = o;
     }
     if (!rc) {
-        double* xg = dg ? X + N : nullptr;
-        hipLaunchKernelGGL(k_fct_x, g1(N), dim3(256), 0, st, M.fg, dc, 0LL, X, 0LL, (int)N);
-        if (dg) hipLaunchKernelGGL(k_fct_x, g1(N), dim3(256), 0, st, M.fg, dg, 0LL, xg, 0LL, (int)N);
-        if (limited)
-            hipLaunchKernelGGL(k_fct_R, cell_grid(M.fg, 1), dim3(256), 0, st, M.fg, X, xg, dt, R, R + N, 0LL, 0LL);
-        hipLaunchKernelGGL(k_fct_div, cell_grid(M.fg, 1), dim3(256), 0, st, M.fg, X, xg, R, R + N, limited, dd, 0LL,
-                           0LL, 0LL, (const double*)nullptr, 0LL);
+        hipLaunchKernelGGL(k_fct_tile, tile_grid(M.fg, 1), dim3(256), 0, st, M.fg, dc, 0LL, dg, dt, limited, dd, 0LL,
+                           (const double*)nullptr, 0LL);
         if (hipGetLastError() != hipSuccess) rc = fail(ORBIT_ERR_HIP, "fct: kernel launch failed");
     }
     if (!rc && host && hipMemcpyAsync(d, dd, N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -752,9 +912,6 @@ struct orbit_fct_grid {
     orbit_ctx* ctx = nullptr;
     std::vector<void*> bufs;
     FctGrid fg{};
-    double* R = nullptr;   // 2 * N * cap scratch for R+ and R-
-    double* X = nullptr;   // 2 * N * cap scratch for x and the ghost's x
-    int cap = 0;
 };
 
 extern "C" int orbit_fct_grid_create(orbit_ctx* ctx, const orbit_grid_metrics* gm, orbit_fct_grid** out) {
@@ -784,8 +941,6 @@ extern "C" int orbit_fct_grid_destroy(orbit_fct_grid* G) {
         hipStreamSynchronize(G->ctx->stream);
     }
     for (void* p : G->bufs) hipFree(p);
-    if (G->R) hipFree(G->R);
-    if (G->X) hipFree(G->X);
     delete G;
     return ORBIT_OK;
 }
@@ -801,30 +956,19 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     const size_t N = (size_t)G->fg.nz * G->fg.ny * G->fg.nx;
-    if (G->cap < nvec) {
-        if (G->R || G->X) {
-            hipStreamSynchronize(st);
-            if (G->R) hipFree(G->R);
-            if (G->X) hipFree(G->X);
-            G->R = G->X = nullptr;
-            G->cap = 0;
-        }
-        ORBIT_HIP_CHECK(hipMalloc((void**)&G->R, 2 * N * (size_t)nvec * 8));
-        ORBIT_HIP_CHECK(hipMalloc((void**)&G->X, 2 * N * (size_t)nvec * 8));
-        G->cap = nvec;
-    }
-    double* x = G->X;
-    double* xg = ghost_dev ? G->X + N * nvec : nullptr;
-    dim3 gx(g1(N).x, nvec);
-    hipLaunchKernelGGL(k_fct_x, gx, dim3(256), 0, st, G->fg, c_dev, c_stride, x, (long long)N, (int)N);
-    if (ghost_dev)
-        hipLaunchKernelGGL(k_fct_x, gx, dim3(256), 0, st, G->fg, ghost_dev, c_stride, xg, (long long)N, (int)N);
-    const dim3 grid = cell_grid(G->fg, nvec);
-    if (limited)
-        hipLaunchKernelGGL(k_fct_R, grid, dim3(256), 0, st, G->fg, (const double*)x, (const double*)xg, dt, G->R,
-                           G->R + N * nvec, (long long)N, (long long)N);
-    hipLaunchKernelGGL(k_fct_div, grid, dim3(256), 0, st, G->fg, (const double*)x, (const double*)xg, G->R,
-                       G->R + N * nvec, limited, out_dev, (long long)N, (long long)N, out_stride, base_dev, base_stride);
+    // tiles read c (and the ghost) over a halo that other tiles' outputs cover:
+    // the output must not share memory with them (base may be the output itself)
+    auto span = [&](long long stride) { return (size_t)(nvec - 1) * (size_t)(stride < 0 ? -stride : stride) + N; };
+    if (c_stride < 0 || out_stride < 0 || base_stride < 0)
+        return fail(ORBIT_ERR_INVALID, "fct_source_batch: negative stride");
+    if (overlaps(out_dev, span(out_stride), c_dev, span(c_stride)) ||
+        (ghost_dev && overlaps(out_dev, span(out_stride), ghost_dev, span(c_stride))))
+        return fail(ORBIT_ERR_INVALID, "fct_source_batch: out must not overlap c or ghost");
+    if (base_dev && overlaps(out_dev, span(out_stride), base_dev, span(base_stride)) &&
+        !(base_dev == out_dev && base_stride == out_stride))
+        return fail(ORBIT_ERR_INVALID, "fct_source_batch: base overlaps out other than as out itself");
+    hipLaunchKernelGGL(k_fct_tile, tile_grid(G->fg, nvec), dim3(256), 0, st, G->fg, c_dev, c_stride, ghost_dev, dt,
+                       limited, out_dev, out_stride, base_dev, base_stride);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
