@@ -139,6 +139,8 @@ def make_grid(nx: int, ny: int, nz: int, rng: np.random.Generator, tau: int,
               lon0: float = 60.0, lat0: float = 4.0, dlon: float = 0.625,
               dlat: float = 0.5, wind_std: float = 5.0,
               dP_base: Optional[np.ndarray] = None) -> Grid:
+    if lat0 + ny * dlat >= 90.0:
+        raise ValueError(f"{ny} rows of {dlat} deg from {lat0} deg N reach the pole; use a smaller dlat")
     lat = lat0 + (np.arange(ny) + 0.5) * dlat
     dy = EARTH_RADIUS * dlat * DEG_TO_RAD
     dx = EARTH_RADIUS * np.cos(lat * DEG_TO_RAD) * dlon * DEG_TO_RAD
@@ -329,17 +331,19 @@ def to_concentration_form(T: sp.csr_matrix, rho: np.ndarray) -> sp.csr_matrix:
 
 def make_workload(nx: int, ny: int, nz: int = 15, n_tracers: int = 1, seed=0,
                   convection: bool = False, emis_frac: float = 0.3,
-                  wind_std: float = 5.0, dtau: float = DTAU) -> Workload:
+                  wind_std: float = 5.0, dtau: float = DTAU, dlon: float = 0.625,
+                  dlat: float = 0.5) -> Workload:
     """Draw one seeded orbit set (8 bins, ``n_tracers`` tracers sharing T).
 
     ``seed`` may be an int or a sequence (passed to ``np.random.default_rng``).
+    dlon, dlat: cell size in degrees (rows start at 4 deg N).
     """
     rng = np.random.default_rng(seed)
     N = nz * ny * nx
     grids, Ts = [], []
     dP_base = None
     for tau in range(N_BINS):
-        g = make_grid(nx, ny, nz, rng, tau, wind_std=wind_std, dP_base=dP_base)
+        g = make_grid(nx, ny, nz, rng, tau, wind_std=wind_std, dP_base=dP_base, dlon=dlon, dlat=dlat)
         if dP_base is None:
             # terrain-like +-5% smooth variation of the column air mass
             ps = _box3(_box3(rng.normal(0.0, 1.0, (ny, nx)), 0), 1)

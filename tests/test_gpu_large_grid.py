@@ -1,0 +1,82 @@
+"""A grid far beyond the benchmark size: 256 x 200 x 15 at 0.25 deg (768,000
+cells, 51,200 columns, 0.8 M padded unknowns per tracer and bin; 64 x 50 deg
+from 4 deg N, so the cells are 16-28 km wide and the horizontal coupling is
+stronger than at the benchmark's 0.625 x 0.5 deg).
+
+The CPU oracle cannot factor eight such bins in test time, so parity here is
+through size-independent properties of the reference's recursions
+(orbit/core/orbit.py:110-120, orbit/core/orbit_adjoint.py:50-67, 121-128),
+checked against the host CSR operators A_tau = I + dtau L_tau with one sparse
+product per bin:
+
+* forward: A_tau x_{tau+1} = x_tau + dtau e_tau for every bin, and x_8 = x_0;
+* adjoint (R_0 = r, R_{tau>0} = 0): A_tau^T (lam_tau - R_tau) = lam_{tau+1} for
+  tau = 1..7, and for tau = 0 to the solve's tolerance (lam_0 is the GMRES
+  solution, as in the reference);
+* mixed precision against the plain FP64 solve at 1e-6 in c0.
+
+These catch what a small grid cannot: 32-bit index products in the launch
+grids, permutes and reductions at this many columns."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from orbit_amd import synth
+from orbit_amd.core import Context, OperatorSet
+
+NB = 8
+
+
+@pytest.fixture(scope="module")
+def big():
+    wl = synth.make_workload(256, 200, 15, 2, seed=7, dlon=0.25, dlat=0.25)
+    ctx = Context(0)
+    op = OperatorSet.from_workload(wl, ctx)
+    yield wl, op
+    op.close()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_large_grid_forward_recursion_and_periodicity(big):
+    wl, op = big
+    e = torch.from_numpy(wl.e).cuda()
+    res = {}
+    for prec, tol in (("fp64", 1e-10), ("mixed", 1e-8)):
+        orbit, info = op.solve_forward(e, tol=tol, precision=prec)
+        orbit = orbit.cpu().numpy()
+        for s in range(wl.n_tracers):
+            assert info[s]["converged"] and info[s]["residual"] <= tol, (prec, s, info[s])
+            assert info[s]["periodicity"] < 10 * tol, (prec, s, info[s])
+            for tau in range(NB):
+                rhs = orbit[s, tau] + wl.dtau * wl.e[s, tau]
+                r = wl.A(tau, s) @ orbit[s, tau + 1] - rhs
+                assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(rhs), (prec, s, tau)
+        res[prec] = orbit
+    for s in range(wl.n_tracers):
+        assert _rel(res["mixed"][s, 0], res["fp64"][s, 0]) < 1e-6, s
+
+
+def test_large_grid_adjoint_recursion(big):
+    wl, op = big
+    rng = np.random.default_rng(11)
+    r = np.zeros(wl.N)
+    cells = rng.choice(wl.nx * wl.ny, 100, replace=False)  # surface receptor
+    r[cells] = rng.random(100)
+    R = np.zeros((1, NB, wl.N))
+    R[0, 0] = r
+    for prec, tol in (("fp64", 1e-10), ("mixed", 1e-8)):
+        lam, info = op.solve_adjoint(torch.from_numpy(R).cuda(), tracers=[1], tol=tol, precision=prec)
+        lam = lam.cpu().numpy()[0]
+        assert info[0]["converged"], (prec, info[0])
+        assert _rel(lam[NB], lam[0]) < 10 * tol, prec
+        for tau in range(NB):
+            lhs = wl.A(tau, 1).T @ (lam[tau] - R[0, tau])
+            # lam_0 is the GMRES solution itself (lam_8 = lam_0, orbit_adjoint.py:121-128),
+            # not the swept state, so bin 0 closes only to the solve's tolerance
+            bar = 100 * tol if tau == 0 else 1e-10
+            assert np.linalg.norm(lhs - lam[tau + 1]) <= bar * np.linalg.norm(lam[tau + 1]), (prec, tau)
