@@ -2236,7 +2236,13 @@ extern "C" int orbit_opset_check(orbit_ctx* ctx, orbit_opset* op, long long* cou
     hipStream_t st = ctx->stream;
     unsigned long long* dc = nullptr;
     ORBIT_HIP_CHECK(hipMalloc(&dc, 14 * sizeof(unsigned long long)));
-    ORBIT_HIP_CHECK(hipMemsetAsync(dc, 0, 14 * sizeof(unsigned long long), st));
+    {
+        const hipError_t e0 = hipMemsetAsync(dc, 0, 14 * sizeof(unsigned long long), st);
+        if (e0 != hipSuccess) {
+            hipFree(dc);
+            return fail(ORBIT_ERR_HIP, std::string("opset_check: ") + hipGetErrorString(e0));
+        }
+    }
     const size_t ncol = op->ncol, G = op->ngroups, T = op->ntr;
     const size_t nV = G * ncol * 256, nH = G * ncol * 64, nd = T * op->n16, nP = T * ncol * 256;
     for (const BinOps& B : op->bins) {
