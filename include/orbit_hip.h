@@ -74,6 +74,7 @@ typedef struct {
     double dtau;
     double device_bytes;  /* HBM held by the opset (blocks, inverses, stencils) */
     double sweep_contraction;  /* measured error contraction of one red-black sweep (with margin) */
+    int periodic_x;  /* 1 when the operators couple i = 0 and i = nx-1 (periodic longitude) */
 } orbit_opset_stats;
 
 /* Air-mass-balance statistics (reference orbit/core/operator.py:598-652). */
@@ -170,7 +171,7 @@ typedef struct {
     int nz, ny, nx;
     int split_fluxes;          /* has_split_fluxes: UAvg/VAvg_plus/minus drive advection */
     int concentration_state;   /* 1: return diag(rho) T diag(1/rho), rho = dP/(g Dz) */
-    int periodic_lon;          /* must be 0: the wrap face is not supported */
+    int periodic_lon;          /* 1: the x direction wraps (face between i = nx-1 and i = 0) */
     double dy, dlon, dlat;     /* m, degrees, degrees */
     const double* lat;         /* (ny,) degrees, cell centres */
     const double* dx;          /* (ny,) m */
@@ -197,6 +198,10 @@ typedef struct {
     const double* V_face_n;          /* (nz,nx) true north outer face, or NULL */
     const double* V_plus_face_n;
     const double* V_minus_face_n;
+    const double* UAvg_wrap;         /* (nz,ny) wind on the wrap face (periodic_lon, nx > 1) */
+    const double* UAvg_plus_wrap;    /* split fluxes only */
+    const double* UAvg_minus_wrap;
+    const double* K_meander_u_wrap;  /* (nz,ny) net fluxes only */
 } orbit_grid_fields;
 
 /* Transport block of one bin assembled on the device from its grid fields
@@ -204,7 +209,9 @@ typedef struct {
  * Patankar horizontal convection-diffusion, omega advection, Kzz diffusion and
  * CMFMC convection, then the concentration form).  stencil (host, 7*N doubles)
  * receives, k-major, the diagonal and the W, E, S, N, down (k-1), up (k+1)
- * entries of every row (0 where the neighbour does not exist). */
+ * entries of every row (0 where the neighbour does not exist); with a periodic
+ * longitude W of i = 0 and E of i = nx-1 are the couplings across the wrap
+ * face (convdiff.py:150-200). */
 int orbit_transport_assemble(orbit_ctx* ctx, const orbit_grid_fields* g, double* stencil);
 
 /* Opset of n_bins bins whose shared transport is assembled on the device from
@@ -212,7 +219,8 @@ int orbit_transport_assemble(orbit_ctx* ctx, const orbit_grid_fields* g, double*
  * k-major) is added to it: L_{bin,t} = T_bin + diag(D_{bin,t})
  * (reference assemble_single_species_L, orbit/core/operator.py:356-368), with
  * no host CSR.  The result is the same opset orbit_opset_create_shared builds
- * from the CSR of T. */
+ * from the CSR of T.  A periodic longitude needs an even nx (the red-black
+ * colouring of the sweeps; ORBIT_ERR_STRUCTURE otherwise). */
 int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracers, double dtau,
                              const orbit_grid_fields* grids, const double* D, orbit_opset** out);
 

@@ -72,7 +72,8 @@ class OpsetStats(C.Structure):
     _fields_ = [("nz", C.c_int), ("ny", C.c_int), ("nx", C.c_int), ("n_bins", C.c_int),
                 ("n_tracers", C.c_int), ("n_groups", C.c_int),
                 ("default_inner_sweeps", C.c_int), ("default_acc_sweeps", C.c_int),
-                ("dtau", C.c_double), ("device_bytes", C.c_double), ("sweep_contraction", C.c_double)]
+                ("dtau", C.c_double), ("device_bytes", C.c_double), ("sweep_contraction", C.c_double),
+                ("periodic_x", C.c_int)]
 
 
 class MbStats(C.Structure):

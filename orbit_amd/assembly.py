@@ -32,7 +32,8 @@ MASS_BALANCE_TOL = 0.05   # per day, p90 of interior |row sums| (orbit/cli.py:16
 _ARRAYS = ("lat", "dx", "dP", "Dz", "UAvg", "VAvg", "UAvg_plus", "UAvg_minus", "VAvg_plus",
            "VAvg_minus", "Kxxyy", "K_meander_u", "K_meander_v", "Kzz", "omega_edge_plus",
            "omega_edge_minus", "omega_plus", "omega_minus", "CMFMC", "U_face_e", "U_plus_face_e",
-           "U_minus_face_e", "V_face_n", "V_plus_face_n", "V_minus_face_n")
+           "U_minus_face_e", "V_face_n", "V_plus_face_n", "V_minus_face_n", "UAvg_wrap", "UAvg_plus_wrap",
+           "UAvg_minus_wrap", "K_meander_u_wrap")
 
 
 class GridFields(C.Structure):
@@ -59,9 +60,11 @@ def grid_fields(g, keep: List) -> GridFields:
     nz = f.nz
     use = set(_ARRAYS)
     if f.split_fluxes:
-        use -= {"K_meander_u", "K_meander_v"}
+        use -= {"K_meander_u", "K_meander_v", "K_meander_u_wrap"}
     else:
-        use -= {"UAvg_plus", "UAvg_minus", "VAvg_plus", "VAvg_minus"}
+        use -= {"UAvg_plus", "UAvg_minus", "VAvg_plus", "VAvg_minus", "UAvg_plus_wrap", "UAvg_minus_wrap"}
+    if not (f.periodic_lon and f.nx > 1):  # the wrap face (convdiff.py:150-200)
+        use -= {"UAvg_wrap", "UAvg_plus_wrap", "UAvg_minus_wrap", "K_meander_u_wrap"}
     # interface omega only with nz+1 levels (advection.py:398-400)
     if not (_present(g, "omega_edge_plus") and np.shape(g.omega_edge_plus)[0] == nz + 1):
         use -= {"omega_edge_plus", "omega_edge_minus"}
@@ -91,18 +94,24 @@ def transport_stencil(g, ctx=None) -> dict:
     return dict(zip(("diag", "W", "E", "S", "N", "down", "up"), out))
 
 
-def stencil_to_csc(st: dict, nz: int, ny: int, nx: int) -> sp.csc_matrix:
+def stencil_to_csc(st: dict, nz: int, ny: int, nx: int, periodic: bool = False) -> sp.csc_matrix:
+    """CSC of a 7-point stencil; with periodic, W of i = 0 couples to i = nx-1
+    and E of i = nx-1 to i = 0 (summed with the interior coupling when nx = 2)."""
     N = nz * ny * nx
     n3 = np.arange(N).reshape(nz, ny, nx)
     rows, cols, vals = [n3.ravel()], [n3.ravel()], [st["diag"].ravel()]
     sl = {"W": (np.s_[:, :, 1:], np.s_[:, :, :-1]), "E": (np.s_[:, :, :-1], np.s_[:, :, 1:]),
           "S": (np.s_[:, 1:, :], np.s_[:, :-1, :]), "N": (np.s_[:, :-1, :], np.s_[:, 1:, :]),
           "down": (np.s_[1:], np.s_[:-1]), "up": (np.s_[:-1], np.s_[1:])}
+    if periodic and nx > 1:
+        sl["W_wrap"] = (np.s_[:, :, :1], np.s_[:, :, -1:])
+        sl["E_wrap"] = (np.s_[:, :, -1:], np.s_[:, :, :1])
     for name, (own, nbr) in sl.items():
         rows.append(n3[own].ravel())
         cols.append(n3[nbr].ravel())
-        vals.append(st[name][own].ravel())
+        vals.append(st[name.split("_")[0]][own].ravel())
     A = sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    A.sum_duplicates()
     A.eliminate_zeros()
     return A
 
@@ -110,7 +119,8 @@ def stencil_to_csc(st: dict, nz: int, ny: int, nx: int) -> sp.csc_matrix:
 def assemble_transport_block(g, ctx=None) -> sp.csc_matrix:
     """The reference's assemble_transport_block (orbit/core/operator.py:281-347),
     computed on the device; returns CSC."""
-    return stencil_to_csc(transport_stencil(g, ctx), int(g.nz), int(g.ny), int(g.nx))
+    return stencil_to_csc(transport_stencil(g, ctx), int(g.nz), int(g.ny), int(g.nx),
+                          bool(getattr(g, "periodic_lon", False)))
 
 
 def density_weights(g) -> np.ndarray:

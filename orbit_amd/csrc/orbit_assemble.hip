@@ -19,8 +19,9 @@
 // array needs clearing first; the per-tracer diagonals follow as
 // d = dtau*T_nn + dtau*D_t,n.  Everything is FP64.
 //
-// A periodic longitude (wrap face) is not supported by the column-sweep layout
-// and is rejected.
+// A periodic longitude adds the wrap face between i = nx-1 (its low side) and
+// i = 0 (convdiff.py:150-200) and drops the west / east outflow; its couplings
+// sit in the W slot of i = 0 and the E slot of i = nx-1.
 
 #include <cmath>
 
@@ -42,6 +43,9 @@ constexpr double DEG2RAD = 3.14159265358979323846 / 180.0;
 struct AsmArgs {
     int nz, ny, nx;
     int split, conc;
+    int periodic;             // periodic_lon: no west / east outflow
+    int px;                   // the wrap face exists (periodic_lon and nx > 1)
+    const double *Uw, *Upw, *Umw, *Kmw;  // wrap face wind, split winds, meander K; (nz, ny)
     double dy, trS, trN;
     const double* dx;
     const double *dP, *Dz, *U, *V, *Up, *Um, *Vp, *Vm, *Kh, *Kmu, *Kmv, *Kzz;
@@ -127,7 +131,27 @@ __host__ __device__ void transport_row(const AsmArgs& a, int k, int j, int i, do
         hd += rL * tr;
         E = -rR * tr;
     }
-    if (i == 0 && vj) {  // west outflow
+    if (a.px && vj && (i == 0 || i == nx - 1)) {  // wrap face: low side i = nx-1, high side i = 0
+        const size_t kj = (size_t)k * ny + j;
+        const size_t nl = cid(a, k, j, nx - 1), nr = cid(a, k, j, 0);
+        double Kf = harm(a.Kh[nl], a.Kh[nr]);
+        if (!split) Kf = Kf + a.Kmw[kj];
+        double rL, rR;
+        face_rates(a.Uw[kj], Kf, dxj, split, split ? a.Upw[kj] : 0.0, split ? a.Umw[kj] : 0.0, rL, rR);
+        const double pL = a.dP[nl], pR = a.dP[nr];
+        const bool ok = pL > 0.0 && pR > 0.0;
+        if (i == 0) {
+            const double tr = ok ? (0.5 * (pL + pR)) / pR : 1.0;
+            hd += rR * tr;
+            W = -rL * tr;
+        }
+        if (i == nx - 1) {
+            const double tr = ok ? (0.5 * (pL + pR)) / pL : 1.0;
+            hd += rL * tr;
+            E = -rR * tr;
+        }
+    }
+    if (!a.periodic && i == 0 && vj) {  // west outflow
         if (split) {
             const double um = a.Um[n];
             if (um > 0.0) hd += um / sdx;
@@ -136,7 +160,7 @@ __host__ __device__ void transport_row(const AsmArgs& a, int k, int j, int i, do
             if (u < 0.0) hd += fabs(u) / sdx;
         }
     }
-    if (i == nx - 1 && vj) {  // east outflow through the true outer face
+    if (!a.periodic && i == nx - 1 && vj) {  // east outflow through the true outer face
         const size_t kj = (size_t)k * ny + j;
         if (split) {
             const double up = a.Upe ? a.Upe[kj] : a.Up[n];
@@ -276,8 +300,8 @@ __host__ __device__ void transport_row(const AsmArgs& a, int k, int j, int i, do
     if (a.conc) {  // diag(rho) T diag(1/rho), entry by entry
         const double rn = rho_of(a, n);
         r[0] = (rn * r[0]) * (1.0 / rn);
-        if (i > 0) r[1] = (rn * r[1]) * (1.0 / rho_of(a, n - 1));
-        if (i < nx - 1) r[2] = (rn * r[2]) * (1.0 / rho_of(a, n + 1));
+        if (i > 0 || a.px) r[1] = (rn * r[1]) * (1.0 / rho_of(a, i > 0 ? n - 1 : n + nx - 1));
+        if (i < nx - 1 || a.px) r[2] = (rn * r[2]) * (1.0 / rho_of(a, i < nx - 1 ? n + 1 : n - (nx - 1)));
         if (j > 0) r[3] = (rn * r[3]) * (1.0 / rho_of(a, n - nx));
         if (j < ny - 1) r[4] = (rn * r[4]) * (1.0 / rho_of(a, n + nx));
         if (k > 0) r[5] = (rn * r[5]) * (1.0 / rho_of(a, n - (size_t)ny * nx));
@@ -355,9 +379,13 @@ int check_fields(const orbit_grid_fields* g) {
     if (!g) return fail(ORBIT_ERR_INVALID, "assemble: NULL grid fields");
     if (g->nz < 1 || g->nz > NZP || g->ny < 1 || g->nx < 1)
         return fail(ORBIT_ERR_INVALID, "assemble: nz must be in [1, 16], ny and nx positive");
-    if (g->periodic_lon)
-        return fail(ORBIT_ERR_STRUCTURE, "assemble: a periodic longitude (wrap face) is not supported by the "
-                                         "column-sweep layout");
+    if (g->periodic_lon && g->nx > 1) {
+        if (!g->UAvg_wrap) return fail(ORBIT_ERR_INVALID, "assemble: a periodic longitude needs UAvg_wrap");
+        if (g->split_fluxes && (!g->UAvg_plus_wrap || !g->UAvg_minus_wrap))
+            return fail(ORBIT_ERR_INVALID, "assemble: split fluxes need UAvg_plus_wrap and UAvg_minus_wrap");
+        if (!g->split_fluxes && !g->K_meander_u_wrap)
+            return fail(ORBIT_ERR_INVALID, "assemble: net fluxes need K_meander_u_wrap");
+    }
     if (!g->lat || !g->dx || !g->dP || !g->Dz || !g->UAvg || !g->VAvg || !g->Kxxyy || !g->Kzz)
         return fail(ORBIT_ERR_INVALID, "assemble: lat, dx, dP, Dz, UAvg, VAvg, Kxxyy and Kzz are required");
     if (g->split_fluxes && (!g->UAvg_plus || !g->UAvg_minus || !g->VAvg_plus || !g->VAvg_minus))
@@ -379,6 +407,7 @@ int upload_fields(hipStream_t st, const orbit_grid_fields* g, FieldUpload& fu) {
     };
     AsmArgs& a = fu.a;
     fu.st = st;
+    const bool wrap = g->periodic_lon && nx > 1;
     const Item items[] = {
         {g->dx, ny, &a.dx}, {g->dP, N, &a.dP}, {g->Dz, N, &a.Dz}, {g->UAvg, N, &a.U}, {g->VAvg, N, &a.V},
         {g->split_fluxes ? g->UAvg_plus : nullptr, N, &a.Up}, {g->split_fluxes ? g->UAvg_minus : nullptr, N, &a.Um},
@@ -391,6 +420,10 @@ int upload_fields(hipStream_t st, const orbit_grid_fields* g, FieldUpload& fu) {
         {g->CMFMC, N1, &a.cm},
         {g->U_face_e, nz * ny, &a.Ue}, {g->U_plus_face_e, nz * ny, &a.Upe}, {g->U_minus_face_e, nz * ny, &a.Ume},
         {g->V_face_n, nz * nx, &a.Vn}, {g->V_plus_face_n, nz * nx, &a.Vpn}, {g->V_minus_face_n, nz * nx, &a.Vmn},
+        {wrap ? g->UAvg_wrap : nullptr, nz * ny, &a.Uw},
+        {wrap && g->split_fluxes ? g->UAvg_plus_wrap : nullptr, nz * ny, &a.Upw},
+        {wrap && g->split_fluxes ? g->UAvg_minus_wrap : nullptr, nz * ny, &a.Umw},
+        {wrap && !g->split_fluxes ? g->K_meander_u_wrap : nullptr, nz * ny, &a.Kmw},
     };
     size_t total = 0;
     for (const Item& it : items)
@@ -409,6 +442,8 @@ int upload_fields(hipStream_t st, const orbit_grid_fields* g, FieldUpload& fu) {
     a.nx = g->nx;
     a.split = g->split_fluxes ? 1 : 0;
     a.conc = g->concentration_state ? 1 : 0;
+    a.periodic = g->periodic_lon ? 1 : 0;
+    a.px = wrap ? 1 : 0;
     a.dy = g->dy;
     // zonal width of the outer south / north faces over the edge rows' width
     // (convdiff.py:335-341, 360-365)
@@ -452,7 +487,14 @@ extern "C" int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracer
         if (rc) return rc;
         if (grids[b].nz != grids[0].nz || grids[b].ny != grids[0].ny || grids[b].nx != grids[0].nx)
             return fail(ORBIT_ERR_INVALID, "create_grids: every bin needs the same grid dimensions");
+        if ((grids[b].periodic_lon != 0) != (grids[0].periodic_lon != 0))
+            return fail(ORBIT_ERR_INVALID, "create_grids: every bin needs the same periodic_lon");
     }
+    // the sweeps' red-black colouring pairs columns 0 and nx-1 of a row only with an even nx
+    const int wrap = grids[0].periodic_lon && grids[0].nx > 1;
+    if (wrap && grids[0].nx % 2)
+        return fail(ORBIT_ERR_STRUCTURE, "create_grids: a periodic longitude needs an even nx (red-black "
+                                         "colouring); nx = " + std::to_string(grids[0].nx));
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     const int nz = grids[0].nz, ny = grids[0].ny, nx = grids[0].nx, ncol = nx * ny, N = nz * ncol;
@@ -461,6 +503,7 @@ extern "C" int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracer
     op->nz = nz; op->ny = ny; op->nx = nx; op->nbins = n_bins; op->ntr = n_tracers;
     op->ngroups = 1; op->ncol = ncol; op->n16 = ncol * 16; op->N = N; op->dtau = dtau;
     op->has_tdiag = true;
+    op->px = wrap;
     int rc = alloc_bins(op);
     double* dD = nullptr;
     if (!rc && hipMalloc(&dD, (size_t)n_tracers * N * 8) != hipSuccess)

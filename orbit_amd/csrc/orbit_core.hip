@@ -164,14 +164,15 @@ __global__ void k_convert(const S* __restrict__ src, D* __restrict__ dst, size_t
 // Transposed 4-point stencil: A^T[c, nbr] = A[nbr, c], i.e. the W coefficient of
 // cell c in A^T is the E coefficient of its west neighbour in A.
 template <class T>
-__global__ void k_transpose_stencil(const T* __restrict__ H, T* __restrict__ HT, int nx, int ny) {
+__global__ void k_transpose_stencil(const T* __restrict__ H, T* __restrict__ HT, int nx, int ny, int px) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     int ncol = nx * ny;
     if (g >= ncol * 16) return;
     int col = g >> 4, k = g & 15;
     int j = col / nx, i = col - j * nx;
-    T w = i > 0 ? H[(size_t)(col - 1) * 64 + 16 + k] : T(0);
-    T e = i < nx - 1 ? H[(size_t)(col + 1) * 64 + 0 + k] : T(0);
+    const int cw = west_col(col, i, nx, px), ce = east_col(col, i, nx, px);
+    T w = cw >= 0 ? H[(size_t)cw * 64 + 16 + k] : T(0);
+    T e = ce >= 0 ? H[(size_t)ce * 64 + 0 + k] : T(0);
     T s = j > 0 ? H[(size_t)(col - nx) * 64 + 48 + k] : T(0);
     T n = j < ny - 1 ? H[(size_t)(col + nx) * 64 + 32 + k] : T(0);
     HT[(size_t)col * 64 + 0 + k] = w;
@@ -209,6 +210,7 @@ struct SweepArgs {
     const int* counts;  // nullable: per-slot sweep count; slots past theirs skip
     int sweep_idx;
     int colour;         // 0 red ((i + j) even), 1 black
+    int px;             // periodic x (wrap couplings; nx even keeps the colouring)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
@@ -302,9 +304,10 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     const CT bv = (CT)b[base];
     const CT xo = OZ ? CT(0) : (CT)x[base];
     CT xw = 0, xe = 0, xs = 0, xn = 0;
+    const int cw = west_col(col, i, a.nx, a.px), ce = east_col(col, i, a.nx, a.px);
     if (!NZ) {
-        xw = (CT)x[(size_t)(i > 0 ? col - 1 : col) * 16 + lane];
-        xe = (CT)x[(size_t)(i < a.nx - 1 ? col + 1 : col) * 16 + lane];
+        xw = (CT)x[(size_t)(cw >= 0 ? cw : col) * 16 + lane];
+        xe = (CT)x[(size_t)(ce >= 0 ? ce : col) * 16 + lane];
         xs = (CT)x[(size_t)(j > 0 ? col - a.nx : col) * 16 + lane];
         xn = (CT)x[(size_t)(j < a.ny - 1 ? col + a.nx : col) * 16 + lane];
     }
@@ -346,8 +349,8 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     CT r = bv;
     if (!NZ) {
         CT hx = 0;
-        hx += hw * (i > 0 ? xw : CT(0));
-        hx += he * (i < a.nx - 1 ? xe : CT(0));
+        hx += hw * (cw >= 0 ? xw : CT(0));
+        hx += he * (ce >= 0 ? xe : CT(0));
         hx += hs * (j > 0 ? xs : CT(0));
         hx += hn * (j < a.ny - 1 ? xn : CT(0));
         r -= hx;
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(256) void k_apply_A(const double* __restrict__ V, l
                                                  const double* __restrict__ d, long long dstride,
                                                  const double* __restrict__ x, double* __restrict__ y,
                                                  long long xstride, const int* tmap, int ncol, int nx,
-                                                 int ny) {
+                                                 int ny, int px) {
     __shared__ double sx[256];
     const int slot = blockIdx.y;
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
@@ -402,8 +405,9 @@ __global__ __launch_bounds__(256) void k_apply_A(const double* __restrict__ V, l
     for (int m = 0; m < 16; ++m) acc += Vt[(size_t)col * 256 + lane * 16 + m] * sx[(grp << 4) + m];
     const int j = col / nx, i = col - j * nx;
     const double* Hc = Ht + (size_t)col * 64 + lane;
-    if (i > 0) acc += Hc[0] * xs[(size_t)(col - 1) * 16 + lane];
-    if (i < nx - 1) acc += Hc[16] * xs[(size_t)(col + 1) * 16 + lane];
+    const int cw = west_col(col, i, nx, px), ce = east_col(col, i, nx, px);
+    if (cw >= 0) acc += Hc[0] * xs[(size_t)cw * 16 + lane];
+    if (ce >= 0) acc += Hc[16] * xs[(size_t)ce * 16 + lane];
     if (j > 0) acc += Hc[32] * xs[(size_t)(col - nx) * 16 + lane];
     if (j < ny - 1) acc += Hc[48] * xs[(size_t)(col + nx) * 16 + lane];
     if (valid) y[(size_t)slot * xstride + base] = acc;
@@ -868,7 +872,7 @@ int derive_bin(orbit_ctx* ctx, orbit_opset* op, int b) {
     hipLaunchKernelGGL((k_convert<double, float>), grid1(nV), dim3(256), 0, st, B.VT64, B.VT32, nV);
     for (int g = 0; g < G; ++g) {
         hipLaunchKernelGGL(k_transpose_stencil<double>, grid1((size_t)ncol * 16), dim3(256), 0, st,
-                           B.H64 + (size_t)g * ncol * 64, B.HT64 + (size_t)g * ncol * 64, op->nx, op->ny);
+                           B.H64 + (size_t)g * ncol * 64, B.HT64 + (size_t)g * ncol * 64, op->nx, op->ny, op->px);
     }
     hipLaunchKernelGGL((k_convert<double, float>), grid1(nH), dim3(256), 0, st, B.HT64, B.HT32, nH);
     const long long vstride = G == 1 ? 0 : (long long)ncol * 256;
@@ -906,8 +910,11 @@ struct HostBin {
 };
 
 // Parse one CSR matrix of L into the structured form.  Returns 0 or an error.
+// Couplings across the x wrap face (i = 0 <-> i = nx - 1, the reference's
+// periodic longitude, orbit/core/convdiff.py:150-200) go to the W / E slots and
+// set *wrap; with nx <= 2 they coincide with the ordinary neighbours.
 int parse_csr(int nz, int ny, int nx, double dtau, const int32_t* rp, const int32_t* ci,
-              const double* v, double* V, double* H, double* diag) {
+              const double* v, double* V, double* H, double* diag, int* wrap) {
     const int ncol = nx * ny, N = nz * ncol;
     for (int c = 0; c < ncol; ++c)
         for (int k = 0; k < NZP; ++k) V[(size_t)c * 256 + k * 16 + k] = 1.0;
@@ -935,6 +942,8 @@ int parse_csr(int nz, int ny, int nx, double dtau, const int32_t* rp, const int3
                 else if (j2 == j && i2 == i + 1) dir = 1;
                 else if (i2 == i && j2 == j - 1) dir = 2;
                 else if (i2 == i && j2 == j + 1) dir = 3;
+                else if (j2 == j && nx > 2 && i == 0 && i2 == nx - 1) dir = 0, *wrap = 1;
+                else if (j2 == j && nx > 2 && i == nx - 1 && i2 == 0) dir = 1, *wrap = 1;
                 if (dir < 0) goto bad;
                 H[(size_t)col * 64 + dir * 16 + k] += a;
                 continue;
@@ -975,6 +984,16 @@ int finish_opset(orbit_ctx* ctx, orbit_opset* op, std::vector<HostBin>& hb) {
     return finalize_opset(ctx, op);
 }
 
+// A periodic x direction keeps the red-black colouring only with an even nx:
+// with an odd one, columns 0 and nx - 1 of a row have one colour and would be
+// updated together while coupled.
+int check_wrap(int wrap, int nx) {
+    if (wrap && nx % 2)
+        return fail(ORBIT_ERR_STRUCTURE, "couplings across a periodic x wrap face need an even nx (red-black "
+                                         "colouring); nx = " + std::to_string(nx));
+    return ORBIT_OK;
+}
+
 int check_dims(int nz, int ny, int nx, int nbins, int ntr) {
     if (nz < 1 || nz > NZP) return fail(ORBIT_ERR_INVALID, "nz must be in [1, 16]");
     if (ny < 1 || nx < 1) return fail(ORBIT_ERR_INVALID, "ny and nx must be positive");
@@ -998,6 +1017,7 @@ extern "C" int orbit_opset_create_csr(orbit_ctx* ctx, int nz, int ny, int nx, in
     std::vector<HostBin> hb(n_bins);
     bool shared = true;
     std::vector<std::vector<double>> Vt(n_tracers), Ht(n_tracers);
+    int wrap = 0;
     for (int b = 0; b < n_bins && shared; ++b) {
         for (int t = 0; t < n_tracers; ++t) {
             Vt[t].assign((size_t)ncol * 256, 0.0);
@@ -1006,7 +1026,7 @@ extern "C" int orbit_opset_create_csr(orbit_ctx* ctx, int nz, int ny, int nx, in
             const int idx = b * n_tracers + t;
             if (!rowptr[idx] || !colidx[idx] || !vals[idx]) return fail(ORBIT_ERR_INVALID, "NULL CSR");
             rc = parse_csr(nz, ny, nx, dtau, rowptr[idx], colidx[idx], vals[idx], Vt[t].data(),
-                           Ht[t].data(), dg.data());
+                           Ht[t].data(), dg.data(), &wrap);
             if (rc) return rc;
             if (t > 0 && (Vt[t] != Vt[0] || Ht[t] != Ht[0])) shared = false;
             if (t == 0) hb[b].d.assign((size_t)n_tracers * ncol * 16, 0.0);
@@ -1028,15 +1048,18 @@ extern "C" int orbit_opset_create_csr(orbit_ctx* ctx, int nz, int ny, int nx, in
                 rc = parse_csr(nz, ny, nx, dtau, rowptr[idx], colidx[idx], vals[idx],
                                hb[b].V.data() + (size_t)t * ncol * 256,
                                hb[b].H.data() + (size_t)t * ncol * 64,
-                               hb[b].d.data() + (size_t)t * ncol * 16);
+                               hb[b].d.data() + (size_t)t * ncol * 16, &wrap);
                 if (rc) return rc;
             }
         }
     }
+    rc = check_wrap(wrap, nx);
+    if (rc) return rc;
     auto* op = new orbit_opset();
     op->ctx = ctx;
     op->nz = nz; op->ny = ny; op->nx = nx; op->nbins = n_bins; op->ntr = n_tracers;
     op->ngroups = G; op->ncol = ncol; op->n16 = ncol * 16; op->N = nz * ncol; op->dtau = dtau;
+    op->px = wrap;
     rc = finish_opset(ctx, op, hb);
     if (rc) {
         orbit_opset_destroy(op);
@@ -1057,13 +1080,14 @@ extern "C" int orbit_opset_create_shared(orbit_ctx* ctx, int nz, int ny, int nx,
     hipSetDevice(ctx->device);
     const int ncol = nx * ny, N = nz * ncol;
     std::vector<HostBin> hb(n_bins);
+    int wrap = 0;
     for (int b = 0; b < n_bins; ++b) {
         hb[b].V.assign((size_t)ncol * 256, 0.0);
         hb[b].H.assign((size_t)ncol * 64, 0.0);
         hb[b].tdiag.assign((size_t)ncol * 16, 0.0);
         if (!rowptr[b] || !colidx[b] || !T_vals[b]) return fail(ORBIT_ERR_INVALID, "NULL CSR");
         rc = parse_csr(nz, ny, nx, dtau, rowptr[b], colidx[b], T_vals[b], hb[b].V.data(),
-                       hb[b].H.data(), hb[b].tdiag.data());
+                       hb[b].H.data(), hb[b].tdiag.data(), &wrap);
         if (rc) return rc;
         hb[b].d.assign((size_t)n_tracers * ncol * 16, 0.0);
         for (int t = 0; t < n_tracers; ++t) {
@@ -1075,11 +1099,14 @@ extern "C" int orbit_opset_create_shared(orbit_ctx* ctx, int nz, int ny, int nx,
             }
         }
     }
+    rc = check_wrap(wrap, nx);
+    if (rc) return rc;
     auto* op = new orbit_opset();
     op->ctx = ctx;
     op->nz = nz; op->ny = ny; op->nx = nx; op->nbins = n_bins; op->ntr = n_tracers;
     op->ngroups = 1; op->ncol = ncol; op->n16 = ncol * 16; op->N = N; op->dtau = dtau;
     op->has_tdiag = true;
+    op->px = wrap;
     rc = finish_opset(ctx, op, hb);
     if (rc) {
         orbit_opset_destroy(op);
@@ -1107,6 +1134,7 @@ extern "C" int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st) {
     st->n_tracers = op->ntr; st->n_groups = op->ngroups;
     st->default_inner_sweeps = op->def_inner; st->default_acc_sweeps = op->def_acc;
     st->dtau = op->dtau; st->device_bytes = op->bytes; st->sweep_contraction = op->rho;
+    st->periodic_x = op->px;
     return ORBIT_OK;
 }
 
@@ -1217,6 +1245,7 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     if (a.ncols == 0) return ORBIT_OK;
     a.nx = op->nx;
     a.ny = op->ny;
+    a.px = op->px;
     a.vstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 256;
     a.hstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 64;
     a.dstride = op->n16;
@@ -1302,7 +1331,8 @@ int calibrate_opset(orbit_ctx* ctx, orbit_opset* op) {
             if (rc) return rc;
             if (k == 3 || k == 8) {
                 hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, 1), dim3(256), 0, st, O.V64, vstride, O.H64,
-                                   hstride, O.d64, (long long)n16, x, y, (long long)n16, tmap, op->ncol, op->nx, op->ny);
+                                   hstride, O.d64, (long long)n16, x, y, (long long)n16, tmap, op->ncol, op->nx, op->ny,
+                                   op->px);
                 hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, 1), dim3(256), 0, st, b, (long long)n16, y,
                                    (long long)n16, -1.0, y, (long long)n16, (int)n16, nullptr);
                 hipLaunchKernelGGL(k_norm2, dim3(nb, 1), dim3(RED_THREADS), 0, st, y, (long long)n16, (int)n16, nb, red);
@@ -2013,7 +2043,7 @@ extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int trace
         total += chunk;
         hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, 1), dim3(256), 0, st, trans ? O.VT64 : O.V64, vstride,
                            trans ? O.HT64 : O.H64, hstride, O.d64, (long long)n16, xs, y, (long long)n16, tmap,
-                           op->ncol, op->nx, op->ny);
+                           op->ncol, op->nx, op->ny, op->px);
         hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, 1), dim3(256), 0, st, rhs, (long long)n16, y, (long long)n16, -1.0,
                            rr, (long long)n16, (int)n16, nullptr);
         hipLaunchKernelGGL(k_norm2, dim3(nb, 1), dim3(RED_THREADS), 0, st, rr, (long long)n16, (int)n16, nb, red);

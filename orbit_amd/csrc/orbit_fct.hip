@@ -496,7 +496,8 @@ __global__ void k_fct_linop(FctGrid g, const double* __restrict__ wx, const doub
 // (dtau*T stored), in the boundary k-major layout.
 __global__ void k_rowsum_day(const double* __restrict__ V, const double* __restrict__ H,
                              const double* __restrict__ tdiag, const double* __restrict__ rho16,
-                             double* __restrict__ out, int ncol, int nz, int nx, int ny, double inv_dtau) {
+                             double* __restrict__ out, int ncol, int nz, int nx, int ny, int px,
+                             double inv_dtau) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ncol * nz) return;
     const int k = g / ncol, col = g - k * ncol;
@@ -506,8 +507,9 @@ __global__ void k_rowsum_day(const double* __restrict__ V, const double* __restr
     for (int m = 0; m < 16; ++m)
         if (m != k) acc += Vr[m] * rho16[(size_t)col * 16 + m];
     const double* Hc = H + (size_t)col * 64 + k;
-    if (i > 0) acc += Hc[0] * rho16[(size_t)(col - 1) * 16 + k];
-    if (i < nx - 1) acc += Hc[16] * rho16[(size_t)(col + 1) * 16 + k];
+    const int cw = west_col(col, i, nx, px), ce = east_col(col, i, nx, px);
+    if (cw >= 0) acc += Hc[0] * rho16[(size_t)cw * 16 + k];
+    if (ce >= 0) acc += Hc[16] * rho16[(size_t)ce * 16 + k];
     if (j > 0) acc += Hc[32] * rho16[(size_t)(col - nx) * 16 + k];
     if (j < ny - 1) acc += Hc[48] * rho16[(size_t)(col + nx) * 16 + k];
     out[g] = acc * inv_dtau / rho16[(size_t)col * 16 + k] * 86400.0;
@@ -858,7 +860,7 @@ extern "C" int orbit_mass_balance(orbit_ctx* ctx, orbit_opset* op, const double*
     for (int b = 0; b < op->nbins && !rc; ++b) {
         const BinOps& B = op->bins[b];
         hipLaunchKernelGGL(k_rowsum_day, g1(N), dim3(256), 0, s, B.V64, B.H64, B.tdiag, drho, dout, ncol, nz, nx, ny,
-                           1.0 / op->dtau);
+                           op->px, 1.0 / op->dtau);
         if (hipMemcpyAsync(r.data(), dout, (size_t)N * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess) {
             rc = fail(ORBIT_ERR_HIP, "mass_balance: copy back failed");

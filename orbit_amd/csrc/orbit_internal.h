@@ -79,6 +79,17 @@ struct BinOps {
     double* tdiag = nullptr;
 };
 
+// West / east neighbour column of column `col` at x index i: the adjacent
+// column, the column across the wrap face when x is periodic (px: couplings
+// between i = 0 and i = nx - 1, nx even), or -1 at an open edge.  With nx = 2
+// both neighbours of a column are the other column.
+__host__ __device__ inline int west_col(int col, int i, int nx, int px) {
+    return i > 0 ? col - 1 : (px ? col + nx - 1 : -1);
+}
+__host__ __device__ inline int east_col(int col, int i, int nx, int px) {
+    return i < nx - 1 ? col + 1 : (px ? col - nx + 1 : -1);
+}
+
 // Launch shape of the half-sweep kernel: slots per wave (1 or 4), threads per
 // workgroup (64, 128, 256) and block order (XCD-contiguous or linear).  Chosen
 // per sweep precision and slot-count bucket; every shape computes the same
@@ -109,6 +120,7 @@ struct Opset {
     Ctx* ctx = nullptr;
     int nz = 0, ny = 0, nx = 0, nbins = 0, ntr = 0, ngroups = 1;
     int ncol = 0, n16 = 0, N = 0;
+    int px = 0;  // periodic x: the operators couple i = 0 and i = nx - 1 (nx even)
     double dtau = 0.0;
     bool has_tdiag = false;
     int def_inner = 6, def_acc = 14;

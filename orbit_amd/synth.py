@@ -176,11 +176,14 @@ def _coo_add(rows, cols, vals, r, c, v):
 
 
 def transport_pressure_form(g: Grid, Kh: float = 1.0e3,
-                            conv: Optional[np.ndarray] = None) -> sp.csr_matrix:
+                            conv: Optional[np.ndarray] = None, periodic: bool = False) -> sp.csr_matrix:
     """Pressure-form transport block T (dx/dt = -T x) for one bin.
 
     conv: optional (ny, nx, nz, nz) convective exchange rates F[j,i,q,p] >= 0
     (dP/s moving from layer p to layer q, balanced per layer).
+    periodic: the x direction wraps (a global longitude band): the face west of
+    cell 0 joins cell nx-1 to cell 0 like an interior face, with UAvg[:, :, 0]
+    as its wind, in place of the two open x boundaries.
     """
     nz, ny, nx = g.nz, g.ny, g.nx
     N = nz * ny * nx
@@ -208,14 +211,35 @@ def transport_pressure_form(g: Grid, Kh: float = 1.0e3,
     air = dPf * u / dxf                    # air mass flux L->R (dP/s)
     G[:, :, 1:] += air
     G[:, :, :-1] -= air
-    # boundary x faces: outflow only carries tracer, inflow is clean air
-    uw = g.UAvg[:, :, 0]
-    ue = g.U_east
     dxw = np.broadcast_to(g.dx[None, :], (nz, ny))
-    _coo_add(rows, cols, vals, n3[:, :, 0], n3[:, :, 0], np.maximum(-uw, 0.0) / dxw)
-    _coo_add(rows, cols, vals, n3[:, :, -1], n3[:, :, -1], np.maximum(ue, 0.0) / dxw)
-    G[:, :, 0] += dP[:, :, 0] * uw / dxw
-    G[:, :, -1] -= dP[:, :, -1] * ue / dxw
+    if periodic:
+        if nx < 3:
+            raise ValueError("a periodic x direction needs nx >= 3")
+        # wrap face: L = cell nx-1, R = cell 0, wind of the face west of cell 0
+        u = g.UAvg[:, :, 0]
+        up, um = np.maximum(u, 0.0), np.maximum(-u, 0.0)
+        dPL, dPR = dP[:, :, -1], dP[:, :, 0]
+        dPf = 0.5 * (dPL + dPR)
+        trL, trR = dPf / dPL, dPf / dPR
+        kh = Kh / dxw
+        aL = (up + kh) / dxw
+        aR = (um + kh) / dxw
+        nL, nR = n3[:, :, -1], n3[:, :, 0]
+        _coo_add(rows, cols, vals, nL, nL, trL * aL)
+        _coo_add(rows, cols, vals, nL, nR, -trL * aR)
+        _coo_add(rows, cols, vals, nR, nR, trR * aR)
+        _coo_add(rows, cols, vals, nR, nL, -trR * aL)
+        air = dPf * u / dxw
+        G[:, :, 0] += air
+        G[:, :, -1] -= air
+    else:
+        # boundary x faces: outflow only carries tracer, inflow is clean air
+        uw = g.UAvg[:, :, 0]
+        ue = g.U_east
+        _coo_add(rows, cols, vals, n3[:, :, 0], n3[:, :, 0], np.maximum(-uw, 0.0) / dxw)
+        _coo_add(rows, cols, vals, n3[:, :, -1], n3[:, :, -1], np.maximum(ue, 0.0) / dxw)
+        G[:, :, 0] += dP[:, :, 0] * uw / dxw
+        G[:, :, -1] -= dP[:, :, -1] * ue / dxw
 
     # ---- y faces: interior (between j-1 = S and j = N) ----
     v = g.VAvg[:, 1:, :]
@@ -332,11 +356,12 @@ def to_concentration_form(T: sp.csr_matrix, rho: np.ndarray) -> sp.csr_matrix:
 def make_workload(nx: int, ny: int, nz: int = 15, n_tracers: int = 1, seed=0,
                   convection: bool = False, emis_frac: float = 0.3,
                   wind_std: float = 5.0, dtau: float = DTAU, dlon: float = 0.625,
-                  dlat: float = 0.5) -> Workload:
+                  dlat: float = 0.5, periodic: bool = False) -> Workload:
     """Draw one seeded orbit set (8 bins, ``n_tracers`` tracers sharing T).
 
     ``seed`` may be an int or a sequence (passed to ``np.random.default_rng``).
-    dlon, dlat: cell size in degrees (rows start at 4 deg N).
+    dlon, dlat: cell size in degrees (rows start at 4 deg N).  periodic: the
+    x direction wraps (transport_pressure_form).
     """
     rng = np.random.default_rng(seed)
     N = nz * ny * nx
@@ -351,7 +376,9 @@ def make_workload(nx: int, ny: int, nz: int = 15, n_tracers: int = 1, seed=0,
             g.dP = g.dP * ps[None]
             dP_base = g.dP.copy()
         conv = convection_exchange(ny, nx, nz, g.dP, rng) if convection else None
-        Tp = transport_pressure_form(g, conv=conv)
+        if periodic:  # the face east of the last cell is the face west of the first
+            g.U_east = g.UAvg[:, :, 0].copy()
+        Tp = transport_pressure_form(g, conv=conv, periodic=periodic)
         grids.append(g)
         Ts.append(Tp)
     rho = density_weights(grids[0])
@@ -374,7 +401,7 @@ def make_workload(nx: int, ny: int, nz: int = 15, n_tracers: int = 1, seed=0,
             e[t, tau, :ncol] = mag * emis_diurnal[tau]
     return Workload(nx=nx, ny=ny, nz=nz, n_tracers=n_tracers, grids=grids, T=Tc,
                     D=D, e=e, rho=rho, dtau=dtau,
-                    tag=dict(seed=seed, convection=convection))
+                    tag=dict(seed=seed, convection=convection, periodic=periodic))
 
 
 def config_workload(cfg: int, n_tracers: Optional[int] = None, month: int = 0,
@@ -401,11 +428,12 @@ def config_workload(cfg: int, n_tracers: Optional[int] = None, month: int = 0,
 # --------------------------------------------------------------------------- #
 # GridData-like fields for the device-side transport assembly
 # --------------------------------------------------------------------------- #
-def _mass_divergence(g, U, V, Ue, Vn) -> np.ndarray:
+def _mass_divergence(g, U, V, Ue, Vn, Uw=None) -> np.ndarray:
     """Horizontal air-mass divergence per cell (Pa/s) of the Patankar block's
     advective part: dP_n (T_hcd 1)_n = sum over faces of the outgoing
     dP_face u / h, with the y faces weighted by dx_face/dx_cell (the terrain
-    weights of orbit/core/convdiff.py:119-130, 285-298)."""
+    weights of orbit/core/convdiff.py:119-130, 285-298).  Uw: the wind of a
+    periodic wrap face (i = nx-1 -> 0), in place of the west / east outer faces."""
     nz, ny, nx = g.nz, g.ny, g.nx
     dP = g.dP
     dx3 = np.broadcast_to(g.dx[None, :, None], (nz, ny, nx))
@@ -413,8 +441,13 @@ def _mass_divergence(g, U, V, Ue, Vn) -> np.ndarray:
     fx = 0.5 * (dP[:, :, :-1] + dP[:, :, 1:]) * U[:, :, 1:] / dx3[:, :, 1:]    # interior x faces
     Dv[:, :, :-1] += fx
     Dv[:, :, 1:] -= fx
-    Dv[:, :, 0] -= dP[:, :, 0] * U[:, :, 0] / dx3[:, :, 0]                     # outer faces
-    Dv[:, :, -1] += dP[:, :, -1] * Ue / dx3[:, :, -1]
+    if Uw is None:
+        Dv[:, :, 0] -= dP[:, :, 0] * U[:, :, 0] / dx3[:, :, 0]                 # outer faces
+        Dv[:, :, -1] += dP[:, :, -1] * Ue / dx3[:, :, -1]
+    else:
+        fw = 0.5 * (dP[:, :, -1] + dP[:, :, 0]) * Uw / dx3[:, :, 0]             # wrap face
+        Dv[:, :, -1] += fw
+        Dv[:, :, 0] -= fw
     dxf = 0.5 * (dx3[:, :-1, :] + dx3[:, 1:, :])
     fy = dxf * 0.5 * (dP[:, :-1, :] + dP[:, 1:, :]) * V[:, 1:, :] / g.dy
     Dv[:, :-1, :] += fy / dx3[:, :-1, :]
@@ -425,7 +458,7 @@ def _mass_divergence(g, U, V, Ue, Vn) -> np.ndarray:
 
 
 def make_reference_grids(nx: int, ny: int, nz: int = 15, seed=0, wind_std: float = 5.0,
-                         cmfmc: bool = True, interface_omega: bool = True):
+                         cmfmc: bool = True, interface_omega: bool = True, periodic: bool = False):
     """Seeded GridData-like fields of the 8 bins, with the reference's attribute
     names (orbit/core/grid_data.py:20-240), for the device-side assembly of the
     transport block: winds on west/south faces plus the true east/north outer
@@ -437,7 +470,9 @@ def make_reference_grids(nx: int, ny: int, nz: int = 15, seed=0, wind_std: float
 
     interface_omega=False gives the pre-2026-09-26 reading instead: a
     cell-centred omega that does not close the air budget (the air-mass
-    pre-flight flags it)."""
+    pre-flight flags it).  periodic=True wraps the x direction: the face west
+    of cell 0 is the wrap face (UAvg_wrap, K_meander_u_wrap), with no west /
+    east outer faces (grid_data.py:427, convdiff.py:150-200)."""
     import types
     rng = np.random.default_rng(seed)
     grids = []
@@ -456,13 +491,18 @@ def make_reference_grids(nx: int, ny: int, nz: int = 15, seed=0, wind_std: float
         g.U_face_e, g.V_face_n = b.U_east, b.V_north
         g.has_split_fluxes = False
         g.concentration_state = True
-        g.periodic_lon = False
+        g.periodic_lon = periodic
         g.Kxxyy = 1.0e3 * np.exp(rng.normal(0.0, 0.3, (nz, ny, nx)))
         g.K_meander_u = 50.0 * rng.random((nz, ny, nx))
         g.K_meander_v = 50.0 * rng.random((nz, ny, nx))
         kz = np.concatenate([b.Kzz[:, :, :], b.Kzz[-1:, :, :]], axis=0)   # interfaces -> cells
         g.Kzz = kz
-        div = _mass_divergence(g, g.UAvg, g.VAvg, g.U_face_e, g.V_face_n)
+        if periodic:
+            g.UAvg_wrap = g.UAvg[:, :, 0].copy()
+            g.U_face_e = g.UAvg_wrap
+            g.K_meander_u_wrap = g.K_meander_u[:, :, 0].copy()
+        div = _mass_divergence(g, g.UAvg, g.VAvg, g.U_face_e, g.V_face_n,
+                               Uw=g.UAvg_wrap if periodic else None)
         om = np.zeros((nz + 1, ny, nx))
         om[1:] = np.cumsum(div, axis=0)        # net downward flux through each bottom face
         if interface_omega:

@@ -3,7 +3,7 @@ from grid fields.
 
 * The device stencil reproduces the reference's own assemble_transport_block
   (golden vectors of tests/golden/assembly_golden.npz) to 1e-12 relative, on
-  every non-periodic case; a periodic grid is refused with a structure error.
+  every case, the two periodic-longitude grids (wrap face) included.
 * An operator set built from grids (no host CSR) holds exactly the arrays the
   CSR path builds from the same T: their solves agree bit for bit; its
   periodic solve matches the oracle (splu + GMRES on T_oracle + D) to 1e-8.
@@ -32,11 +32,6 @@ def test_device_stencil_matches_reference_golden_vectors():
     for name in names:
         g = toa.golden_grid(z, name)
         N = g.nz * g.ny * g.nx
-        if g.periodic_lon:
-            with pytest.raises(_lib.OrbitError) as ei:
-                assembly.transport_stencil(g)
-            assert ei.value.code == _lib.ORBIT_ERR_STRUCTURE
-            continue
         A = assembly.assemble_transport_block(g)
         B = toa.golden_matrix(z, name, "T", N)
         assert abs(A - B).max() <= 1e-12 * abs(B).max(), name
@@ -46,7 +41,7 @@ def test_device_stencil_matches_reference_golden_vectors():
         for k in ref:
             assert np.abs(st[k] - ref[k]).max() <= 1e-12 * max(np.abs(ref[k]).max(), 1e-300), (name, k)
         checked += 1
-    assert checked >= 4
+    assert checked == len(names) >= 6
 
 
 @pytest.fixture(scope="module")
@@ -111,6 +106,49 @@ def test_grid_opset_rejects_bad_input(gridset):
     g.has_split_fluxes = True          # split fluxes without UAvg_plus etc.
     with pytest.raises(_lib.OrbitError):
         assembly.transport_stencil(g)
+
+
+def test_periodic_month_from_grids():
+    """A periodic longitude: the wrap face in the device rows, in the opset's
+    edge stencils, in the solves and in the air-mass check."""
+    grids, D, e = synth.grid_workload(12, 10, 6, 2, seed=31, periodic=True)
+    g0 = grids[0]
+    op_g = OperatorSet.from_grids(grids, D)
+    assert op_g.stats()["periodic_x"] == 1
+    Ts = [assembly.assemble_transport_block(g) for g in grids]
+    To = [asm.assemble_transport_block(g) for g in grids]
+    for A, B in zip(Ts, To):
+        assert abs(A - B).max() <= 1e-12 * abs(B).max()
+    op_c = OperatorSet.from_shared(Ts, D, g0.nz, g0.ny, g0.nx)
+    assert op_c.stats()["periodic_x"] == 1
+    for prec in ("fp64", "mixed"):
+        og, ig = op_g.solve_forward(e, tol=1e-10, precision=prec)
+        oc, _ = op_c.solve_forward(e, tol=1e-10, precision=prec)
+        assert np.array_equal(og, oc), prec
+        assert all(i["converged"] for i in ig)
+    og, _ = op_g.solve_forward(e, tol=1e-11, precision="fp64")
+    lg, _ = op_g.solve_adjoint(e, tol=1e-11, precision="fp64")
+    for s in range(e.shape[0]):
+        lus = [orb.factor_one_bin((To[t] + sp.diags(D[t, s])).tocsr()) for t in range(8)]
+        ref, _ = orb.solve_orbit_one_species(lus, list(e[s]), tol=1e-12, maxiter=500)
+        assert _rel(og[s, 0], ref[0]) < 1e-8, s
+        lref, _ = orb.solve_orbit_adjoint_one_species(lus, list(e[s]), tol=1e-12, maxiter=500)
+        assert _rel(lg[s, 0], lref[0]) < 1e-8, s
+    # omega closes the air budget through the wrap face as well
+    mb = assembly.air_mass_preflight(op_g, grids)
+    assert mb["interior_p90"] < 1e-8
+    ref = mass_ref.mass_balance_diagnostics(To, asm.density_weights(g0), g0.nz, g0.ny, g0.nx)
+    np.testing.assert_allclose(mb["top_p90"], ref["top_p90"], rtol=1e-9)
+
+
+def test_periodic_odd_nx_stencil_but_no_opset():
+    grids, D, _ = synth.grid_workload(11, 6, 4, 1, seed=32, periodic=True)
+    A = assembly.assemble_transport_block(grids[2])
+    B = asm.assemble_transport_block(grids[2])
+    assert abs(A - B).max() <= 1e-12 * abs(B).max()
+    with pytest.raises(_lib.OrbitError, match="even nx") as ei:
+        OperatorSet.from_grids(grids, D)
+    assert ei.value.code == _lib.ORBIT_ERR_STRUCTURE
 
 
 def test_fct_forward_month_on_grid_built_operators_matches_the_oracle():

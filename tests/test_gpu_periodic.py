@@ -1,0 +1,94 @@
+"""A periodic longitude: operators that couple i = 0 and i = nx - 1 across the
+wrap face (reference orbit/core/convdiff.py:150-200, grid_data.py:427).
+
+The CSR import puts the wrap couplings in the W / E stencil slots of the edge
+columns and the sweeps, operator applies, transposes and the mass-balance row
+sums follow them.  Checked against the CPU oracle (splu on the same CSR) at
+the bars of test_gpu_orbit.py: the LU shim both ways, forward and adjoint
+orbits in FP64 and mixed precision, the air-mass statistics; an odd nx is
+refused (the red-black colouring would pair two coupled columns of one
+colour); the smallest ring (nx = 4) matches too."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import mass_ref, orbit_ref as orb
+from orbit_amd import synth
+from orbit_amd._lib import ORBIT_ERR_STRUCTURE, OrbitError
+from orbit_amd.core import OperatorSet
+
+NB = 8
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def wrapped():
+    wl = synth.make_workload(12, 10, 6, 2, seed=9, convection=True, periodic=True)
+    op = OperatorSet.from_workload(wl)
+    lus = [[orb.factor_one_bin(wl.L(t, s)) for t in range(NB)] for s in range(wl.n_tracers)]
+    return wl, op, lus
+
+
+def test_periodic_opset_and_lu_shim(wrapped):
+    wl, op, lus = wrapped
+    assert op.stats()["periodic_x"] == 1
+    rng = np.random.default_rng(1)
+    b = rng.random(wl.N)
+    for tau in (0, 5):
+        for trans in ("N", "T"):
+            got = op.apply_P(tau, 1, b, trans=trans == "T")
+            ref = lus[1][tau].solve(b, trans=trans)
+            assert _rel(got, ref) < 1e-12, (tau, trans)
+
+
+def test_periodic_forward_and_adjoint_match_oracle(wrapped):
+    wl, op, lus = wrapped
+    for prec, tol, bar in (("fp64", 1e-11, 1e-8), ("mixed", 1e-8, 1e-6)):
+        orbit, info = op.solve_forward(wl.e, tol=tol, precision=prec)
+        for s in range(wl.n_tracers):
+            ref, _ = orb.solve_orbit_one_species(lus[s], list(wl.e[s]), tol=1e-12, maxiter=500)
+            assert info[s]["converged"], (prec, s)
+            for k in (0, 3, 8):
+                assert _rel(orbit[s, k], ref[k]) < bar, (prec, s, k)
+    R = np.random.default_rng(2).random((wl.n_tracers, NB, wl.N))
+    for prec, tol, bar in (("fp64", 1e-11, 1e-8), ("mixed", 1e-8, 1e-6)):
+        lam, info = op.solve_adjoint(R, tol=tol, precision=prec)
+        for s in range(wl.n_tracers):
+            ref, _ = orb.solve_orbit_adjoint_one_species(lus[s], list(R[s]), tol=1e-12, maxiter=500)
+            assert info[s]["converged"], (prec, s)
+            for k in (0, 4, 7):
+                assert _rel(lam[s, k], ref[k]) < bar, (prec, s, k)
+
+
+def test_periodic_mass_balance_matches_oracle():
+    wl = synth.make_workload(10, 8, 5, 1, seed=4, convection=True, periodic=True)
+    op = OperatorSet.from_workload(wl)
+    got = op.mass_balance(wl.rho)
+    ref = mass_ref.mass_balance_diagnostics(wl.T, wl.rho, wl.nz, wl.ny, wl.nx)
+    for k in ("interior_median", "interior_p90", "interior_p99", "interior_max", "top_median", "top_p90"):
+        assert got[k] == pytest.approx(ref[k], rel=1e-9, abs=1e-12), k
+    assert got["interior_max"] < 1e-9  # the wrap face conserves air mass like an interior face
+
+
+def test_periodic_odd_nx_is_refused():
+    wl = synth.make_workload(9, 6, 4, 1, seed=2, periodic=True)
+    with pytest.raises(OrbitError, match="even nx") as ei:
+        OperatorSet.from_workload(wl)
+    assert ei.value.code == ORBIT_ERR_STRUCTURE
+
+
+def test_four_column_ring_matches_oracle():
+    # the smallest ring: each edge column's wrap neighbour is two columns from
+    # its ordinary one, so a wrong W / E slot shows at once
+    wl = synth.make_workload(4, 5, 4, 1, seed=6, periodic=True)
+    op = OperatorSet.from_workload(wl)
+    assert op.stats()["periodic_x"] == 1
+    lus = [orb.factor_one_bin(wl.L(t, 0)) for t in range(NB)]
+    ref, _ = orb.solve_orbit_one_species(lus, list(wl.e[0]), tol=1e-12, maxiter=500)
+    for prec, tol, bar in (("fp64", 1e-11, 1e-8), ("mixed", 1e-8, 1e-6)):
+        orbit, info = op.solve_forward(wl.e, tol=tol, precision=prec)
+        assert info[0]["converged"] and _rel(orbit[0, 0], ref[0]) < bar, prec

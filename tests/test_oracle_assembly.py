@@ -266,3 +266,26 @@ def test_concentration_form_keeps_rho_as_equilibrium():
     assert np.abs(rc[inner]).max() < 1e-12 * abs(Tc).max()
     D = sp.diags(rho) @ Tp @ sp.diags(1.0 / rho)
     assert abs(D - Tc).max() <= 1e-14 * abs(Tc).max()
+
+
+def test_periodic_reference_grids_close_the_air_budget_through_the_wrap_face():
+    """The periodic reference-like grids of synth (wrap face instead of the west
+    and east outer faces, omega diagnosed with the wrap flux): every row of the
+    oracle's concentration-form block that is not on the south / north edge or
+    the top layer has T rho = 0, the x-edge columns included, and the wrap
+    couplings are present and of M-matrix sign."""
+    from orbit_amd import synth
+    grids, _, _ = synth.grid_workload(8, 6, 5, 1, seed=3, periodic=True)
+    g = grids[1]
+    T = asm.assemble_transport_block(g).tocsr()
+    rho = asm.density_weights(g)
+    r = (T @ rho / rho).reshape(g.nz, g.ny, g.nx)
+    assert np.abs(r[:-1, 1:-1, :]).max() < 1e-12 * abs(T).max()
+    n3 = np.arange(g.nz * g.ny * g.nx).reshape(g.nz, g.ny, g.nx)
+    w = np.array([T[a, b] for a, b in zip(n3[:, :, 0].ravel(), n3[:, :, -1].ravel())])
+    e = np.array([T[a, b] for a, b in zip(n3[:, :, -1].ravel(), n3[:, :, 0].ravel())])
+    assert (w < 0).all() and (e < 0).all()
+    # the same grid without the wrap has no such couplings
+    gp = synth.grid_workload(8, 6, 5, 1, seed=3)[0][1]
+    Tn = asm.assemble_transport_block(gp).tocsr()
+    assert all(Tn[a, b] == 0 for a, b in zip(n3[:, :, 0].ravel(), n3[:, :, -1].ravel()))
