@@ -214,6 +214,15 @@ typedef struct {
  * face (convdiff.py:150-200). */
 int orbit_transport_assemble(orbit_ctx* ctx, const orbit_grid_fields* g, double* stencil);
 
+/* Lateral boundary inflow rate r_in >= 0 (1/s) of every cell of one bin
+ * (reference compute_lateral_boundary_inflow, orbit/core/lateral_bc.py:58-92):
+ * the open boundary's outflow rule with the wind reversed (west / true east
+ * outer face unless periodic_lon, south / true north outer face with the outer
+ * faces' zonal width over the edge rows'), no terrain ratio.  rate: host, N
+ * doubles, k-major.  The orbit source of a boundary concentration c_bc is
+ * r_in * c_bc (lateral_bc.py:95-137). */
+int orbit_lateral_inflow(orbit_ctx* ctx, const orbit_grid_fields* g, double* rate);
+
 /* Opset of n_bins bins whose shared transport is assembled on the device from
  * grids[n_bins] and whose per-tracer diagonal D[n_bins][n_tracers][N] (host,
  * k-major) is added to it: L_{bin,t} = T_bin + diag(D_{bin,t})

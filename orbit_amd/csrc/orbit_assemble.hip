@@ -347,6 +347,50 @@ __global__ void k_assemble_opset(AsmArgs a, double dtau, double* __restrict__ V,
     tdiag[(size_t)col * 16 + k] = dtau * r[0];
 }
 
+// Lateral boundary inflow rate r_in >= 0 (1/s) per cell (reference
+// compute_lateral_boundary_inflow, orbit/core/lateral_bc.py:58-92): the
+// first-order upwind rate of the air entering through the cell's boundary
+// faces -- the west face of i = 0 and the true east outer face of i = nx-1
+// (none on a periodic longitude), the south face of j = 0 and the true north
+// outer face of j = ny-1 times the outer faces' zonal width over the edge
+// rows' -- the transport block's open-boundary outflow with the wind reversed,
+// and no terrain ratio.  Accumulated west, east, south, north.
+__global__ void k_lateral_inflow(AsmArgs a, double* __restrict__ out) {
+    const size_t N = (size_t)a.nz * a.ny * a.nx;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int i = (int)(n % a.nx), j = (int)((n / a.nx) % a.ny), k = (int)(n / ((size_t)a.nx * a.ny));
+    const bool split = a.split != 0;
+    const double dxj = a.dx[j];
+    const bool vj = dxj > 0.0;
+    const double sdx = vj ? dxj : 1.0;
+    double r = 0.0;
+    if (!a.periodic) {
+        if (i == 0) {
+            const size_t nw = cid(a, k, j, 0);
+            const double u = split ? a.Up[nw] : fmax(a.U[nw], 0.0);
+            if (vj && u > 0.0) r += u / sdx;
+        }
+        if (i == a.nx - 1) {
+            const size_t kj = (size_t)k * a.ny + j, ne = cid(a, k, j, a.nx - 1);
+            const double u = split ? (a.Ume ? a.Ume[kj] : a.Um[ne]) : fmax(-(a.Ue ? a.Ue[kj] : a.U[ne]), 0.0);
+            if (vj && u > 0.0) r += u / sdx;
+        }
+    }
+    const double sdy = a.dy > 0.0 ? a.dy : 1.0;
+    if (j == 0) {
+        const size_t ns = cid(a, k, 0, i);
+        const double v = split ? a.Vp[ns] : fmax(a.V[ns], 0.0);
+        if (vj && v > 0.0) r += v / sdy * a.trS;
+    }
+    if (j == a.ny - 1) {
+        const size_t ki = (size_t)k * a.nx + i, nn = cid(a, k, a.ny - 1, i);
+        const double v = split ? (a.Vmn ? a.Vmn[ki] : a.Vm[nn]) : fmax(-(a.Vn ? a.Vn[ki] : a.V[nn]), 0.0);
+        if (vj && v > 0.0) r += v / sdy * a.trN;
+    }
+    out[n] = r;
+}
+
 // d[t] = tdiag + dtau * D[t] (D k-major, N per tracer), zero on padded levels.
 __global__ void k_tracer_diag(const double* __restrict__ tdiag, const double* __restrict__ D, double dtau,
                               double* __restrict__ d, int ncol, int nz, int ntr) {
@@ -473,6 +517,27 @@ extern "C" int orbit_transport_assemble(orbit_ctx* ctx, const orbit_grid_fields*
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     hipFree(out);
     if (e != hipSuccess) return fail(ORBIT_ERR_HIP, std::string("assemble: ") + hipGetErrorString(e));
+    return ORBIT_OK;
+}
+
+extern "C" int orbit_lateral_inflow(orbit_ctx* ctx, const orbit_grid_fields* g, double* rate) {
+    if (!ctx || !rate) return fail(ORBIT_ERR_INVALID, "lateral_inflow: NULL argument");
+    int rc = check_fields(g);
+    if (rc) return rc;
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    FieldUpload fu;
+    rc = upload_fields(st, g, fu);
+    if (rc) return rc;
+    const size_t N = (size_t)g->nz * g->ny * g->nx;
+    double* out = nullptr;
+    if (hipMalloc(&out, N * 8) != hipSuccess) return fail(ORBIT_ERR_NOMEM, "lateral_inflow: device allocation failed");
+    hipLaunchKernelGGL(k_lateral_inflow, blocks_for(N), dim3(256), 0, st, fu.a, out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(rate, out, N * 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(out);
+    if (e != hipSuccess) return fail(ORBIT_ERR_HIP, std::string("lateral_inflow: ") + hipGetErrorString(e));
     return ORBIT_OK;
 }
 

@@ -32,8 +32,11 @@ def make_fct_grids(grids: Sequence, ctx=None) -> List[FctGrid]:
 def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, n_passes: int = 6,
                 tol: float = 1e-8, precision: str = "mixed", ghost=None, maxiter: int = 200,
                 warm: str = "c0", ctx=None, **kw):
-    """e: (nt, n_bins, N) CUDA float64.  ghost: optional (nt, n_bins, N) lateral
-    inflow concentration per bin.  warm: "orbit" starts each pass from the whole
+    """e: (nt, n_bins, N) CUDA float64.  ghost: optional lateral inflow
+    concentration per bin, the FCT value beyond the edge: an (nt, n_bins, N)
+    tensor, or a list of nt (n_bins, N) tensors with None for a tracer without
+    one (its edge faces stay closed; lateral_bc.batch_inputs builds it).
+    warm: "orbit" starts each pass from the whole
     previous orbit (mixed precision), "c0" from its c0 only.  ctx: the context
     (stream) the solves run on, the FCT grids' own (default: the operator set's).
     Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
@@ -46,12 +49,21 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
     e_prev = e
     rho = op.stats()["sweep_contraction"]
     eps = min(1e-12, 0.01 * tol)
+    groups = _ghost_groups(ghost, e)
     for _ in range(1, n_passes):
         e_eff = torch.empty_like(e)
         for tau in range(op.n_bins):
             # queued on the solves' stream: the next solve reads e_eff in order
-            fct_grids[tau].source_batch(orbit[:, tau + 1], e_eff[:, tau], base=e[:, tau], dt=op.dtau,
-                                        ghost=None if ghost is None else ghost[:, tau], sync=False)
+            if groups is None:
+                fct_grids[tau].source_batch(orbit[:, tau + 1], e_eff[:, tau], base=e[:, tau], dt=op.dtau,
+                                            ghost=None if ghost is None else ghost[:, tau], sync=False)
+                continue
+            for idx, gh in groups:  # tracers with and without a ghost: one launch each
+                out = torch.empty((idx.numel(), e.shape[2]), dtype=e.dtype, device=e.device)
+                # sync: torch reads `out` on its own stream next
+                fct_grids[tau].source_batch(orbit[idx, tau + 1], out, base=e[idx, tau], dt=op.dtau,
+                                            ghost=None if gh is None else gh[:, tau].contiguous(), sync=True)
+                e_eff[:, tau].index_copy_(0, idx, out)
         if precision == "mixed" and warm == "orbit":
             # The previous orbit starts the first accurate sweep.  Its states are
             # off by at most dtau * sum_tau ||de_tau|| (the propagators contract),
@@ -73,6 +85,24 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
         e_prev = e_eff
         hist.append(info)
     return orbit, hist
+
+
+def _ghost_groups(ghost, e):
+    """None for no ghost or one tensor for all tracers; else [(index tensor,
+    stacked ghosts or None)] for the tracers with and those without a ghost."""
+    if ghost is None or torch.is_tensor(ghost):
+        return None
+    if len(ghost) != e.shape[0]:
+        raise ValueError("ghost needs one entry per tracer")
+    with_g = [q for q, g in enumerate(ghost) if g is not None]
+    without = [q for q, g in enumerate(ghost) if g is None]
+    out = []
+    if without:
+        out.append((torch.tensor(without, device=e.device), None))
+    if with_g:
+        out.append((torch.tensor(with_g, device=e.device),
+                    torch.stack([ghost[q].to(e.device, torch.float64) for q in with_g])))
+    return out
 
 
 def frozen_operators(op: OperatorSet, fct_grids: Sequence[FctGrid], orbit_base) -> List[List[FctLinearOperator]]:

@@ -156,7 +156,7 @@ class FctGrid:
         for t in (c, out) + ((base,) if base is not None else ()) + ((ghost,) if ghost is not None else ()):
             if t.stride(-1) != 1 or t.shape[-1] != self.N:
                 raise ValueError("FCT batch views need unit-stride rows of length N")
-        if ghost is not None and ghost.stride(0) != c.stride(0):
+        if ghost is not None and nvec > 1 and ghost.stride(0) != c.stride(0):
             raise ValueError("ghost rows must share c's row stride")
         _torch_sync(c)
         check(lib().orbit_fct_source_batch_dev(

@@ -452,13 +452,19 @@ def _mass_divergence(g, U, V, Ue, Vn, Uw=None) -> np.ndarray:
     fy = dxf * 0.5 * (dP[:, :-1, :] + dP[:, 1:, :]) * V[:, 1:, :] / g.dy
     Dv[:, :-1, :] += fy / dx3[:, :-1, :]
     Dv[:, 1:, :] -= fy / dx3[:, 1:, :]
-    Dv[:, 0, :] -= dP[:, 0, :] * V[:, 0, :] / g.dy
-    Dv[:, -1, :] += dP[:, -1, :] * Vn / g.dy
+    # outer south / north faces: the zonal width of the face over the edge row's
+    # (the ratio of the reference's open-boundary rates, convdiff.py:335-365)
+    zonal = lambda la: EARTH_RADIUS * np.cos(la * DEG_TO_RAD) * g.dlon * DEG_TO_RAD  # noqa: E731
+    tr_s = zonal(g.lat[0] - g.dlat / 2.0) / (g.dx[0] if g.dx[0] > 0 else 1.0)
+    tr_n = zonal(g.lat[-1] + g.dlat / 2.0) / (g.dx[-1] if g.dx[-1] > 0 else 1.0)
+    Dv[:, 0, :] -= dP[:, 0, :] * V[:, 0, :] / g.dy * tr_s
+    Dv[:, -1, :] += dP[:, -1, :] * Vn / g.dy * tr_n
     return Dv
 
 
 def make_reference_grids(nx: int, ny: int, nz: int = 15, seed=0, wind_std: float = 5.0,
-                         cmfmc: bool = True, interface_omega: bool = True, periodic: bool = False):
+                         cmfmc: bool = True, interface_omega: bool = True, periodic: bool = False,
+                         split: bool = False):
     """Seeded GridData-like fields of the 8 bins, with the reference's attribute
     names (orbit/core/grid_data.py:20-240), for the device-side assembly of the
     transport block: winds on west/south faces plus the true east/north outer
@@ -472,7 +478,10 @@ def make_reference_grids(nx: int, ny: int, nz: int = 15, seed=0, wind_std: float
     cell-centred omega that does not close the air budget (the air-mass
     pre-flight flags it).  periodic=True wraps the x direction: the face west
     of cell 0 is the wrap face (UAvg_wrap, K_meander_u_wrap), with no west /
-    east outer faces (grid_data.py:427, convdiff.py:150-200)."""
+    east outer faces (grid_data.py:427, convdiff.py:150-200).  split=True gives
+    split fluxes: UAvg_plus - UAvg_minus = UAvg on every face (each side carrying
+    a 0.3 m/s counter-flow, as a sub-grid exchange), the outer and wrap faces
+    included, so the net air budget is that of the net winds."""
     import types
     rng = np.random.default_rng(seed)
     grids = []
@@ -501,6 +510,23 @@ def make_reference_grids(nx: int, ny: int, nz: int = 15, seed=0, wind_std: float
             g.UAvg_wrap = g.UAvg[:, :, 0].copy()
             g.U_face_e = g.UAvg_wrap
             g.K_meander_u_wrap = g.K_meander_u[:, :, 0].copy()
+        if split:
+            cf = 0.3
+            g.has_split_fluxes = True
+            for name, a in (("UAvg", g.UAvg), ("VAvg", g.VAvg), ("U_face_e", g.U_face_e),
+                            ("V_face_n", g.V_face_n)):
+                base = {"UAvg": "UAvg", "VAvg": "VAvg", "U_face_e": "U", "V_face_n": "V"}[name]
+                plus = np.maximum(a, 0.0) + cf
+                minus = np.maximum(-a, 0.0) + cf
+                if name in ("UAvg", "VAvg"):
+                    setattr(g, base + "_plus", plus)
+                    setattr(g, base + "_minus", minus)
+                else:
+                    setattr(g, base + "_plus" + name[1:], plus)
+                    setattr(g, base + "_minus" + name[1:], minus)
+            if periodic:
+                g.UAvg_plus_wrap = g.UAvg_plus[:, :, 0].copy()
+                g.UAvg_minus_wrap = g.UAvg_minus[:, :, 0].copy()
         div = _mass_divergence(g, g.UAvg, g.VAvg, g.U_face_e, g.V_face_n,
                                Uw=g.UAvg_wrap if periodic else None)
         om = np.zeros((nz + 1, ny, nx))
