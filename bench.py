@@ -180,6 +180,10 @@ def main():
         "speedup_vs_plain_fp64": round(value / plain_value, 2),
         "max_rel_err_c0_vs_fp64": max(errs),
         "gmres_iters_mixed_per_pass": iters_m[-1] if iters_m else None,
+        # the operator set's calibrated sweeps: relaxation omega of the red-black
+        # correction, measured contraction (with its 20% margin), sweep counts
+        "sweeps": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in op.stats().items()
+                   if k in ("relaxation", "sweep_contraction", "default_acc_sweeps", "default_inner_sweeps")},
     }
 
     if not args.quick:

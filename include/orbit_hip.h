@@ -75,6 +75,7 @@ typedef struct {
     double device_bytes;  /* HBM held by the opset (blocks, inverses, stencils) */
     double sweep_contraction;  /* measured error contraction of one red-black sweep (with margin) */
     int periodic_x;  /* 1 when the operators couple i = 0 and i = nx-1 (periodic longitude) */
+    double relaxation;  /* omega of the sweeps' correction x += omega P (b - A x) */
 } orbit_opset_stats;
 
 /* Air-mass-balance statistics (reference orbit/core/operator.py:598-652). */
@@ -132,6 +133,15 @@ int orbit_opset_create_shared(orbit_ctx* ctx, int nz, int ny, int nx, int n_bins
                               const double* D, orbit_opset** out);
 int orbit_opset_destroy(orbit_opset* op);
 int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st);
+/* Relaxation of the red-black sweeps (the correction x += omega P (b - A x);
+ * A x = b stays the fixed point, so omega changes the contraction and the
+ * sweep counts, not the answer).  Opsets choose omega at creation by measuring
+ * the contraction; omega = 0 chooses again, 1 is plain block Gauss-Seidel, any
+ * value in [0.5, 1.5] is measured and used (ORBIT_ERR_INVALID, and the opset
+ * unchanged, if its sweeps then do not contract).  No counterpart in the
+ * reference (its direct factorisation has no iteration).  Not to be called
+ * while a solve on the opset runs. */
+int orbit_opset_set_relaxation(orbit_ctx* ctx, orbit_opset* op, double omega);
 /* Replace the per-tracer diagonals (D laid out [bin][t][N]) and refactor the
  * column blocks in place: the outer chemistry loop changes only loss rates. */
 int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const double* D);

@@ -36,7 +36,7 @@ EXPORTS = [
     "orbit_spmat_apply_dev", "orbit_spmat_destroy", "orbit_opset_autotune",
     "orbit_opset_get_shapes", "orbit_opset_set_shapes", "orbit_transport_assemble",
     "orbit_opset_create_grids", "orbit_lut_create", "orbit_lut_query_dev", "orbit_lut_destroy",
-    "orbit_species_diagonals", "orbit_lateral_inflow",
+    "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
 N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
@@ -73,7 +73,7 @@ class OpsetStats(C.Structure):
                 ("n_tracers", C.c_int), ("n_groups", C.c_int),
                 ("default_inner_sweeps", C.c_int), ("default_acc_sweeps", C.c_int),
                 ("dtau", C.c_double), ("device_bytes", C.c_double), ("sweep_contraction", C.c_double),
-                ("periodic_x", C.c_int)]
+                ("periodic_x", C.c_int), ("relaxation", C.c_double)]
 
 
 class MbStats(C.Structure):
@@ -145,6 +145,7 @@ def _declare(lib):
         "orbit_opset_set_shapes": (_I, [_P, C.POINTER(SweepShape)]),
         "orbit_transport_assemble": (_I, [_P, _P, _P]),
         "orbit_lateral_inflow": (_I, [_P, _P, _P]),
+        "orbit_opset_set_relaxation": (_I, [_P, _P, _D]),
         "orbit_opset_create_grids": (_I, [_P, _I, _I, _D, _P, _P, C.POINTER(_P)]),
         "orbit_lut_create": (_I, [_P, _I, _P, _P, _I, _P, C.POINTER(_P)]),
         "orbit_lut_query_dev": (_I, [_P, _P, C.c_longlong, _P, _P, _P, _P]),

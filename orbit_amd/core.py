@@ -146,6 +146,15 @@ class OperatorSet:
     def from_workload(cls, wl, ctx: Optional[Context] = None) -> "OperatorSet":
         return cls.from_shared(wl.T, wl.D, wl.nz, wl.ny, wl.nx, wl.dtau, ctx)
 
+    def set_relaxation(self, omega: float = 0.0) -> dict:
+        """Relaxation of the red-black sweeps' correction x += omega P (b - A x):
+        0 chooses it by measuring the sweep contraction (as at creation), 1 is
+        plain block Gauss-Seidel, any value in [0.5, 1.5] is measured and used.
+        The fixed point A x = b does not depend on omega; the contraction and
+        the sweep counts do.  Returns the new stats."""
+        check(lib().orbit_opset_set_relaxation(self.ctx.handle, self.handle, float(omega)), allow_warn=False)
+        return self.stats()
+
     def stats(self) -> dict:
         s = _lib.OpsetStats()
         check(lib().orbit_opset_stats_get(self.handle, C.byref(s)))

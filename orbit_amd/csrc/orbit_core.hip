@@ -211,6 +211,7 @@ struct SweepArgs {
     int sweep_idx;
     int colour;         // 0 red ((i + j) even), 1 black
     int px;             // periodic x (wrap couplings; nx even keeps the colouring)
+    float omega;        // relaxation of the correction (1: Gauss-Seidel)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
@@ -377,7 +378,7 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     CT dx = 0;
 #pragma unroll
     for (int m = 0; m < 16; ++m) dx += (CT)ldv(prow[m]) * sr[(grp << 4) + m];
-    if (on) x[base] = (XT)(xo + dx);
+    if (on) x[base] = (XT)(xo + (CT)a.omega * dx);
 }
 
 // y = A x (one operator apply, FP64), used for residual checks of the inner solves.
@@ -812,7 +813,7 @@ extern "C" void* orbit_ctx_stream(orbit_ctx* c) { return c ? (void*)c->stream : 
 // ------------------------------------------------------------------------- //
 // opset construction
 // ------------------------------------------------------------------------- //
-int calibrate_opset(orbit_ctx* ctx, orbit_opset* op);
+int calibrate_opset(orbit_ctx* ctx, orbit_opset* op, double omega);
 
 namespace orbit {
 // Default sweep launch shapes: reduced-precision sweeps of four or more slots
@@ -895,7 +896,7 @@ int finalize_opset(orbit_ctx* ctx, orbit_opset* op) {
     op->ncols_c[1] = op->ncol / 2;
     default_shapes(op);
     ORBIT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    return calibrate_opset(ctx, op);
+    return calibrate_opset(ctx, op, 0.0);
 }
 
 }  // namespace orbit
@@ -1135,7 +1136,31 @@ extern "C" int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st) {
     st->default_inner_sweeps = op->def_inner; st->default_acc_sweeps = op->def_acc;
     st->dtau = op->dtau; st->device_bytes = op->bytes; st->sweep_contraction = op->rho;
     st->periodic_x = op->px;
+    st->relaxation = op->omega;
     return ORBIT_OK;
+}
+
+namespace {
+void drop_graphs(orbit_opset* op);
+}  // namespace
+
+extern "C" int orbit_opset_set_relaxation(orbit_ctx* ctx, orbit_opset* op, double omega) {
+    if (!ctx || !op) return fail(ORBIT_ERR_INVALID, "set_relaxation: NULL argument");
+    if (!(omega == 0.0 || (omega >= 0.5 && omega <= 1.5)))
+        return fail(ORBIT_ERR_INVALID, "set_relaxation: omega must be 0 (choose) or in [0.5, 1.5]");
+    hipSetDevice(ctx->device);
+    const double old_w = op->omega;
+    const double old_rho = op->rho;
+    const int old_acc = op->def_acc, old_inner = op->def_inner;
+    drop_graphs(op);  // captured sweeps carry the old relaxation
+    int rc = calibrate_opset(ctx, op, omega);
+    if (rc) {  // keep the opset usable as it was
+        op->omega = old_w;
+        op->rho = old_rho;
+        op->def_acc = old_acc;
+        op->def_inner = old_inner;
+    }
+    return rc;
 }
 
 extern "C" int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const double* D) {
@@ -1171,7 +1196,11 @@ extern "C" int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const do
     }
     for (auto& kv : op->graphs) hipGraphExecDestroy(kv.second);
     op->graphs.clear();
-    return calibrate_opset(ctx, op);
+    // new diagonals: re-measure at the current relaxation (choose again only if
+    // the sweeps stopped contracting with it)
+    int rc = calibrate_opset(ctx, op, op->omega);
+    if (rc == ORBIT_ERR_INVALID) rc = calibrate_opset(ctx, op, 0.0);
+    return rc;
 }
 
 // ------------------------------------------------------------------------- //
@@ -1246,6 +1275,7 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     a.nx = op->nx;
     a.ny = op->ny;
     a.px = op->px;
+    a.omega = (float)op->omega;
     a.vstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 256;
     a.hstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 64;
     a.dstride = op->n16;
@@ -1297,73 +1327,128 @@ int inner_solve(const Batch& B, int bin, int dir, int prec, int sweeps, const vo
 
 }  // namespace
 
-// Contraction of one red-black sweep, measured on every bin of tracer 0 with a
-// fixed smooth right-hand side: rho = (||r_8|| / ||r_3||)^(1/5).  The sweep
-// counts of the solves follow from it: 1e-12 for an accurate apply (20% margin
-// on the rate), 1e-5 for the inexact FP32 one.
-int calibrate_opset(orbit_ctx* ctx, orbit_opset* op) {
+// Largest contraction of one red-black sweep with relaxation op->omega over
+// every bin and the sample tracers tr[0..ns) (device list), with a fixed smooth
+// right-hand side: rho = (||r_8|| / ||r_3||)^(1/5).  A bin / tracer whose
+// residual after 3 sweeps is already at round-off (a single column, or no
+// horizontal coupling: the column solve is exact) says nothing about the rate
+// -- its r8/r3 would be round-off noise near 1 -- and is skipped; 0 when all are.
+int measure_contraction(orbit_ctx* ctx, orbit_opset* op, int ns, int* tr, const double* b, double bnorm,
+                        double* x, double* y, double* red, double* out, double& rmax) {
     hipStream_t st = ctx->stream;
     const size_t n16 = op->n16;
     const int nb = (int)((n16 + RED_CHUNK - 1) / RED_CHUNK);
     const int nbins = op->nbins;
-    int rc = ctx->ensure(7, (3 * n16 + (size_t)nb + 2 * (size_t)nbins + 8) * 8 + 64);
-    if (rc) return rc;
-    double* b = ctx->buf<double>(7);
-    double* x = b + n16;
-    double* y = b + 2 * n16;
-    double* red = b + 3 * n16;
-    double* out = red + nb;
-    int* tmap = (int*)(out + 2 * nbins + 4);
-    std::vector<double> hb(n16, 0.0);
-    for (int c = 0; c < op->ncol; ++c)
-        for (int k = 0; k < op->nz; ++k) hb[(size_t)c * 16 + k] = 1.0 + 0.5 * std::sin(0.37 * c + 1.3 * k);
-    const int zero = 0;
-    { int rc0 = h2d_sync(st, tmap, &zero, 4); if (rc0) return rc0; }
-    ORBIT_HIP_CHECK(hipMemcpyAsync(b, hb.data(), n16 * 8, hipMemcpyHostToDevice, st));
-    Batch B{ctx, op, 1, tmap, nullptr, nullptr};
+    Batch B{ctx, op, ns, tr, nullptr, nullptr};
     const long long vstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 256;
     const long long hstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 64;
+    const long long xs = (long long)n16;
     for (int bin = 0; bin < nbins; ++bin) {
         const BinOps& O = op->bins[bin];
         for (int k = 1; k <= 8; ++k) {
-            rc = launch_halfsweep(B, bin, FWD, SP_F64, 0, b, (long long)n16, x, (long long)n16, k == 1, k == 1, nullptr);
-            if (!rc) rc = launch_halfsweep(B, bin, FWD, SP_F64, 1, b, (long long)n16, x, (long long)n16, k == 1, false, nullptr);
+            int rc = launch_halfsweep(B, bin, FWD, SP_F64, 0, b, 0, x, xs, k == 1, k == 1, nullptr);
+            if (!rc) rc = launch_halfsweep(B, bin, FWD, SP_F64, 1, b, 0, x, xs, k == 1, false, nullptr);
             if (rc) return rc;
             if (k == 3 || k == 8) {
-                hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, 1), dim3(256), 0, st, O.V64, vstride, O.H64,
-                                   hstride, O.d64, (long long)n16, x, y, (long long)n16, tmap, op->ncol, op->nx, op->ny,
-                                   op->px);
-                hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, 1), dim3(256), 0, st, b, (long long)n16, y,
-                                   (long long)n16, -1.0, y, (long long)n16, (int)n16, nullptr);
-                hipLaunchKernelGGL(k_norm2, dim3(nb, 1), dim3(RED_THREADS), 0, st, y, (long long)n16, (int)n16, nb, red);
-                hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, red, nb, out + 2 * bin + (k == 8), 1);
+                hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, ns), dim3(256), 0, st, O.V64, vstride, O.H64,
+                                   hstride, O.d64, (long long)n16, x, y, xs, tr, op->ncol, op->nx, op->ny, op->px);
+                hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, ns), dim3(256), 0, st, b, 0LL, y, xs, -1.0, y, xs,
+                                   (int)n16, nullptr);
+                hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, y, xs, (int)n16, nb, red);
+                hipLaunchKernelGGL(k_sum_partials, grid_waves(ns), dim3(256), 0, st, red, nb,
+                                   out + (size_t)(2 * bin + (k == 8)) * ns, ns);
             }
         }
     }
     ORBIT_HIP_CHECK(hipGetLastError());
-    std::vector<double> ho(2 * nbins);
-    ORBIT_HIP_CHECK(hipMemcpyAsync(ho.data(), out, 2 * nbins * 8, hipMemcpyDeviceToHost, st));
+    std::vector<double> ho((size_t)2 * nbins * ns);
+    ORBIT_HIP_CHECK(hipMemcpyAsync(ho.data(), out, ho.size() * 8, hipMemcpyDeviceToHost, st));
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
-    // A bin whose residual after 3 sweeps is already at round-off (a single
-    // column, or no horizontal coupling: the column solve is exact) says nothing
-    // about the rate; its r8/r3 would be round-off noise near 1.
+    rmax = 0.0;
+    for (int bin = 0; bin < nbins; ++bin)
+        for (int q = 0; q < ns; ++q) {
+            const double r3 = std::sqrt(ho[(size_t)(2 * bin) * ns + q]), r8 = std::sqrt(ho[(size_t)(2 * bin + 1) * ns + q]);
+            if (!(r3 > 1e-12 * bnorm) || !(r8 > 0.0)) continue;  // converged to round-off already
+            rmax = std::max(rmax, std::pow(r8 / r3, 0.2));
+        }
+    return ORBIT_OK;
+}
+
+namespace {
+constexpr int CAL_SLOTS = 8;  // tracers sampled by the sweep calibration
+
+// Sweep counts from a measured contraction: 1e-12 for an accurate apply (20%
+// margin on the rate), 1e-5 for the inexact FP32 one.  The inexact apply sets
+// how much one refinement round gains, not the accuracy of the answer (every
+// round's residual is an accurate FP64 sweep): 1e-5 per apply keeps the round
+// counts of 1e-6 at fewer sweeps (measured on the 64x70x15 FCT passes: 5
+// sweeps, same rounds, 12% less time than 7).
+void set_sweep_counts(orbit_opset* op, double rmax) {
+    op->rho = std::min(0.97, std::max(0.02, 1.2 * rmax));
+    op->def_acc = std::min(80, std::max(4, (int)std::ceil(std::log(1e-12) / std::log(op->rho))));
+    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(1e-5) / std::log(std::min(0.97, rmax)))));
+}
+}  // namespace
+
+// Calibrates the sweeps of an opset.  omega > 0: that relaxation; omega = 0:
+// choose it.  The correction-form sweep x += omega P (b - A x) keeps A x = b as
+// its fixed point for any omega, so omega changes only the contraction: with
+// the complex spectrum of the advective block Jacobi matrix it falls from
+// omega = 1 (Gauss-Seidel) to a minimum a few hundredths above 1 and rises
+// steeply after it (64x70x15: 0.099 at 1, 0.063 at 1.02, 0.088 at 1.03, 0.15 at
+// 1.08).  The search measures 1, 1.02, 1.04, 1.06, 1.08, then +-0.01 around the
+// best, and keeps omega != 1 only when it contracts at least 5% better.
+int calibrate_opset(orbit_ctx* ctx, orbit_opset* op, double omega) {
+    hipStream_t st = ctx->stream;
+    const size_t n16 = op->n16;
+    const int nb = (int)((n16 + RED_CHUNK - 1) / RED_CHUNK);
+    const int nbins = op->nbins;
+    const int ns = std::min(op->ntr, CAL_SLOTS);
+    int rc = ctx->ensure(7, ((1 + 2 * (size_t)ns) * n16 + (size_t)ns * nb + 2 * (size_t)nbins * ns + 8) * 8 +
+                                (size_t)ns * 4 + 64);
+    if (rc) return rc;
+    double* b = ctx->buf<double>(7);
+    double* x = b + n16;
+    double* y = x + (size_t)ns * n16;
+    double* red = y + (size_t)ns * n16;
+    double* out = red + (size_t)ns * nb;
+    int* tr = (int*)(out + 2 * (size_t)nbins * ns + 4);
+    std::vector<double> hb(n16, 0.0);
+    for (int c = 0; c < op->ncol; ++c)
+        for (int k = 0; k < op->nz; ++k) hb[(size_t)c * 16 + k] = 1.0 + 0.5 * std::sin(0.37 * c + 1.3 * k);
     double bnorm = 0.0;
     for (double v : hb) bnorm += v * v;
     bnorm = std::sqrt(bnorm);
-    double rmax = 0.0;
-    for (int bin = 0; bin < nbins; ++bin) {
-        const double r3 = std::sqrt(ho[2 * bin]), r8 = std::sqrt(ho[2 * bin + 1]);
-        if (!(r3 > 1e-12 * bnorm) || !(r8 > 0.0)) continue;  // converged to round-off already
-        rmax = std::max(rmax, std::pow(r8 / r3, 0.2));
+    std::vector<int> ht(ns);
+    for (int q = 0; q < ns; ++q) ht[q] = (int)((long long)q * op->ntr / ns);  // spread over the tracers
+    { int rc0 = h2d_sync(st, tr, ht.data(), ns * 4); if (rc0) return rc0; }
+    ORBIT_HIP_CHECK(hipMemcpyAsync(b, hb.data(), n16 * 8, hipMemcpyHostToDevice, st));
+    auto eval = [&](double w, double& r) {
+        op->omega = w;
+        return measure_contraction(ctx, op, ns, tr, b, bnorm, x, y, red, out, r);
+    };
+    double best_w = omega > 0.0 ? omega : 1.0, best = 0.0;
+    rc = eval(best_w, best);
+    if (rc) return rc;
+    if (omega <= 0.0 && best < 1.0) {
+        const double gs = best;
+        for (double w : {1.02, 1.04, 1.06, 1.08}) {
+            double r = 0.0;
+            if ((rc = eval(w, r))) return rc;
+            if (r < best) best = r, best_w = w;
+        }
+        const double c = best_w;
+        for (double w : {c - 0.01, c + 0.01}) {
+            if (w < 1.0) continue;
+            double r = 0.0;
+            if ((rc = eval(w, r))) return rc;
+            if (r < best) best = r, best_w = w;
+        }
+        if (!(best < 0.95 * gs)) best = gs, best_w = 1.0;
     }
-    if (!(rmax < 1.0)) return fail(ORBIT_ERR_INVALID, "opset: red-black sweeps do not contract on this operator");
-    op->rho = std::min(0.97, std::max(0.02, 1.2 * rmax));
-    op->def_acc = std::min(80, std::max(4, (int)std::ceil(std::log(1e-12) / std::log(op->rho))));
-    // The inexact FP32 apply sets how much one refinement round gains, not the
-    // accuracy of the answer (every round's residual is an accurate FP64 sweep):
-    // 1e-5 per apply keeps the round counts of 1e-6 at fewer sweeps (measured on
-    // the 64x70x15 FCT passes: 5 sweeps, same rounds, 12% less time than 7).
-    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(1e-5) / std::log(std::min(0.97, rmax)))));
+    op->omega = best_w;
+    if (!(best < 1.0)) return fail(ORBIT_ERR_INVALID, "opset: red-black sweeps do not contract on this operator");
+    set_sweep_counts(op, best);
     return ORBIT_OK;
 }
 

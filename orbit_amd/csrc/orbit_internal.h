@@ -125,6 +125,7 @@ struct Opset {
     bool has_tdiag = false;
     int def_inner = 6, def_acc = 14;
     double rho = 0.1;  // calibrated contraction of one red-black sweep (with margin)
+    double omega = 1.0;  // relaxation of the sweeps' correction (chosen by the calibration)
     std::vector<BinOps> bins;
     int ncols_c[2] = {0, 0};
     std::vector<void*> owned;

@@ -1,0 +1,66 @@
+"""Over-relaxed red-black sweeps: x += omega P (b - A x).
+
+A x = b is the fixed point for any omega, so the relaxation an operator set
+chooses at creation changes the sweep counts, not the answer: the orbit with
+the chosen omega and with plain block Gauss-Seidel (omega = 1) agree to the
+solve tolerance and both match the CPU oracle; choosing again is deterministic
+(bit-identical solves); an omega outside [0.5, 1.5] is refused and leaves the
+operator set as it was."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import orbit_ref as orb
+from orbit_amd import synth
+from orbit_amd._lib import OrbitError
+from orbit_amd.core import OperatorSet
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def test_relaxation_changes_counts_not_the_orbit():
+    wl = synth.make_workload(12, 10, 6, 3, seed=81, convection=True, wind_std=8.0)
+    op = OperatorSet.from_workload(wl)
+    st = op.stats()
+    w = st["relaxation"]
+    assert 1.0 <= w <= 1.09
+    a, ia = op.solve_forward(wl.e, tol=1e-11, precision="fp64")
+    am, _ = op.solve_forward(wl.e, tol=1e-8, precision="mixed")
+    gs = op.set_relaxation(1.0)
+    assert gs["relaxation"] == 1.0
+    if w != 1.0:  # chosen only when it contracts >= 5% better
+        assert gs["sweep_contraction"] > st["sweep_contraction"]
+        assert gs["default_acc_sweeps"] >= st["default_acc_sweeps"]
+    b, ib = op.solve_forward(wl.e, tol=1e-11, precision="fp64")
+    for s in range(wl.n_tracers):
+        assert ia[s]["converged"] and ib[s]["converged"]
+        assert _rel(a[s], b[s]) < 1e-9, s
+        lus = [orb.factor_one_bin(wl.L(t, s)) for t in range(8)]
+        ref, _ = orb.solve_orbit_one_species(lus, list(wl.e[s]), tol=1e-12, maxiter=500)
+        assert _rel(a[s, 0], ref[0]) < 1e-8 and _rel(b[s, 0], ref[0]) < 1e-8, s
+        assert _rel(am[s, 0], ref[0]) < 1e-6, s
+    # choosing again is deterministic
+    again = op.set_relaxation(0.0)
+    assert again["relaxation"] == w and again["default_acc_sweeps"] == st["default_acc_sweeps"]
+    c, _ = op.solve_forward(wl.e, tol=1e-11, precision="fp64")
+    assert np.array_equal(a, c)
+    # out of range: refused, the opset unchanged
+    with pytest.raises(OrbitError):
+        op.set_relaxation(2.0)
+    assert op.stats()["relaxation"] == w
+    d, _ = op.solve_forward(wl.e, tol=1e-11, precision="fp64")
+    assert np.array_equal(a, d)
+
+
+def test_adjoint_with_chosen_relaxation_matches_oracle():
+    wl = synth.make_workload(12, 10, 6, 2, seed=82, convection=True)
+    op = OperatorSet.from_workload(wl)
+    R = np.random.default_rng(5).random((2, 8, wl.N))
+    lam, info = op.solve_adjoint(R, tol=1e-11, precision="fp64")
+    for s in range(2):
+        lus = [orb.factor_one_bin(wl.L(t, s)) for t in range(8)]
+        ref, _ = orb.solve_orbit_adjoint_one_species(lus, list(R[s]), tol=1e-12, maxiter=500)
+        assert info[s]["converged"] and _rel(lam[s, 0], ref[0]) < 1e-8, s
