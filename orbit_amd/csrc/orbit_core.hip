@@ -1378,15 +1378,16 @@ namespace {
 constexpr int CAL_SLOTS = 8;  // tracers sampled by the sweep calibration
 
 // Sweep counts from a measured contraction: 1e-12 for an accurate apply (20%
-// margin on the rate), 1e-5 for the inexact FP32 one.  The inexact apply sets
+// margin on the rate), 3e-5 for the inexact FP32 one.  The inexact apply sets
 // how much one refinement round gains, not the accuracy of the answer (every
-// round's residual is an accurate FP64 sweep): 1e-5 per apply keeps the round
-// counts of 1e-6 at fewer sweeps (measured on the 64x70x15 FCT passes: 5
-// sweeps, same rounds, 12% less time than 7).
+// round's residual is an accurate FP64 sweep).  Measured on the 64x70x15 FCT
+// passes: an inexact apply good to 1.6e-5 (4 sweeps at rho 0.063) or better
+// keeps the refinement rounds of a more accurate one at fewer sweeps (-6% step
+// time against 5 sweeps), while 9.5e-5 (4 sweeps at rho 0.099) adds rounds.
 void set_sweep_counts(orbit_opset* op, double rmax) {
     op->rho = std::min(0.97, std::max(0.02, 1.2 * rmax));
     op->def_acc = std::min(80, std::max(4, (int)std::ceil(std::log(1e-12) / std::log(op->rho))));
-    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(1e-5) / std::log(std::min(0.97, rmax)))));
+    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(3e-5) / std::log(std::min(0.97, rmax)))));
 }
 }  // namespace
 
