@@ -1397,8 +1397,10 @@ void set_sweep_counts(orbit_opset* op, double rmax) {
 // the complex spectrum of the advective block Jacobi matrix it falls from
 // omega = 1 (Gauss-Seidel) to a minimum a few hundredths above 1 and rises
 // steeply after it (64x70x15: 0.099 at 1, 0.063 at 1.02, 0.088 at 1.03, 0.15 at
-// 1.08).  The search measures 1, 1.02, 1.04, 1.06, 1.08, then +-0.01 around the
-// best, and keeps omega != 1 only when it contracts at least 5% better.
+// 1.08).  The search measures 1, 1.02, 1.04, 1.06, 1.08, then +-0.01, +-0.005
+// and +-0.0025 around the running best (48x40x15: 0.060 at 1.015 against 0.071
+// at 1.02, one accurate sweep fewer), and keeps omega != 1 only when it
+// contracts at least 5% better.
 int calibrate_opset(orbit_ctx* ctx, orbit_opset* op, double omega) {
     hipStream_t st = ctx->stream;
     const size_t n16 = op->n16;
@@ -1438,12 +1440,14 @@ int calibrate_opset(orbit_ctx* ctx, orbit_opset* op, double omega) {
             if ((rc = eval(w, r))) return rc;
             if (r < best) best = r, best_w = w;
         }
-        const double c = best_w;
-        for (double w : {c - 0.01, c + 0.01}) {
-            if (w < 1.0) continue;
-            double r = 0.0;
-            if ((rc = eval(w, r))) return rc;
-            if (r < best) best = r, best_w = w;
+        for (double h : {0.01, 0.005, 0.0025}) {  // the minimum is sharp: narrow in on it
+            const double c = best_w;
+            for (double w : {c - h, c + h}) {
+                if (w < 1.0) continue;
+                double r = 0.0;
+                if ((rc = eval(w, r))) return rc;
+                if (r < best) best = r, best_w = w;
+            }
         }
         if (!(best < 0.95 * gs)) best = gs, best_w = 1.0;
     }
