@@ -67,37 +67,17 @@ __global__ void k_fct_x(FctGrid g, const double* __restrict__ c, long long cs, d
     x[(size_t)blockIdx.y * xs + n] = c[(size_t)blockIdx.y * cs + n] / rho_at(g, n);
 }
 
-// x face between (k,j,i) [lo] and (k,j,i+1) [hi].  xat(j, i): x of an
-// in-domain cell of level k; gat(j, i): x of the ghost there (read only when
-// has_g).  The global-memory kernels and the LDS-tiled one share this code, so
-// a face's arithmetic is the same expression whichever kernel evaluates it.
-template <class XA, class GA>
-__device__ __forceinline__ Face face_x_t(const FctGrid& g, XA xat, GA gat, bool has_g, int k, int j, int i) {
-    const int nx = g.nx;
-    const size_t row = ((size_t)k * g.ny + j) * nx;
-    const double xl = xat(j, i), xh = xat(j, i + 1);
-    double xll, xhh;
-    if (i > 0) xll = xat(j, i - 1);
-    else xll = has_g ? gat(j, 0) : xl;
-    if (i + 2 < nx) xhh = xat(j, i + 2);
-    else xhh = has_g ? gat(j, nx - 1) : xh;
-    double phl = vl_phi(xl, xll, xh), phh = vl_phi(xh, xhh, xl);
-    if (!has_g) {
-        if (i == 0) phl = 0.0;
-        if (i == nx - 2) phh = 0.0;
-    }
-    const size_t fi = row + i + 1;  // UAvg index of the face west of cell i+1
-    double sp, sm;
-    if (g.split) {
-        sp = g.up[fi];
-        sm = g.um[fi];
-    } else {
-        sp = fmax(g.up[fi], 0.0);
-        sm = fmax(-g.up[fi], 0.0);
-    }
-    Face f;
-    f.beta = 0.5 * (sp * phl + sm * phh);
-    f.A = f.beta * (xh - xl);
+// The tracer-independent part of a face: its terrain weights (fct.py:330-345),
+// whether both cells have air, and 1 / its width.
+struct FaceGeo {
+    double tr_lo, tr_hi, inv;
+    bool valid;
+};
+
+// x face between (k,j,i) [lo] and (k,j,i+1) [hi]
+__device__ __forceinline__ FaceGeo face_geo_x(const FctGrid& g, int k, int j, int i) {
+    const size_t row = ((size_t)k * g.ny + j) * g.nx;
+    FaceGeo f;
     const double dl = g.dP[row + i], dh = g.dP[row + i + 1];
     const double df = 0.5 * (dl + dh);
     f.valid = dl > 0.0 && dh > 0.0;
@@ -109,10 +89,58 @@ __device__ __forceinline__ Face face_x_t(const FctGrid& g, XA xat, GA gat, bool 
 }
 
 // y face between (k,j,i) [lo = south] and (k,j+1,i) [hi = north]
+__device__ __forceinline__ FaceGeo face_geo_y(const FctGrid& g, int k, int j, int i) {
+    const size_t nl = ((size_t)k * g.ny + j) * g.nx + i, nh = nl + g.nx;
+    FaceGeo f;
+    const double dl = g.dP[nl], dh = g.dP[nh];
+    const double df = 0.5 * (dl + dh);
+    f.valid = dl > 0.0 && dh > 0.0;
+    const double dxl = g.dx[j], dxh = g.dx[j + 1], dxf = 0.5 * (dxl + dxh);
+    const double wl = (dxf / (dxl > 0.0 ? dxl : 1.0)) * (df / (dl > 0.0 ? dl : 1.0));
+    const double wh = (dxf / (dxh > 0.0 ? dxh : 1.0)) * (df / (dh > 0.0 ? dh : 1.0));
+    f.tr_lo = f.valid ? wl : 1.0;
+    f.tr_hi = f.valid ? wh : 1.0;
+    f.inv = g.dy > 0.0 ? 1.0 / g.dy : 1.0;
+    return f;
+}
+
+// The flux part of the x face (lo = (k,j,i)): van Leer reconstruction gain
+// beta and anti-diffusive flux A.  xat(j, i): x of an in-domain cell of level
+// k; gat(j, i): x of the ghost there (read only when has_g).  The per-cell
+// kernels and the LDS-tiled one share this code, so a face's arithmetic is the
+// same expression whichever kernel evaluates it.
 template <class XA, class GA>
-__device__ __forceinline__ Face face_y_t(const FctGrid& g, XA xat, GA gat, bool has_g, int k, int j, int i) {
+__device__ __forceinline__ void face_flux_x(const FctGrid& g, XA xat, GA gat, bool has_g, int k, int j, int i,
+                                            double& beta, double& A) {
+    const int nx = g.nx;
+    const double xl = xat(j, i), xh = xat(j, i + 1);
+    double xll, xhh;
+    if (i > 0) xll = xat(j, i - 1);
+    else xll = has_g ? gat(j, 0) : xl;
+    if (i + 2 < nx) xhh = xat(j, i + 2);
+    else xhh = has_g ? gat(j, nx - 1) : xh;
+    double phl = vl_phi(xl, xll, xh), phh = vl_phi(xh, xhh, xl);
+    if (!has_g) {
+        if (i == 0) phl = 0.0;
+        if (i == nx - 2) phh = 0.0;
+    }
+    const size_t fi = ((size_t)k * g.ny + j) * nx + i + 1;  // UAvg index of the face west of cell i+1
+    double sp, sm;
+    if (g.split) {
+        sp = g.up[fi];
+        sm = g.um[fi];
+    } else {
+        sp = fmax(g.up[fi], 0.0);
+        sm = fmax(-g.up[fi], 0.0);
+    }
+    beta = 0.5 * (sp * phl + sm * phh);
+    A = beta * (xh - xl);
+}
+
+template <class XA, class GA>
+__device__ __forceinline__ void face_flux_y(const FctGrid& g, XA xat, GA gat, bool has_g, int k, int j, int i,
+                                            double& beta, double& A) {
     const int nx = g.nx, ny = g.ny;
-    const size_t lvl = (size_t)k * ny * nx;
     const double xl = xat(j, i), xh = xat(j + 1, i);
     double xll, xhh;
     if (j > 0) xll = xat(j - 1, i);
@@ -124,7 +152,7 @@ __device__ __forceinline__ Face face_y_t(const FctGrid& g, XA xat, GA gat, bool 
         if (j == 0) phl = 0.0;
         if (j == ny - 2) phh = 0.0;
     }
-    const size_t fi = lvl + (size_t)(j + 1) * nx + i;
+    const size_t fi = ((size_t)k * ny + j + 1) * nx + i;
     double sp, sm;
     if (g.split) {
         sp = g.vp[fi];
@@ -133,20 +161,86 @@ __device__ __forceinline__ Face face_y_t(const FctGrid& g, XA xat, GA gat, bool 
         sp = fmax(g.vp[fi], 0.0);
         sm = fmax(-g.vp[fi], 0.0);
     }
+    beta = 0.5 * (sp * phl + sm * phh);
+    A = beta * (xh - xl);
+}
+
+template <class XA, class GA>
+__device__ __forceinline__ Face face_x_t(const FctGrid& g, XA xat, GA gat, bool has_g, int k, int j, int i) {
     Face f;
-    f.beta = 0.5 * (sp * phl + sm * phh);
-    f.A = f.beta * (xh - xl);
-    const size_t nl = lvl + (size_t)j * nx + i, nh = nl + nx;
-    const double dl = g.dP[nl], dh = g.dP[nh];
-    const double df = 0.5 * (dl + dh);
-    f.valid = dl > 0.0 && dh > 0.0;
-    const double dxl = g.dx[j], dxh = g.dx[j + 1], dxf = 0.5 * (dxl + dxh);
-    const double wl = (dxf / (dxl > 0.0 ? dxl : 1.0)) * (df / (dl > 0.0 ? dl : 1.0));
-    const double wh = (dxf / (dxh > 0.0 ? dxh : 1.0)) * (df / (dh > 0.0 ? dh : 1.0));
-    f.tr_lo = f.valid ? wl : 1.0;
-    f.tr_hi = f.valid ? wh : 1.0;
-    f.inv = g.dy > 0.0 ? 1.0 / g.dy : 1.0;
+    face_flux_x(g, xat, gat, has_g, k, j, i, f.beta, f.A);
+    const FaceGeo q = face_geo_x(g, k, j, i);
+    f.tr_lo = q.tr_lo;
+    f.tr_hi = q.tr_hi;
+    f.inv = q.inv;
+    f.valid = q.valid;
     return f;
+}
+
+template <class XA, class GA>
+__device__ __forceinline__ Face face_y_t(const FctGrid& g, XA xat, GA gat, bool has_g, int k, int j, int i) {
+    Face f;
+    face_flux_y(g, xat, gat, has_g, k, j, i, f.beta, f.A);
+    const FaceGeo q = face_geo_y(g, k, j, i);
+    f.tr_lo = q.tr_lo;
+    f.tr_hi = q.tr_hi;
+    f.inv = q.inv;
+    f.valid = q.valid;
+    return f;
+}
+
+// Per-bin geometry of the tiled source, tracer- and pass-independent: per x
+// face (k, j, i < nx-1) and per y face (k, j < ny-1, i) tr_lo, tr_hi and
+// valid (1 / 0); 1/dx per row; rho per cell.  Each value is face_geo_x /
+// face_geo_y / rho_at itself, computed once instead of per tracer and pass.
+struct FctGeo {
+    const double* gx;
+    const double* gy;
+    const double* invx;
+    const double* rho;
+};
+
+inline size_t geo_doubles(int nz, int ny, int nx) {
+    const size_t nfx = nx > 1 ? (size_t)nz * ny * (nx - 1) : 0, nfy = ny > 1 ? (size_t)nz * (ny - 1) * nx : 0;
+    return 3 * nfx + 3 * nfy + (size_t)ny + (size_t)nz * ny * nx;
+}
+
+__global__ void k_fct_geometry(FctGrid g, double* __restrict__ gx, double* __restrict__ gy,
+                               double* __restrict__ invx, double* __restrict__ rho) {
+    const size_t N = (size_t)g.nz * g.ny * g.nx;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int i = (int)(n % g.nx), j = (int)((n / g.nx) % g.ny), k = (int)(n / ((size_t)g.nx * g.ny));
+    rho[n] = rho_at(g, n);
+    if (g.nx > 1 && i < g.nx - 1) {
+        const FaceGeo q = face_geo_x(g, k, j, i);
+        const size_t f = 3 * (((size_t)k * g.ny + j) * (g.nx - 1) + i);
+        gx[f] = q.tr_lo;
+        gx[f + 1] = q.tr_hi;
+        gx[f + 2] = q.valid ? 1.0 : 0.0;
+    }
+    if (g.ny > 1 && j < g.ny - 1) {
+        const FaceGeo q = face_geo_y(g, k, j, i);
+        const size_t f = 3 * (((size_t)k * (g.ny - 1) + j) * g.nx + i);
+        gy[f] = q.tr_lo;
+        gy[f + 1] = q.tr_hi;
+        gy[f + 2] = q.valid ? 1.0 : 0.0;
+    }
+    if (k == 0 && i == 0) {
+        const double dxf = g.dx[j];
+        invx[j] = 1.0 / (dxf > 0.0 ? dxf : 1.0);
+    }
+}
+
+// Carve the geometry arrays out of one allocation of geo_doubles() doubles.
+inline FctGeo geo_view(double* p, int nz, int ny, int nx) {
+    const size_t nfx = nx > 1 ? (size_t)nz * ny * (nx - 1) : 0, nfy = ny > 1 ? (size_t)nz * (ny - 1) * nx : 0;
+    FctGeo v;
+    v.gx = p;
+    v.gy = p + 3 * nfx;
+    v.invx = p + 3 * nfx + 3 * nfy;
+    v.rho = p + 3 * nfx + 3 * nfy + ny;
+    return v;
 }
 
 // The same faces on x (and the ghost's x) stored per cell in global memory.
@@ -253,7 +347,7 @@ constexpr int XF_W = FT_I + 3, XF_H = FT_J + 2;  // x faces: lo i in [i0-2, i0+F
 constexpr int YF_W = FT_I + 2, YF_H = FT_J + 3;  // y faces: lo j in [j0-2, j0+FT_J], i in [i0-1, i0+FT_I]
 constexpr int RR_W = FT_I + 2, RR_H = FT_J + 2;  // R: the tile and its 1-cell ring
 
-__global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, const double* __restrict__ c, long long cs,
+__global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const double* __restrict__ c, long long cs,
                                                   const double* __restrict__ cg, double dt, int limited,
                                                   double* __restrict__ d, long long ds,
                                                   const double* __restrict__ base, long long bs) {
@@ -275,32 +369,31 @@ __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, const double* __res
         const int j = j0 - FT_H + jj, i = i0 - FT_H + ii;
         if (j >= 0 && j < ny && i >= 0 && i < nx) {
             const size_t n = lvl + (size_t)j * nx + i;
-            sx[jj][ii] = cv[n] / rho_at(g, n);
+            sx[jj][ii] = cv[n] / geo.rho[n];
         }
     }
     if (threadIdx.x < XF_H) {
         const int j = j0 - 1 + threadIdx.x;
-        if (j >= 0 && j < ny) {
-            const double dxf = g.dx[j];
-            sinvx[threadIdx.x] = 1.0 / (dxf > 0.0 ? dxf : 1.0);
-        }
+        if (j >= 0 && j < ny) sinvx[threadIdx.x] = geo.invx[j];
     }
     __syncthreads();
     auto xat = [&](int j, int i) { return sx[j - j0 + FT_H][i - i0 + FT_H]; };
     auto gat = [&](int j, int i) {
         const size_t n = lvl + (size_t)j * nx + i;
-        return gv[n] / rho_at(g, n);
+        return gv[n] / geo.rho[n];
     };
     if (nx > 1)
         for (int t = threadIdx.x; t < XF_H * XF_W; t += blockDim.x) {
             const int fj = t / XF_W, fi = t - fj * XF_W;
             const int j = j0 - 1 + fj, i = i0 - 2 + fi;
             if (j >= 0 && j < ny && i >= 0 && i <= nx - 2) {
-                const Face f = face_x_t(g, xat, gat, has_g, k, j, i);
-                xfA[fj][fi] = f.A;
-                xfL[fj][fi] = f.tr_lo;
-                xfH[fj][fi] = f.tr_hi;
-                xfV[fj][fi] = f.valid;
+                double beta, A;
+                face_flux_x(g, xat, gat, has_g, k, j, i, beta, A);
+                const double* q = geo.gx + 3 * (((size_t)k * ny + j) * (nx - 1) + i);
+                xfA[fj][fi] = A;
+                xfL[fj][fi] = q[0];
+                xfH[fj][fi] = q[1];
+                xfV[fj][fi] = q[2] != 0.0;
             }
         }
     if (ny > 1)
@@ -308,11 +401,13 @@ __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, const double* __res
             const int fj = t / YF_W, fi = t - fj * YF_W;
             const int j = j0 - 2 + fj, i = i0 - 1 + fi;
             if (i >= 0 && i < nx && j >= 0 && j <= ny - 2) {
-                const Face f = face_y_t(g, xat, gat, has_g, k, j, i);
-                yfA[fj][fi] = f.A;
-                yfL[fj][fi] = f.tr_lo;
-                yfH[fj][fi] = f.tr_hi;
-                yfV[fj][fi] = f.valid;
+                double beta, A;
+                face_flux_y(g, xat, gat, has_g, k, j, i, beta, A);
+                const double* q = geo.gy + 3 * (((size_t)k * (ny - 1) + j) * nx + i);
+                yfA[fj][fi] = A;
+                yfL[fj][fi] = q[0];
+                yfH[fj][fi] = q[1];
+                yfV[fj][fi] = q[2] != 0.0;
             }
         }
     __syncthreads();
@@ -418,7 +513,7 @@ __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, const double* __res
             }
         }
         const size_t n = lvl + (size_t)j * nx + i;
-        const double val = acc * rho_at(g, n);
+        const double val = acc * geo.rho[n];
         d[(size_t)v * ds + n] = base ? base[(size_t)v * bs + n] + val : val;
     }
 }
@@ -628,9 +723,14 @@ int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, con
         dg = b;
         dd = o;
     }
+    double* geo = rc ? nullptr : dalloc(geo_doubles(gm->nz, gm->ny, gm->nx));
+    if (!rc && !geo) rc = fail(ORBIT_ERR_NOMEM, "fct: device allocation failed");
     if (!rc) {
-        hipLaunchKernelGGL(k_fct_tile, tile_grid(M.fg, 1), dim3(256), 0, st, M.fg, dc, 0LL, dg, dt, limited, dd, 0LL,
-                           (const double*)nullptr, 0LL);
+        const FctGeo gv = geo_view(geo, gm->nz, gm->ny, gm->nx);
+        hipLaunchKernelGGL(k_fct_geometry, g1(N), dim3(256), 0, st, M.fg, (double*)gv.gx, (double*)gv.gy,
+                           (double*)gv.invx, (double*)gv.rho);
+        hipLaunchKernelGGL(k_fct_tile, tile_grid(M.fg, 1), dim3(256), 0, st, M.fg, gv, dc, 0LL, dg, dt, limited, dd,
+                           0LL, (const double*)nullptr, 0LL);
         if (hipGetLastError() != hipSuccess) rc = fail(ORBIT_ERR_HIP, "fct: kernel launch failed");
     }
     if (!rc && host && hipMemcpyAsync(d, dd, N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -914,6 +1014,7 @@ struct orbit_fct_grid {
     orbit_ctx* ctx = nullptr;
     std::vector<void*> bufs;
     FctGrid fg{};
+    FctGeo geo{};  // in bufs
 };
 
 extern "C" int orbit_fct_grid_create(orbit_ctx* ctx, const orbit_grid_metrics* gm, orbit_fct_grid** out) {
@@ -931,7 +1032,22 @@ extern "C" int orbit_fct_grid_create(orbit_ctx* ctx, const orbit_grid_metrics* g
     }
     G->fg = M.fg;
     G->bufs.swap(M.bufs);  // ownership moves to the grid object
-    ORBIT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    // the source's geometry, once for every tracer and pass
+    void* geo = nullptr;
+    const size_t N = (size_t)gm->nz * gm->ny * gm->nx;
+    if (hipMalloc(&geo, geo_doubles(gm->nz, gm->ny, gm->nx) * 8) != hipSuccess) {
+        orbit_fct_grid_destroy(G);
+        return fail(ORBIT_ERR_NOMEM, "fct_grid: device allocation failed");
+    }
+    G->bufs.push_back(geo);
+    G->geo = geo_view((double*)geo, gm->nz, gm->ny, gm->nx);
+    hipLaunchKernelGGL(k_fct_geometry, g1(N), dim3(256), 0, ctx->stream, G->fg, (double*)G->geo.gx,
+                       (double*)G->geo.gy, (double*)G->geo.invx, (double*)G->geo.rho);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        orbit_fct_grid_destroy(G);
+        return fail(ORBIT_ERR_HIP, "fct_grid: geometry kernel failed");
+    }
     *out = G;
     return ORBIT_OK;
 }
@@ -969,7 +1085,7 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
     if (base_dev && overlaps(out_dev, span(out_stride), base_dev, span(base_stride)) &&
         !(base_dev == out_dev && base_stride == out_stride))
         return fail(ORBIT_ERR_INVALID, "fct_source_batch: base overlaps out other than as out itself");
-    hipLaunchKernelGGL(k_fct_tile, tile_grid(G->fg, nvec), dim3(256), 0, st, G->fg, c_dev, c_stride, ghost_dev, dt,
+    hipLaunchKernelGGL(k_fct_tile, tile_grid(G->fg, nvec), dim3(256), 0, st, G->fg, G->geo, c_dev, c_stride, ghost_dev, dt,
                        limited, out_dev, out_stride, base_dev, base_stride);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
