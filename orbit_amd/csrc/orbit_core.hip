@@ -1401,14 +1401,59 @@ void set_sweep_counts(orbit_opset* op, double rmax) {
 // and +-0.0025 around the running best (48x40x15: 0.060 at 1.015 against 0.071
 // at 1.02, one accurate sweep fewer), and keeps omega != 1 only when it
 // contracts at least 5% better.
+// The tracers the calibration measures (scratch slot 7): all of them when there are at most ns;
+// else the ns/2 with the smallest diagonal on bin 0 (the weakest loss
+// contracts slowest: its sweeps set the counts) and the rest spread evenly
+// over the others.
+int calibration_tracers(orbit_ctx* ctx, orbit_opset* op, int ns, std::vector<int>& ht) {
+    const int T = op->ntr;
+    ht.clear();
+    if (T <= ns) {
+        for (int t = 0; t < T; ++t) ht.push_back(t);
+        return ORBIT_OK;
+    }
+    hipStream_t st = ctx->stream;
+    const size_t n16 = op->n16;
+    const int nb = (int)((n16 + RED_CHUNK - 1) / RED_CHUNK);
+    int rc = ctx->ensure(7, ((size_t)T * nb + (size_t)T) * 8 + 64);
+    if (rc) return rc;
+    double* part = ctx->buf<double>(7);
+    double* sums = part + (size_t)T * nb;
+    hipLaunchKernelGGL(k_norm2, dim3(nb, T), dim3(RED_THREADS), 0, st, op->bins[0].d64, (long long)n16, (int)n16, nb,
+                       part);
+    hipLaunchKernelGGL(k_sum_partials, grid_waves(T), dim3(256), 0, st, part, nb, sums, T);
+    ORBIT_HIP_CHECK(hipGetLastError());
+    std::vector<double> hs(T);
+    ORBIT_HIP_CHECK(hipMemcpyAsync(hs.data(), sums, (size_t)T * 8, hipMemcpyDeviceToHost, st));
+    ORBIT_HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<int> order(T);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return hs[a] < hs[b]; });
+    const int weak = ns / 2;
+    std::vector<char> taken(T, 0);
+    for (int q = 0; q < weak; ++q) {
+        ht.push_back(order[q]);
+        taken[order[q]] = 1;
+    }
+    std::vector<int> rest;
+    for (int t = 0; t < T; ++t)
+        if (!taken[t]) rest.push_back(t);
+    const int more = ns - weak;
+    for (int q = 0; q < more; ++q) ht.push_back(rest[(size_t)q * rest.size() / more]);
+    return ORBIT_OK;
+}
+
 int calibrate_opset(orbit_ctx* ctx, orbit_opset* op, double omega) {
     hipStream_t st = ctx->stream;
     const size_t n16 = op->n16;
     const int nb = (int)((n16 + RED_CHUNK - 1) / RED_CHUNK);
     const int nbins = op->nbins;
     const int ns = std::min(op->ntr, CAL_SLOTS);
-    int rc = ctx->ensure(7, ((1 + 2 * (size_t)ns) * n16 + (size_t)ns * nb + 2 * (size_t)nbins * ns + 8) * 8 +
-                                (size_t)ns * 4 + 64);
+    std::vector<int> ht;  // first: it uses the same scratch slot
+    int rc = calibration_tracers(ctx, op, ns, ht);
+    if (rc) return rc;
+    rc = ctx->ensure(7, ((1 + 2 * (size_t)ns) * n16 + (size_t)ns * nb + 2 * (size_t)nbins * ns + 8) * 8 +
+                            (size_t)ns * 4 + 64);
     if (rc) return rc;
     double* b = ctx->buf<double>(7);
     double* x = b + n16;
@@ -1422,8 +1467,6 @@ int calibrate_opset(orbit_ctx* ctx, orbit_opset* op, double omega) {
     double bnorm = 0.0;
     for (double v : hb) bnorm += v * v;
     bnorm = std::sqrt(bnorm);
-    std::vector<int> ht(ns);
-    for (int q = 0; q < ns; ++q) ht[q] = (int)((long long)q * op->ntr / ns);  // spread over the tracers
     { int rc0 = h2d_sync(st, tr, ht.data(), ns * 4); if (rc0) return rc0; }
     ORBIT_HIP_CHECK(hipMemcpyAsync(b, hb.data(), n16 * 8, hipMemcpyHostToDevice, st));
     auto eval = [&](double w, double& r) {
