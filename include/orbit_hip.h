@@ -348,6 +348,20 @@ int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, cons
                                const double* base_dev, long long base_stride, double* out_dev,
                                long long out_stride);
 
+/* Forward FCT deferred correction on the device (reference orbit/cli.py:1212-1241,
+ * the passes of forward mode): pass 0 solves the periodic orbits of e; pass p
+ * = 1..n_passes-1 adds d_AD of the previous orbit's end-of-bin states (limited,
+ * grids[bin] of the operator set's shape and context) to e and solves again
+ * from the previous c0.  e_dev (nt, n_bins, N) and ghost_dev (nullable, the
+ * same layout: the concentration beyond the lateral edge per bin) are device
+ * arrays; orbit_dev (nt, n_bins+1, N) receives the last pass's orbits and
+ * info[p*nt + s] the solve information of pass p.  opts as for
+ * orbit_solve_forward (rhs_is_s refused).  Returns the worst status of the
+ * passes; the caller's stream is not synchronised with torch's. */
+int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids, const int* tracers,
+                          int nt, const double* e_dev, const double* ghost_dev, int n_passes,
+                          const orbit_opts* opts, double* orbit_dev, orbit_info* info);
+
 /* Frozen L_AD from a device-resident grid and a device baseline state. */
 int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,
                                orbit_fct** out);

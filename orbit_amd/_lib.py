@@ -36,7 +36,7 @@ EXPORTS = [
     "orbit_spmat_apply_dev", "orbit_spmat_destroy", "orbit_opset_autotune",
     "orbit_opset_get_shapes", "orbit_opset_set_shapes", "orbit_transport_assemble",
     "orbit_opset_create_grids", "orbit_lut_create", "orbit_lut_query_dev", "orbit_lut_destroy",
-    "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation",
+    "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation", "orbit_forward_fct_dev",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
 N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
@@ -153,6 +153,7 @@ def _declare(lib):
         "orbit_species_diagonals": (_I, [_P, _I, _I, _I, _P, _P, _P, _P]),
         "orbit_fct_source_batch_dev": (_I, [_P, _P, _I, _P, C.c_longlong, _P, _D, _I, _P, C.c_longlong,
                                             _P, C.c_longlong]),
+        "orbit_forward_fct_dev": (_I, [_P, _P, _P, _P, _I, _P, _P, _I, C.POINTER(Opts), _P, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

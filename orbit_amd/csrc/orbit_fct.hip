@@ -348,7 +348,7 @@ constexpr int YF_W = FT_I + 2, YF_H = FT_J + 3;  // y faces: lo j in [j0-2, j0+F
 constexpr int RR_W = FT_I + 2, RR_H = FT_J + 2;  // R: the tile and its 1-cell ring
 
 __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const double* __restrict__ c, long long cs,
-                                                  const double* __restrict__ cg, double dt, int limited,
+                                                  const double* __restrict__ cg, long long gs, double dt, int limited,
                                                   double* __restrict__ d, long long ds,
                                                   const double* __restrict__ base, long long bs) {
     __shared__ double sx[FX_H][FX_W];
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const d
     const int k = blockIdx.z % g.nz, v = blockIdx.z / g.nz;
     const size_t lvl = (size_t)k * ny * nx;
     const double* cv = c + (size_t)v * cs;
-    const double* gv = cg ? cg + (size_t)v * cs : nullptr;
+    const double* gv = cg ? cg + (size_t)v * gs : nullptr;
     const bool has_g = gv != nullptr;
     for (int t = threadIdx.x; t < FX_H * FX_W; t += blockDim.x) {
         const int jj = t / FX_W, ii = t - jj * FX_W;
@@ -729,8 +729,8 @@ int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, con
         const FctGeo gv = geo_view(geo, gm->nz, gm->ny, gm->nx);
         hipLaunchKernelGGL(k_fct_geometry, g1(N), dim3(256), 0, st, M.fg, (double*)gv.gx, (double*)gv.gy,
                            (double*)gv.invx, (double*)gv.rho);
-        hipLaunchKernelGGL(k_fct_tile, tile_grid(M.fg, 1), dim3(256), 0, st, M.fg, gv, dc, 0LL, dg, dt, limited, dd,
-                           0LL, (const double*)nullptr, 0LL);
+        hipLaunchKernelGGL(k_fct_tile, tile_grid(M.fg, 1), dim3(256), 0, st, M.fg, gv, dc, 0LL, dg, 0LL, dt, limited,
+                           dd, 0LL, (const double*)nullptr, 0LL);
         if (hipGetLastError() != hipSuccess) rc = fail(ORBIT_ERR_HIP, "fct: kernel launch failed");
     }
     if (!rc && host && hipMemcpyAsync(d, dd, N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -1063,10 +1063,11 @@ extern "C" int orbit_fct_grid_destroy(orbit_fct_grid* G) {
     return ORBIT_OK;
 }
 
-extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, const double* c_dev,
-                                          long long c_stride, const double* ghost_dev, double dt, int limited,
-                                          const double* base_dev, long long base_stride, double* out_dev,
-                                          long long out_stride) {
+namespace {
+// The batched source with the ghost rows on a stride of their own.
+int fct_source_batch(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, const double* c_dev, long long c_stride,
+                     const double* ghost_dev, long long ghost_stride, double dt, int limited, const double* base_dev,
+                     long long base_stride, double* out_dev, long long out_stride) {
     if (!ctx || !G || !c_dev || !out_dev || nvec < 1) return fail(ORBIT_ERR_INVALID, "fct_source_batch: bad argument");
     if (!(dt > 0.0)) return fail(ORBIT_ERR_INVALID, "fct_source_batch: dt must be positive");
     if (nvec > 65535 || (long long)nvec * G->fg.nz > 65535)
@@ -1076,21 +1077,80 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
     const size_t N = (size_t)G->fg.nz * G->fg.ny * G->fg.nx;
     // tiles read c (and the ghost) over a halo that other tiles' outputs cover:
     // the output must not share memory with them (base may be the output itself)
-    auto span = [&](long long stride) { return (size_t)(nvec - 1) * (size_t)(stride < 0 ? -stride : stride) + N; };
-    if (c_stride < 0 || out_stride < 0 || base_stride < 0)
+    auto span = [&](long long stride) { return (size_t)(nvec - 1) * (size_t)stride + N; };
+    if (c_stride < 0 || out_stride < 0 || base_stride < 0 || ghost_stride < 0)
         return fail(ORBIT_ERR_INVALID, "fct_source_batch: negative stride");
     if (overlaps(out_dev, span(out_stride), c_dev, span(c_stride)) ||
-        (ghost_dev && overlaps(out_dev, span(out_stride), ghost_dev, span(c_stride))))
+        (ghost_dev && overlaps(out_dev, span(out_stride), ghost_dev, span(ghost_stride))))
         return fail(ORBIT_ERR_INVALID, "fct_source_batch: out must not overlap c or ghost");
     if (base_dev && overlaps(out_dev, span(out_stride), base_dev, span(base_stride)) &&
         !(base_dev == out_dev && base_stride == out_stride))
         return fail(ORBIT_ERR_INVALID, "fct_source_batch: base overlaps out other than as out itself");
-    hipLaunchKernelGGL(k_fct_tile, tile_grid(G->fg, nvec), dim3(256), 0, st, G->fg, G->geo, c_dev, c_stride, ghost_dev, dt,
-                       limited, out_dev, out_stride, base_dev, base_stride);
+    hipLaunchKernelGGL(k_fct_tile, tile_grid(G->fg, nvec), dim3(256), 0, st, G->fg, G->geo, c_dev, c_stride, ghost_dev,
+                       ghost_stride, dt, limited, out_dev, out_stride, base_dev, base_stride);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
+}  // namespace
 
+extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, const double* c_dev,
+                                          long long c_stride, const double* ghost_dev, double dt, int limited,
+                                          const double* base_dev, long long base_stride, double* out_dev,
+                                          long long out_stride) {
+    return fct_source_batch(ctx, G, nvec, c_dev, c_stride, ghost_dev, c_stride, dt, limited, base_dev, base_stride,
+                            out_dev, out_stride);
+}
+
+// Forward FCT deferred correction (reference orbit/cli.py:1212-1241) with
+// every pass on the device: pass 0 solves with e; pass p adds the limited
+// anti-diffusive source of the previous orbit's end-of-bin states
+// d_AD(c[tau+1]) to e and solves again, warm-started from the previous c0.
+// The same launches as the host loop over orbit_fct_source_batch_dev and
+// orbit_solve_forward_dev, without returning to the caller between passes.
+extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids,
+                                     const int* tracers, int nt, const double* e_dev, const double* ghost_dev,
+                                     int n_passes, const orbit_opts* opts, double* orbit_dev, orbit_info* info) {
+    if (!ctx || !op || !grids || !tracers || nt < 1 || !e_dev || !orbit_dev || !info)
+        return fail(ORBIT_ERR_INVALID, "forward_fct: NULL argument or empty tracer list");
+    if (n_passes < 1) return fail(ORBIT_ERR_INVALID, "forward_fct: n_passes must be >= 1");
+    if (opts && opts->rhs_is_s) return fail(ORBIT_ERR_INVALID, "forward_fct: the passes need e, not s");
+    const int nbn = op->nbins;
+    const size_t N = (size_t)op->N;
+    for (int b = 0; b < nbn; ++b) {
+        const orbit_fct_grid* G = grids[b];
+        if (!G || (size_t)G->fg.nz * G->fg.ny * G->fg.nx != N || G->fg.nx != op->nx || G->fg.ny != op->ny)
+            return fail(ORBIT_ERR_INVALID, "forward_fct: every bin needs an FCT grid of the operator set's shape");
+        if (G->ctx != ctx) return fail(ORBIT_ERR_INVALID, "forward_fct: FCT grids and the solves must share ctx");
+    }
+    orbit_opts o;
+    orbit_opts_default(&o);
+    if (opts) o = *opts;
+    o.warm_orbit = 0;
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    int rc = orbit_solve_forward_dev(ctx, op, tracers, nt, e_dev, nullptr, &o, orbit_dev, info);
+    if (rc < 0) return rc;
+    int worst = rc;
+    if (n_passes == 1) return worst;
+    rc = ctx->ensure(8, ((size_t)nt * nbn * N + (size_t)nt * N) * 8);
+    if (rc) return rc;
+    double* e_eff = ctx->buf<double>(8);
+    double* c0 = e_eff + (size_t)nt * nbn * N;
+    const long long os = (long long)(nbn + 1) * N, es = (long long)nbn * N;
+    for (int p = 1; p < n_passes; ++p) {
+        for (int tau = 0; tau < nbn; ++tau) {
+            rc = fct_source_batch(ctx, grids[tau], nt, orbit_dev + (size_t)(tau + 1) * N, os,
+                                  ghost_dev ? ghost_dev + (size_t)tau * N : nullptr, es, op->dtau, 1,
+                                  e_dev + (size_t)tau * N, es, e_eff + (size_t)tau * N, es);
+            if (rc) return rc;
+        }
+        ORBIT_HIP_CHECK(hipMemcpy2DAsync(c0, N * 8, orbit_dev, (size_t)os * 8, N * 8, nt, hipMemcpyDeviceToDevice, st));
+        rc = orbit_solve_forward_dev(ctx, op, tracers, nt, e_eff, c0, &o, orbit_dev, info + (size_t)p * nt);
+        if (rc < 0) return rc;
+        worst = std::max(worst, rc);
+    }
+    return worst;
+}
 
 extern "C" int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,
                                           orbit_fct** out) {

@@ -55,8 +55,9 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> owned;  // scratch owned by the context
-    // reusable scratch, grown on demand
-    DevBuf scratch[8];
+    // reusable scratch, grown on demand: 0-6 the periodic solve, 7 calibration
+    // and benchmarks, 8 the forward FCT driver's effective emissions and c0
+    DevBuf scratch[9];
     int ensure(int slot, size_t bytes);
     template <class T>
     T* buf(int slot) { return reinterpret_cast<T*>(scratch[slot].p); }

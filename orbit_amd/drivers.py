@@ -31,7 +31,7 @@ def make_fct_grids(grids: Sequence, ctx=None) -> List[FctGrid]:
 
 def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, n_passes: int = 6,
                 tol: float = 1e-8, precision: str = "mixed", ghost=None, maxiter: int = 200,
-                warm: str = "c0", ctx=None, **kw):
+                warm: str = "c0", ctx=None, native: bool = True, **kw):
     """e: (nt, n_bins, N) CUDA float64.  ghost: optional lateral inflow
     concentration per bin, the FCT value beyond the edge: an (nt, n_bins, N)
     tensor, or a list of nt (n_bins, N) tensors with None for a tracer without
@@ -39,10 +39,16 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
     warm: "orbit" starts each pass from the whole
     previous orbit (mixed precision), "c0" from its c0 only.  ctx: the context
     (stream) the solves run on, the FCT grids' own (default: the operator set's).
+    native: with warm "c0" and no ghost or one ghost tensor for all tracers, run
+    every pass inside the library (orbit_forward_fct_dev: the same launches,
+    no return to Python between passes); else, or with native=False, the
+    passes are driven from here.
     Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
     ctx = ctx or op.ctx
     if any(g.ctx is not ctx for g in fct_grids):
         raise ValueError("FCT grids and the solves must share one context (stream)")
+    if native and warm == "c0" and (ghost is None or torch.is_tensor(ghost)):
+        return _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, ghost, maxiter, ctx, **kw)
     kw["ctx"] = ctx
     orbit, info = op.solve_forward(e, tracers, tol=tol, precision=precision, maxiter=maxiter, **kw)
     hist = [info]
@@ -85,6 +91,36 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
         e_prev = e_eff
         hist.append(info)
     return orbit, hist
+
+
+def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, ghost, maxiter, ctx,
+                        restart=30, inner_sweeps=0, acc_sweeps=0, use_graph=None, max_rounds=0, verbose=False):
+    import ctypes as C
+    import numpy as np
+    from . import _lib
+    from ._lib import check, lib
+    tracers = list(range(op.n_tracers)) if tracers is None else list(tracers)
+    nt = len(tracers)
+    if tuple(e.shape) != (nt, op.n_bins, op.N) or e.dtype != torch.float64 or not e.is_cuda:
+        raise ValueError(f"e must be a CUDA float64 tensor of shape {(nt, op.n_bins, op.N)}")
+    if len(fct_grids) != op.n_bins:
+        raise ValueError("one FCT grid per bin")
+    e = e.contiguous()
+    if ghost is not None:
+        if tuple(ghost.shape) != tuple(e.shape):
+            raise ValueError("ghost must have the shape of e")
+        ghost = ghost.to(torch.float64).contiguous()
+    opts = op._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps, use_graph, max_rounds, verbose)
+    tr = np.ascontiguousarray(tracers, dtype=np.int32)
+    grids = (C.c_void_p * op.n_bins)(*[g.handle.value for g in fct_grids])
+    info = (_lib.Info * (n_passes * nt))()
+    out = torch.empty((nt, op.n_bins + 1, op.N), dtype=torch.float64, device=e.device)
+    torch.cuda.current_stream(e.device).synchronize()  # the library runs on its own stream
+    rc = lib().orbit_forward_fct_dev(ctx.handle, op.handle, grids, tr.ctypes.data, nt, e.data_ptr(),
+                                     None if ghost is None else ghost.data_ptr(), int(n_passes),
+                                     C.byref(opts), out.data_ptr(), info)
+    check(rc)
+    return out, [[info[p * nt + s].as_dict() for s in range(nt)] for p in range(n_passes)]
 
 
 def _ghost_groups(ghost, e):

@@ -157,7 +157,13 @@ class FctGrid:
             if t.stride(-1) != 1 or t.shape[-1] != self.N:
                 raise ValueError("FCT batch views need unit-stride rows of length N")
         if ghost is not None and nvec > 1 and ghost.stride(0) != c.stride(0):
-            raise ValueError("ghost rows must share c's row stride")
+            # the entry point reads ghost rows on c's stride: lay them out so,
+            # and finish before the copy can be freed
+            import torch
+            g2 = torch.empty((nvec, c.stride(0)), dtype=ghost.dtype, device=ghost.device)[:, :self.N]
+            g2.copy_(ghost)
+            ghost = g2
+            sync = True
         _torch_sync(c)
         check(lib().orbit_fct_source_batch_dev(
             self.ctx.handle, self.handle, nvec, c.data_ptr(), c.stride(0),
