@@ -85,3 +85,22 @@ def test_calibration_samples_the_weakest_tracer():
     lus = [orb.factor_one_bin(wl.L(t, 13)) for t in range(8)]
     ref, _ = orb.solve_orbit_one_species(lus, list(wl.e[13]), tol=1e-12, maxiter=500)
     assert info[0]["converged"] and _rel(orbit[0, 0], ref[0]) < 1e-6
+
+
+def test_changing_relaxation_recaptures_the_sweep_graphs():
+    """The mixed path replays captured sweep graphs, which hold the relaxation
+    of their launches: after set_relaxation the next solve must use the new
+    one.  A set that solved (and captured) with its chosen omega and then
+    switched to omega = 1 gives exactly the solve of a fresh set switched to
+    omega = 1 before any capture."""
+    wl = synth.make_workload(12, 10, 6, 2, seed=84, convection=True, wind_std=8.0)
+    a = OperatorSet.from_workload(wl)
+    if a.stats()["relaxation"] == 1.0:
+        pytest.skip("this operator keeps omega = 1")
+    a.solve_forward(wl.e, tol=1e-8, precision="mixed")  # captures with the chosen omega
+    a.set_relaxation(1.0)
+    ra, _ = a.solve_forward(wl.e, tol=1e-8, precision="mixed")
+    b = OperatorSet.from_workload(wl)
+    b.set_relaxation(1.0)
+    rb, _ = b.solve_forward(wl.e, tol=1e-8, precision="mixed")
+    assert np.array_equal(ra, rb)
