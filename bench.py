@@ -10,8 +10,9 @@ in HBM.  value = orbit solves/s = 12 / step time.
 
 Beside it: the plain FP64 GPU solve of the same work (same algorithm in FP64,
 one orbit at a time, no batching, no graphs), the per-tracer c0 difference,
-p50/p99 per-step latency, the same without FCT, the adjoint (config 3), 144
-uneven batched orbits (config 2), a Poisson serving sweep (config 4), the
+p50/p99 per-step latency, the same without FCT, the adjoint at the same size
+(50-cell receptors), 144 uneven batched orbits (12 months x 12 tracers over
+24x20 / 48x40 / 64x70), a Poisson serving sweep over those months, the
 roofline of the dominant kernel from live HIP-event timing and a CPU baseline
 (the oracle: SuperLU factors + scipy GMRES) on a bounded sample.
 
@@ -112,11 +113,11 @@ def main():
 
     T = args.tracers
     t0 = time.perf_counter()
-    wl = synth.config_workload(1, n_tracers=T)
+    wl = synth.benchmark_workload("fct-month", n_tracers=T)
     ctx = Context(0)
     op = OperatorSet.from_workload(wl, ctx)
     grids = drivers.make_fct_grids(wl.grids, ctx)
-    log(f"config 1: 64x70x15 x {T} tracers, operator set {op.stats()['device_bytes'] / 1e9:.2f} GB "
+    log(f"headline: 64x70x15 x {T} tracers, operator set {op.stats()['device_bytes'] / 1e9:.2f} GB "
         f"on device, setup {time.perf_counter() - t0:.1f}s")
     e_dev = torch.from_numpy(wl.e).to("cuda:0")
     if args.calib_copy:
@@ -202,7 +203,7 @@ def main():
                          "speedup": round(nofct_mixed / nofct_plain, 2)}
         log(f"no FCT: mixed {nofct_mixed:.1f}/s, plain {nofct_plain:.1f}/s")
 
-        # config 3: adjoint, receptor on 50 surface cells, batched mixed vs FP64 one at a time
+        # adjoint: receptor on 50 surface cells, batched mixed vs FP64 one at a time
         rng = np.random.default_rng(3)
         R = np.zeros((T, 8, wl.N))
         for s in range(T):
@@ -249,11 +250,11 @@ def main():
         op_r.close()
         op_r2.close()
 
-        # config 2: 144 uneven orbits = 12 months x 12 tracers, sizes 1:1:1
+        # 144 uneven orbits = 12 months x 12 tracers, sizes 1:1:1
         jobs, base_e = [], []
         sizes = ["24x20", "48x40", "64x70"]
         for month in range(12):
-            wlm = synth.config_workload(2, n_tracers=12, month=month, size=sizes[month % 3])
+            wlm = synth.benchmark_workload("uneven-batch", n_tracers=12, month=month, size=sizes[month % 3])
             opm = OperatorSet.from_workload(wlm, ctx)
             jobs.append((opm, torch.from_numpy(wlm.e).cuda(), list(range(12))))
             base_e.append(wlm.e)
@@ -273,10 +274,10 @@ def main():
                               "untuned_value": round(144 / wall_def, 2),
                               "autotune_s": round(t_tune, 3), "shapes": tuner.report(),
                               "sizes": "24x20, 48x40, 64x70 (x15) in 1:1:1", "streams": 4}
-        log(f"config 2: 144 orbits in {wall2 * 1e3:.1f} ms = {144 / wall2:.1f}/s tuned "
+        log(f"uneven batch: 144 orbits in {wall2 * 1e3:.1f} ms = {144 / wall2:.1f}/s tuned "
             f"({144 / wall_def:.1f}/s untuned; autotune {t_tune:.2f} s)")
 
-        # config 4: Poisson serving sweep over resident month operator sets
+        # serving: Poisson sweep over resident month operator sets
         ops = {(m, sizes[m % 3]): jobs[m][0] for m in range(12)}
         sweeps = []
         srv = serve.OrbitServer(ops, n_workers=4, max_slots=16, hbm_budget=64e9)

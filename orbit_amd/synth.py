@@ -404,16 +404,24 @@ def make_workload(nx: int, ny: int, nz: int = 15, n_tracers: int = 1, seed=0,
                     tag=dict(seed=seed, convection=convection, periodic=periodic))
 
 
-def config_workload(cfg: int, n_tracers: Optional[int] = None, month: int = 0,
-                    size: str = "64x70", convection: Optional[bool] = None) -> Workload:
-    """Seeded workloads of the five benchmark configurations.
+# Seed tag of each benchmark workload kind (the first seed component).
+WORKLOAD_KINDS = {"single": 0, "fct-month": 1, "uneven-batch": 2, "adjoint": 3, "serving": 4}
 
-    cfg 0: 24x20x15, 1 tracer, no convection.
-    cfg 1/3: 64x70x15, 12 tracers, dense convection blocks.
-    cfg 2/4: one (month, size) orbit set of the batched / served mixes.
-    Seeds are ``[cfg, month, size_class]``.
+
+def benchmark_workload(kind: str, n_tracers: Optional[int] = None, month: int = 0,
+                       size: str = "64x70", convection: Optional[bool] = None) -> Workload:
+    """Seeded workloads of the benchmark suite, by kind.
+
+    "single": 24x20x15, 1 tracer, no convection.
+    "fct-month" / "adjoint": 64x70x15, 12 tracers, dense convection blocks.
+    "uneven-batch" / "serving": one (month, size) orbit set of the batched /
+    served mixes, 12 tracers by default.
+    Seeds are ``[kind tag, month, size_class]``.
     """
-    if cfg == 0:
+    if kind not in WORKLOAD_KINDS:
+        raise ValueError(f"unknown workload kind {kind!r}; one of {sorted(WORKLOAD_KINDS)}")
+    tag = WORKLOAD_KINDS[kind]
+    if kind == "single":
         nx, ny, nz = SIZE_CLASSES["24x20"]
         nt = 1 if n_tracers is None else n_tracers
         conv = False if convection is None else convection
@@ -422,7 +430,7 @@ def config_workload(cfg: int, n_tracers: Optional[int] = None, month: int = 0,
         nt = 12 if n_tracers is None else n_tracers
         conv = True if convection is None else convection
     size_id = list(SIZE_CLASSES).index(f"{nx}x{ny}")
-    return make_workload(nx, ny, nz, nt, seed=[cfg, month, size_id], convection=conv)
+    return make_workload(nx, ny, nz, nt, seed=[tag, month, size_id], convection=conv)
 
 
 # --------------------------------------------------------------------------- #

@@ -1,4 +1,4 @@
-"""Config 3 at its full size (64x70x15, 12 tracers): the adjoint gradient
+"""The adjoint gradient at the benchmark size (64x70x15, 12 tracers):
 d(r . c0)/ds through the autograd function, with r an indicator on 50 random
 surface cells weighted U(0,1) per tracer.  At this size the oracle does not
 finish in seconds, so the checks are size-independent: the duality
@@ -18,7 +18,7 @@ from orbit_amd.core import OperatorSet
 
 @pytest.fixture(scope="module")
 def full():
-    wl = synth.config_workload(3)
+    wl = synth.benchmark_workload("adjoint")
     op = OperatorSet.from_workload(wl)
     rng = np.random.default_rng(33)
     ncol = wl.nx * wl.ny
@@ -40,7 +40,7 @@ def _grad(op, s, r, precision):
     return float(J.detach()), st.grad.cpu().numpy()
 
 
-def test_config3_gradient_duality_fd_and_mixed(full):
+def test_full_size_gradient_duality_fd_and_mixed(full):
     wl, op, s, r, src, rng = full
     J, g = _grad(op, s, r, "fp64")
     # duality of the linear solve: r . (I - M)^-1 s = lambda0 . s

@@ -1,8 +1,9 @@
-"""Configs 1 and 2 at their full sizes, through size-independent properties
+"""The FCT month (64x70x15, 12 tracers) and an uneven mix of month sets at
+full size, through size-independent properties
 (the oracle does not finish in seconds here): the mixed-precision FCT month
 against the plain FP64 path per tracer at the 1e-6 bar with converged
 residuals and periodic orbits; a tracer's orbit bit-identical alone and inside
-the 12-tracer batch; the uneven 24x20 / 48x40 / 64x70 mix of config 2 solved
+the 12-tracer batch; an uneven 24x20 / 48x40 / 64x70 mix of month sets solved
 concurrently equal, bit for bit, to each set solved alone."""
 import numpy as np
 import pytest
@@ -15,8 +16,8 @@ from orbit_amd import drivers, serve, synth
 from orbit_amd.core import OperatorSet
 
 
-def test_config1_fct_month_mixed_vs_fp64_and_alone_vs_batch():
-    wl = synth.config_workload(1)
+def test_fct_month_mixed_vs_fp64_and_alone_vs_batch():
+    wl = synth.benchmark_workload("fct-month")
     op = OperatorSet.from_workload(wl)
     grids = drivers.make_fct_grids(wl.grids, op.ctx)
     e = torch.from_numpy(wl.e).cuda()
@@ -33,10 +34,10 @@ def test_config1_fct_month_mixed_vs_fp64_and_alone_vs_batch():
     assert torch.equal(alone[0], om[5])
 
 
-def test_config2_uneven_mix_equals_each_set_alone():
+def test_uneven_mix_equals_each_set_alone():
     jobs = []
     for month, size in enumerate(("24x20", "48x40", "64x70")):
-        wlm = synth.config_workload(2, n_tracers=12, month=month, size=size)
+        wlm = synth.benchmark_workload("uneven-batch", n_tracers=12, month=month, size=size)
         opm = OperatorSet.from_workload(wlm)
         jobs.append((opm, torch.from_numpy(wlm.e).cuda(), list(range(12))))
     res, _ = serve.solve_many(jobs, n_workers=3)

@@ -31,8 +31,8 @@ def small():
 
 
 @pytest.fixture(scope="module")
-def cfg0():
-    wl = synth.config_workload(0)
+def single24():
+    wl = synth.benchmark_workload("single")
     op = OperatorSet.from_workload(wl)
     lus = [orb.factor_one_bin(wl.L(t, 0)) for t in range(N_BINS)]
     return wl, op, lus
@@ -49,10 +49,10 @@ def test_apply_P_matches_splu_both_transposes(small):
                                    rtol=0, atol=1e-12 * np.abs(lus[s][tau].solve(b, trans="T")).max())
 
 
-def test_apply_P_config0_nonnegative_for_nonnegative_rhs(cfg0):
+def test_apply_P_single_tracer_nonnegative_for_nonnegative_rhs(single24):
     """A_tau is an M-matrix: a non-negative source gives a non-negative state
     (reference tests/test_solve.py:19-52)."""
-    wl, op, lus = cfg0
+    wl, op, lus = single24
     b = np.zeros(wl.N)
     b[: wl.nx * wl.ny] = np.random.default_rng(1).random(wl.nx * wl.ny)
     x = op.apply_P(3, 0, b)
@@ -80,8 +80,8 @@ def test_forward_mixed_matches_oracle(small):
         assert _rel(orbit[s, 0], ref[0]) < 1e-6, s
 
 
-def test_forward_config0_both_precisions(cfg0):
-    wl, op, lus = cfg0
+def test_forward_single_tracer_both_precisions(single24):
+    wl, op, lus = single24
     ref, _ = orb.solve_orbit_one_species(lus, list(wl.e[0]), tol=1e-12, maxiter=500)
     for prec, tol, bar in (("fp64", 1e-10, 1e-8), ("mixed", 1e-8, 1e-6)):
         orbit, info = op.solve_forward(wl.e, tol=tol, precision=prec)
@@ -149,9 +149,9 @@ def test_alone_and_in_a_batch_give_identical_results(small):
     assert np.array_equal(lam5[4], lam3[1])
 
 
-def test_lu_protocol_call_surface(cfg0):
+def test_lu_protocol_call_surface(single24):
     from orbit_amd import solve as hsolve, orbit as horbit, orbit_adjoint as hadj
-    wl, _, lus = cfg0
+    wl, _, lus = single24
     hl = hsolve.factor_species([wl.L(t, 0) for t in range(N_BINS)], wl.nz, wl.ny, wl.nx)
     assert hl[0].shape == (wl.N, wl.N)
     b = np.random.default_rng(2).random(wl.N)
@@ -199,7 +199,7 @@ def test_shared_context_many_opsets_stay_finite():
     import torch
     outs = []
     for month, size, nt in ((0, "24x20", 3), (1, "48x40", 3), (0, "24x20", 4), (2, "48x40", 2)):
-        wl = synth.config_workload(2, n_tracers=nt, month=month, size=size)
+        wl = synth.benchmark_workload("uneven-batch", n_tracers=nt, month=month, size=size)
         op = OperatorSet.from_workload(wl)
         assert not any(op.check().values())
         e = torch.from_numpy(wl.e).cuda()

@@ -16,7 +16,7 @@ def opsets():
     out, base = {}, {}
     for month in (0, 1):
         for size in ("24x20",):
-            wl = synth.config_workload(4, n_tracers=4, month=month, size=size)
+            wl = synth.benchmark_workload("serving", n_tracers=4, month=month, size=size)
             out[(month, size)] = OperatorSet.from_workload(wl)
             base[(month, size)] = wl.e
     return out, base
@@ -88,7 +88,7 @@ def test_admission_by_memory_budget(opsets):
 def test_solve_many_uneven_sizes():
     jobs = []
     for month, size in ((0, "24x20"), (1, "48x40")):
-        wl = synth.config_workload(2, n_tracers=3, month=month, size=size)
+        wl = synth.benchmark_workload("uneven-batch", n_tracers=3, month=month, size=size)
         op = OperatorSet.from_workload(wl)
         jobs.append((op, torch.from_numpy(wl.e).cuda(), [0, 1, 2]))
     res, wall = serve.solve_many(jobs, n_workers=2)

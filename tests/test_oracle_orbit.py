@@ -151,7 +151,7 @@ def test_mass_balance_sees_a_leak_at_its_rate():
 
 def test_synthetic_nnz_matches_operator_shapes():
     """7-point operator at 24x20x15 and dense 15x15 columns at 64x70x15."""
-    wl0 = synth.config_workload(0)
+    wl0 = synth.benchmark_workload("single")
     assert wl0.L(0, 0).nnz == 48120
     wl1 = synth.make_workload(64, 70, 15, 1, seed=0, convection=True)
     assert wl1.L(0, 0).nnz == 1272780
