@@ -4,7 +4,8 @@ Mirrors orbit/core/orbit_adjoint.py:36-153: apply_monodromy_adjoint,
 compute_s_adjoint, solve_orbit_adjoint_one_species (GMRES on
 (I - M^T) lam0 = s_adj, unroll lam_tau = P_tau^T lam_{tau+1} + R_tau, residual
 and periodicity diagnostics) and gradient_per_emission_bin
-(dJ/de_tau = dtau (lam_tau - R_tau)).
+(dJ/de_tau = dtau (lam_tau - R_tau)); and receptor_R_list, the adjoint mode's
+pairing of end-of-bin receptors with the recursion (orbit/modes/adjoint.py:192-203).
 """
 from __future__ import annotations
 
@@ -47,3 +48,13 @@ def solve_orbit_adjoint_one_species(lu_list_8, R_list, *, lam_warm=None, tol=1e-
 
 def gradient_per_emission_bin(adjoint, R_list, dtau=DTAU):
     return [dtau * (np.asarray(adjoint[tau]) - np.asarray(R_list[tau])) for tau in range(N_BINS)]
+
+
+def receptor_R_list(receptors, sp_idx):
+    """R list of one species from per-bin receptors (orbit/modes/adjoint.py:192-203).
+
+    receptors[tau, sp_idx] weights the end-of-bin state c[tau + 1]:
+    J = sum_tau <r_tau, c[tau + 1]>.  In lam_tau = P_tau^T lam_{tau+1} + R_tau
+    the term R_tau weights c[tau], so r_tau goes to R[(tau + 1) mod N_BINS]
+    (c[N_BINS] = c[0] on the periodic orbit)."""
+    return [np.array(receptors[(tau - 1) % N_BINS, sp_idx], dtype=np.float64) for tau in range(N_BINS)]
