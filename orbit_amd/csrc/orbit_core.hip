@@ -448,6 +448,34 @@ __global__ __launch_bounds__(256) void k_ext_to_int(const double* __restrict__ s
     }
 }
 
+// scale * (src - sub) from k-major (N,) to the padded layout, src and sub with
+// the same strides (the change of a forcing); same grid as k_ext_to_int.
+__global__ __launch_bounds__(256) void k_ext_to_int_diff(const double* __restrict__ src,
+                                                         const double* __restrict__ sub, long long sstride,
+                                                         double* __restrict__ dst, long long dstride, int nstates,
+                                                         int nz, int ncol, double scale) {
+    __shared__ double tile[PT_COLS * PT_LD];
+    const int s = blockIdx.y;
+    const int slot = s / nstates, st = s - slot * nstates;
+    const int col0 = blockIdx.x * PT_COLS;
+    const size_t off = (size_t)slot * sstride + (size_t)st * (size_t)nz * ncol;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int k = e / PT_COLS, c = e - k * PT_COLS;
+        double v = 0.0;
+        if (k < nz && col0 + c < ncol) {
+            const size_t i = off + (size_t)k * ncol + col0 + c;
+            v = scale * (src[i] - sub[i]);
+        }
+        tile[c * PT_LD + k] = v;
+    }
+    __syncthreads();
+    double* out = dst + (size_t)slot * dstride + (size_t)st * (size_t)ncol * 16 + (size_t)col0 * 16;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int c = e >> 4, k = e & 15;
+        if (col0 + c < ncol) out[e] = tile[c * PT_LD + k];
+    }
+}
+
 // padded device layout -> k-major (N,) per (slot, state); same grid.
 __global__ __launch_bounds__(256) void k_int_to_ext(const double* __restrict__ src, long long sstride,
                                                     double* __restrict__ dst, long long dstride, int nstates,
@@ -1223,6 +1251,16 @@ __global__ void k_add_f32(const float* __restrict__ in, long long is, const doub
     const float v = (float)e[(size_t)slot * es + g];
     out[(size_t)slot * os + g] = in ? in[(size_t)slot * is + g] + v : v;
 }
+
+// out += (double)in   per slot (an FP32-propagated change added to FP64 states)
+__global__ void k_acc_f32(const float* __restrict__ in, long long is, double* __restrict__ out, long long os, int n,
+                          const int* active) {
+    const int slot = blockIdx.y;
+    if (active && !active[slot]) return;
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    out[(size_t)slot * os + g] += (double)in[(size_t)slot * is + g];
+}
 }  // namespace orbit
 
 namespace {
@@ -1548,6 +1586,15 @@ int norms(Solver& S, const double* x, long long xslot, int n, std::vector<double
     return ORBIT_OK;
 }
 
+// Squared norms per slot into dst[0..nslots) on the device, no host sync (the
+// same reduction as norms).
+void norms_dev(Solver& S, const double* x, long long xslot, int n, double* dst) {
+    hipStream_t st = S.B.ctx->stream;
+    const int nb = (n + RED_CHUNK - 1) / RED_CHUNK;
+    hipLaunchKernelGGL(k_norm2, dim3(nb, S.B.nslots), dim3(RED_THREADS), 0, st, x, xslot, n, nb, S.partials);
+    hipLaunchKernelGGL(k_sum_partials, grid_waves(S.B.nslots), dim3(256), 0, st, S.partials, nb, dst, S.B.nslots);
+}
+
 // Forced accurate sweep.  Forward: orbit[0] = c (or 0), orbit[t+1] = P_t(orbit[t] + dtau e_t).
 // Adjoint: orbit[nb] = c (or 0), orbit[t] = P_t^T orbit[t+1] + R_t for t = nb-1..0.
 // Either way the outer residual s - (I - M)c is the last state minus the first.
@@ -1600,8 +1647,9 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
 }
 
 // Inexact forced sweep from zero in FP32 (the mixed path's s~, used for ||s|| and
-// as the first right-hand side): result (FP64) in S.r.
-int forced_sweep_f32(Solver& S) {
+// as the first right-hand side): result (FP64) in S.r.  acc (forward only,
+// nullable): add every end-of-bin state to acc's state t+1 ([slot][nbins+1][n16]).
+int forced_sweep_f32(Solver& S, double* acc = nullptr) {
     hipStream_t st = S.B.ctx->stream;
     const int nbn = S.nbins;
     const size_t n16 = S.n16;
@@ -1617,6 +1665,9 @@ int forced_sweep_f32(Solver& S) {
                                rhs, vs, (int)n16, mask);
             int rc = inner_solve(S.B, t, FWD, SP_F32, S.inner, rhs, vs, y, vs, mask);
             if (rc) return rc;
+            if (acc)
+                hipLaunchKernelGGL(k_acc_f32, g, dim3(256), 0, st, y, vs, acc + (size_t)(t + 1) * n16,
+                                   (long long)(nbn + 1) * n16, (int)n16, mask);
             std::swap(x, y);
         }
     } else {
@@ -1780,7 +1831,7 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
 
 int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, const double* in_dev,
                    const double* warm_dev, const orbit_opts* opts_in, double* out_dev, orbit_info* info,
-                   int dir) {
+                   int dir, PassDelta* pd = nullptr) {
     if (!ctx || !op || !tracers || nt < 1 || !in_dev || !out_dev || !info)
         return fail(ORBIT_ERR_INVALID, "solve: NULL argument or empty tracer list");
     orbit_opts o;
@@ -1794,6 +1845,11 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     for (int s = 0; s < nt; ++s)
         if (tracers[s] < 0 || tracers[s] >= op->ntr)
             return fail(ORBIT_ERR_INVALID, "solve: tracer index out of range");
+    const bool delta = pd && pd->e_prev;
+    if (pd && (dir != FWD || o.precision != ORBIT_PREC_MIXED || o.rhs_is_s || !pd->s_state))
+        return fail(ORBIT_ERR_INVALID, "solve: pass-to-pass state needs a mixed-precision forward solve and s_state");
+    if (delta && !(warm_dev && o.warm_orbit))
+        return fail(ORBIT_ERR_INVALID, "solve: a forcing change needs the previous orbit as the warm whole orbit");
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
 
@@ -1852,7 +1908,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.B.ctx = ctx;
     S.B.op = op;
     S.B.nslots = nt;
-    std::vector<double> host(std::max(nt, 1) * 2 + 8);
+    std::vector<double> host(std::max(nt, 1) * 3 + 8);
     S.host = host.data();
 
     const long long vs = (long long)n16;
@@ -1869,7 +1925,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                            (int)(nbn * n16));
         hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, in_dev, (long long)op->N, S.sv,
                            (long long)n16, 1, op->nz, op->ncol, 1.0);
-    } else {
+    } else if (!delta) {  // (a forcing change loads e after propagating the change)
         hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
                            (long long)nbn * op->N, S.E, (long long)nbn * n16, nbn, op->nz, op->ncol, scale);
     }
@@ -1879,17 +1935,69 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     const int max_cycles = o.max_rounds > 0 ? o.max_rounds : o.maxiter;
     bool need_check = true;
     int warm_first = 0;  // > 0: the first check starts from a caller-given orbit
+    std::vector<int> first_nsw;  // per-slot sweeps of that first check (else warm_first for all)
     if (S.prec == ORBIT_PREC_MIXED) {
         // s~ by an FP32 forced sweep: only its norm and its direction are used; the
         // residuals that decide convergence are accurate FP64 sweeps.
-        if (S.sv)
+        std::vector<double> dnorm;
+        const long long os_ = (long long)(nbn + 1) * n16, es_ = (long long)nbn * n16;
+        if (S.sv) {
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.sv, vs, S.sv, vs, 0.0, S.r, vs, (int)n16, nullptr);
-        else
+        } else if (delta) {
+            // Forcing changed by dE since the previous orbit: propagate dtau*dE from
+            // zero with the FP32 sweeps, add its states to the previous orbit's (the
+            // start of the first check) and its last state to the previous s~.
+            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
+                               (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
+            hipLaunchKernelGGL(k_ext_to_int_diff, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
+                               pd->e_prev, (long long)nbn * op->N, S.E, es_, nbn, op->nz, op->ncol, scale);
+            norms_dev(S, S.E, es_, (int)es_, S.h1);  // h1 is free until GMRES
+            rc = forced_sweep_f32(S, S.orbit);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, pd->s_state, vs, 1.0, S.r, vs, (int)n16,
+                               nullptr);
+            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
+                               (long long)nbn * op->N, S.E, es_, nbn, op->nz, op->ncol, scale);
+        } else {
             rc = forced_sweep_f32(S);
+        }
         if (rc) return rc;
-        rc = norms(S, S.r, vs, (int)n16, snorm);
-        if (rc) return rc;
-        if (warm_dev && o.warm_orbit) {
+        if (pd) ORBIT_HIP_CHECK(hipMemcpyAsync(pd->s_state, S.r, (size_t)nt * n16 * 8, hipMemcpyDeviceToDevice, st));
+        std::vector<double> cn;
+        if (delta) {
+            // ||dtau dE||, ||s~|| and ||c|| with one host round trip
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os_, S.orbit, os_, 0.0, S.c, vs, (int)n16,
+                               nullptr);
+            norms_dev(S, S.r, vs, (int)n16, S.h1 + nt);
+            norms_dev(S, S.c, vs, (int)n16, S.h1 + 2 * nt);
+            ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.h1, (size_t)3 * nt * 8, hipMemcpyDeviceToHost, st));
+            ORBIT_HIP_CHECK(hipStreamSynchronize(st));
+            dnorm.assign(S.host, S.host + nt);
+            snorm.assign(S.host + nt, S.host + 2 * nt);
+            cn.assign(S.host + 2 * nt, S.host + 3 * nt);
+            for (std::vector<double>* v : {&dnorm, &snorm, &cn})
+                for (double& x : *v) x = std::sqrt(x);
+        } else {
+            rc = norms(S, S.r, vs, (int)n16, snorm);
+            if (rc) return rc;
+        }
+        if (delta) {
+            // Error of the start: the previous states are accurate to eps_acc; the
+            // change's states are off by at most nbins FP32 inner solves of relative
+            // error rho^inner each, applied to states no larger than sum_t ||dtau dE_t||
+            // <= sqrt(nbins) ||dtau dE|| (the propagators contract).  A worst-case sum
+            // with rho's calibration margin: no extra sweep on top.
+            const double eps_in = std::pow(S.rho, S.inner);
+            first_nsw.assign(nt, S.acc);
+            for (int s = 0; s < nt; ++s) {
+                if (!(cn[s] > 0.0)) continue;
+                const double rel = S.eps_acc + nbn * eps_in * std::sqrt((double)nbn) * dnorm[s] / cn[s];
+                int k = S.acc;
+                if (rel < 1.0) k = (int)std::ceil(std::log(S.eps_acc / rel) / std::log(S.rho));
+                first_nsw[s] = std::min(S.acc, std::max(2, k));
+                warm_first = std::max(warm_first, first_nsw[s]);
+            }
+        } else if (warm_dev && o.warm_orbit) {
             // whole previous orbit: its states start the first accurate sweep
             const long long os_ = (long long)(nbn + 1) * n16;
             hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
@@ -1939,7 +2047,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     bool have_orbit = warm_first > 0;
     int next_sweeps = warm_first > 0 ? warm_first : S.acc;
     if (warm_first > 0) {
-        std::vector<int> nsw(nt, warm_first);
+        std::vector<int> nsw = first_nsw.empty() ? std::vector<int>(nt, warm_first) : first_nsw;
         int rc0 = set_ints(st, S.nsw_dev, nsw.data(), nt);
         if (rc0) return rc0;
     }
@@ -2107,6 +2215,12 @@ extern "C" int orbit_solve_forward_dev(orbit_ctx* ctx, orbit_opset* op, const in
                                        const double* e_dev, const double* c_warm_dev, const orbit_opts* opts,
                                        double* orbit_out_dev, orbit_info* info) {
     return solve_periodic(ctx, op, tracers, nt, e_dev, c_warm_dev, opts, orbit_out_dev, info, FWD);
+}
+
+int orbit::solve_forward_pass(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, const double* e_dev,
+                              const double* prev_orbit_dev, PassDelta* pd, const orbit_opts* opts, double* out_dev,
+                              orbit_info* info) {
+    return solve_periodic(ctx, op, tracers, nt, e_dev, prev_orbit_dev, opts, out_dev, info, FWD, pd);
 }
 
 extern "C" int orbit_solve_adjoint_dev(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt,

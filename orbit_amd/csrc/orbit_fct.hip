@@ -1104,9 +1104,12 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
 // Forward FCT deferred correction (reference orbit/cli.py:1212-1241) with
 // every pass on the device: pass 0 solves with e; pass p adds the limited
 // anti-diffusive source of the previous orbit's end-of-bin states
-// d_AD(c[tau+1]) to e and solves again, warm-started from the previous c0.
-// The same launches as the host loop over orbit_fct_source_batch_dev and
-// orbit_solve_forward_dev, without returning to the caller between passes.
+// d_AD(c[tau+1]) to e and solves again.  FP64: warm-started from the previous
+// c0, the same launches as the host loop over orbit_fct_source_batch_dev and
+// orbit_solve_forward_dev.  Mixed precision: each pass after the first starts
+// from the previous orbit plus the FP32 propagation of the change of the
+// forcing (solve_forward_pass), so its first accurate check needs only the
+// sweeps for the FP32 error of that change instead of a cold check.
 extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids,
                                      const int* tracers, int nt, const double* e_dev, const double* ghost_dev,
                                      int n_passes, const orbit_opts* opts, double* orbit_dev, orbit_info* info) {
@@ -1126,26 +1129,51 @@ extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_
     orbit_opts_default(&o);
     if (opts) o = *opts;
     o.warm_orbit = 0;
+    o.warm_sweeps = 0;
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
-    int rc = orbit_solve_forward_dev(ctx, op, tracers, nt, e_dev, nullptr, &o, orbit_dev, info);
+    const bool inc = o.precision == ORBIT_PREC_MIXED && n_passes > 1;
+    const size_t ne = (size_t)nt * nbn * N, n16 = (size_t)op->n16;
+    // scratch: two forcings (this pass's and, mixed, the previous pass's in
+    // turn), c0 (FP64 passes) and s~ (mixed passes)
+    double *ebuf[2] = {nullptr, nullptr}, *c0 = nullptr, *s_state = nullptr;
+    if (n_passes > 1) {
+        int rc0 = ctx->ensure(8, (2 * ne + (size_t)nt * N + (size_t)nt * n16) * 8);
+        if (rc0) return rc0;
+        ebuf[0] = ctx->buf<double>(8);
+        ebuf[1] = ebuf[0] + ne;
+        c0 = ebuf[1] + ne;
+        s_state = c0 + (size_t)nt * N;
+    }
+    PassDelta pd;
+    pd.s_state = s_state;
+    int rc = inc ? solve_forward_pass(ctx, op, tracers, nt, e_dev, nullptr, &pd, &o, orbit_dev, info)
+                 : orbit_solve_forward_dev(ctx, op, tracers, nt, e_dev, nullptr, &o, orbit_dev, info);
     if (rc < 0) return rc;
     int worst = rc;
     if (n_passes == 1) return worst;
-    rc = ctx->ensure(8, ((size_t)nt * nbn * N + (size_t)nt * N) * 8);
-    if (rc) return rc;
-    double* e_eff = ctx->buf<double>(8);
-    double* c0 = e_eff + (size_t)nt * nbn * N;
     const long long os = (long long)(nbn + 1) * N, es = (long long)nbn * N;
+    const double* e_prev = e_dev;  // the forcing the current orbit was solved with
     for (int p = 1; p < n_passes; ++p) {
+        double* e_eff = inc ? ebuf[(p - 1) & 1] : ebuf[0];
         for (int tau = 0; tau < nbn; ++tau) {
             rc = fct_source_batch(ctx, grids[tau], nt, orbit_dev + (size_t)(tau + 1) * N, os,
                                   ghost_dev ? ghost_dev + (size_t)tau * N : nullptr, es, op->dtau, 1,
                                   e_dev + (size_t)tau * N, es, e_eff + (size_t)tau * N, es);
             if (rc) return rc;
         }
-        ORBIT_HIP_CHECK(hipMemcpy2DAsync(c0, N * 8, orbit_dev, (size_t)os * 8, N * 8, nt, hipMemcpyDeviceToDevice, st));
-        rc = orbit_solve_forward_dev(ctx, op, tracers, nt, e_eff, c0, &o, orbit_dev, info + (size_t)p * nt);
+        if (inc) {
+            orbit_opts oi = o;
+            oi.warm_orbit = 1;
+            pd.e_prev = e_prev;
+            rc = solve_forward_pass(ctx, op, tracers, nt, e_eff, orbit_dev, &pd, &oi, orbit_dev,
+                                    info + (size_t)p * nt);
+            e_prev = e_eff;
+        } else {
+            ORBIT_HIP_CHECK(
+                hipMemcpy2DAsync(c0, N * 8, orbit_dev, (size_t)os * 8, N * 8, nt, hipMemcpyDeviceToDevice, st));
+            rc = orbit_solve_forward_dev(ctx, op, tracers, nt, e_eff, c0, &o, orbit_dev, info + (size_t)p * nt);
+        }
         if (rc < 0) return rc;
         worst = std::max(worst, rc);
     }

@@ -1,9 +1,13 @@
 """The forward FCT passes inside the library (orbit_forward_fct_dev, reference
 orbit/cli.py:1212-1241) against the same passes driven from Python over
-orbit_fct_source_batch_dev and orbit_solve_forward_dev: the same launches, so
-the same bits -- orbits and every pass's solve information -- with no ghost
-and with a ghost for every tracer, in both precisions; one pass is the plain
-periodic solve; bad arguments are refused."""
+orbit_fct_source_batch_dev and orbit_solve_forward_dev.  FP64: the same
+launches, so the same bits -- orbits and every pass's solve information.
+Mixed precision: the library starts each pass after the first from the
+previous orbit plus the FP32 propagation of the forcing's change, the Python
+loop from the previous c0; both converge every pass and agree to the 1e-6 bar,
+and the library's passes agree with its FP64 passes to 1e-6; a tracer's passes
+are bit-identical alone and in the batch.  With no ghost and with a ghost for
+every tracer; one pass is the plain periodic solve; bad arguments are refused."""
 import numpy as np
 import pytest
 import torch
@@ -24,19 +28,41 @@ def month():
     return wl, op, grids, e, ghost
 
 
-@pytest.mark.parametrize("precision,tol", [("mixed", 1e-8), ("fp64", 1e-10)])
+def _rel(a, b):
+    return float((a - b).abs().max() / b.abs().max())
+
+
 @pytest.mark.parametrize("with_ghost", [False, True])
-def test_native_passes_equal_python_passes(month, precision, tol, with_ghost):
+def test_native_fp64_passes_equal_python_passes(month, with_ghost):
     wl, op, grids, e, ghost = month
     g = ghost if with_ghost else None
-    a, ha = drivers.forward_fct(op, grids, e, n_passes=4, tol=tol, precision=precision, ghost=g, native=True)
-    b, hb = drivers.forward_fct(op, grids, e, n_passes=4, tol=tol, precision=precision, ghost=g, native=False)
+    a, ha = drivers.forward_fct(op, grids, e, n_passes=4, tol=1e-10, precision="fp64", ghost=g, native=True)
+    b, hb = drivers.forward_fct(op, grids, e, n_passes=4, tol=1e-10, precision="fp64", ghost=g, native=False)
     assert torch.equal(a, b)
     assert len(ha) == len(hb) == 4
     for pa, pb in zip(ha, hb):
         for ia, ib in zip(pa, pb):
             assert ia == ib
     assert all(i["converged"] for i in ha[-1])
+
+
+@pytest.mark.parametrize("with_ghost", [False, True])
+def test_native_mixed_passes_match_python_and_fp64(month, with_ghost):
+    wl, op, grids, e, ghost = month
+    g = ghost if with_ghost else None
+    a, ha = drivers.forward_fct(op, grids, e, n_passes=4, tol=1e-8, precision="mixed", ghost=g, native=True)
+    b, hb = drivers.forward_fct(op, grids, e, n_passes=4, tol=1e-8, precision="mixed", ghost=g, native=False)
+    f, _ = drivers.forward_fct(op, grids, e, n_passes=4, tol=1e-11, precision="fp64", ghost=g, native=True)
+    assert len(ha) == len(hb) == 4
+    for h in ha:
+        assert all(i["converged"] and i["residual"] <= 1e-8 for i in h)
+    for t in range(e.shape[0]):
+        assert _rel(a[t, 0], b[t, 0]) <= 1e-6, t
+        assert _rel(a[t], f[t]) <= 1e-6, t
+    # per-slot arithmetic: the same passes alone and in the batch
+    alone, _ = drivers.forward_fct(op, grids, e[1:2], tracers=[1], n_passes=4, tol=1e-8, precision="mixed",
+                                   ghost=None if g is None else g[1:2], native=True)
+    assert torch.equal(alone[0], a[1])
 
 
 def test_one_pass_is_the_periodic_solve(month):

@@ -103,6 +103,22 @@ def test_warm_start_and_zero_source(small):
     assert info2[0]["gmres_iters"] <= info[0]["gmres_iters"]
 
 
+def test_adjoint_zero_receptor_gives_zero_orbit(small):
+    """Reference tests/test_orbit_adjoint.py:87-99 and :102-120: an empty
+    receptor gives an identically zero adjoint orbit (here beside a non-zero
+    slot of the same batch, in both precisions), and a non-zero one is
+    periodic."""
+    wl, op, _ = small
+    R = np.random.default_rng(4).random((wl.n_tracers, N_BINS, wl.N))
+    R[1] = 0.0
+    for prec, tol in (("fp64", 1e-10), ("mixed", 1e-8)):
+        lam, info = op.solve_adjoint(R, tol=tol, precision=prec)
+        assert info[1]["zero_source"] == 1 and not np.any(lam[1]), prec
+        assert info[0]["converged"] and np.any(lam[0]), prec
+        assert info[0]["periodicity"] < 1e-8, prec
+        assert _rel(lam[0, N_BINS], lam[0, 0]) < 1e-8, prec
+
+
 def test_not_converged_is_a_warning_not_an_error(small):
     wl, op, _ = small
     orbit, info = op.solve_forward(wl.e, tol=1e-14, maxiter=1, restart=2)
