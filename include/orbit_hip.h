@@ -351,8 +351,11 @@ int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, cons
 /* Forward FCT deferred correction on the device (reference orbit/cli.py:1212-1241,
  * the passes of forward mode): pass 0 solves the periodic orbits of e; pass p
  * = 1..n_passes-1 adds d_AD of the previous orbit's end-of-bin states (limited,
- * grids[bin] of the operator set's shape and context) to e and solves again
- * from the previous c0.  e_dev (nt, n_bins, N) and ghost_dev (nullable, the
+ * grids[bin] of the operator set's shape and context) to e and solves again:
+ * FP64 from the previous c0; mixed precision from the previous orbit plus the
+ * FP32 propagation of the forcing's change (the solve is linear), so the first
+ * accurate check of each later pass needs fewer sweeps than a cold one.  The
+ * passes converge to the same tolerance either way.  e_dev (nt, n_bins, N) and ghost_dev (nullable, the
  * same layout: the concentration beyond the lateral edge per bin) are device
  * arrays; orbit_dev (nt, n_bins+1, N) receives the last pass's orbits and
  * info[p*nt + s] the solve information of pass p.  opts as for

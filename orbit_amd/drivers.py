@@ -40,8 +40,10 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
     previous orbit (mixed precision), "c0" from its c0 only.  ctx: the context
     (stream) the solves run on, the FCT grids' own (default: the operator set's).
     native: with warm "c0" and no ghost or one ghost tensor for all tracers, run
-    every pass inside the library (orbit_forward_fct_dev: the same launches,
-    no return to Python between passes); else, or with native=False, the
+    every pass inside the library (orbit_forward_fct_dev, no return to Python
+    between passes; FP64: the same launches as the loop here; mixed: each later
+    pass starts from the previous orbit plus the FP32 propagation of the
+    forcing's change, fewer accurate sweeps); else, or with native=False, the
     passes are driven from here.
     Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
     ctx = ctx or op.ctx
