@@ -80,3 +80,34 @@ def test_native_driver_refuses_bad_arguments(month):
         drivers.forward_fct(op, grids, e[:, :7], n_passes=2)
     with pytest.raises(ValueError):
         drivers.forward_fct(op, grids, e, n_passes=2, ghost=e[:, :, :10])
+
+
+def test_mixed_passes_with_a_zero_tracer_and_a_small_krylov_space(month):
+    """A tracer with no emissions stays exactly zero through every pass (its
+    forcing change is zero, so is its s~) beside tracers that are not; with a
+    Krylov space of 3 the passes still converge, agree with the Python loop to
+    the 1e-6 bar and give the zero tracer's neighbours the same bits as
+    without it."""
+    wl, op, grids, e, _ = month
+    ez = e.clone()
+    ez[2] = 0.0
+    a, ha = drivers.forward_fct(op, grids, ez, n_passes=3, tol=1e-8, precision="mixed", restart=3)
+    b, _ = drivers.forward_fct(op, grids, ez, n_passes=3, tol=1e-8, precision="mixed", restart=3, native=False)
+    assert not torch.any(a[2])
+    for h in ha:
+        assert h[2]["zero_source"] == 1
+        assert all(i["converged"] for q, i in enumerate(h) if q != 2)
+    for t in (0, 1):
+        assert _rel(a[t, 0], b[t, 0]) <= 1e-6, t
+    pair, _ = drivers.forward_fct(op, grids, ez[[0, 1]], tracers=[0, 1], n_passes=3, tol=1e-8,
+                                  precision="mixed", restart=3)
+    assert torch.equal(pair, a[[0, 1]])
+
+
+def test_mixed_pass_not_converging_is_a_warning(month):
+    """Iteration caps reached inside the passes give the warning status (the
+    reference prints and goes on): finite orbits, converged = 0, no error."""
+    wl, op, grids, e, _ = month
+    a, h = drivers.forward_fct(op, grids, e, n_passes=3, tol=1e-14, precision="mixed", maxiter=1, restart=2)
+    assert torch.isfinite(a).all()
+    assert any(not i["converged"] for p in h for i in p)
