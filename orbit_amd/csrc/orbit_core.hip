@@ -1944,9 +1944,10 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         if (S.sv) {
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.sv, vs, S.sv, vs, 0.0, S.r, vs, (int)n16, nullptr);
         } else if (delta) {
-            // Forcing changed by dE since the previous orbit: propagate dtau*dE from
-            // zero with the FP32 sweeps, add its states to the previous orbit's (the
-            // start of the first check) and its last state to the previous s~.
+            // The forcing changed by e - e_prev since the previous orbit: propagate
+            // dtau (e - e_prev) from zero with the FP32 sweeps, add its states to the
+            // previous orbit's (the start of the first check) and its last state to
+            // the previous s~ (the solve is linear in the forcing).
             hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
                                (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
             hipLaunchKernelGGL(k_ext_to_int_diff, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
@@ -1965,7 +1966,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         if (pd) ORBIT_HIP_CHECK(hipMemcpyAsync(pd->s_state, S.r, (size_t)nt * n16 * 8, hipMemcpyDeviceToDevice, st));
         std::vector<double> cn;
         if (delta) {
-            // ||dtau dE||, ||s~|| and ||c|| with one host round trip
+            // ||dtau (e - e_prev)||, ||s~|| and ||c|| with one host round trip
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os_, S.orbit, os_, 0.0, S.c, vs, (int)n16,
                                nullptr);
             norms_dev(S, S.r, vs, (int)n16, S.h1 + nt);
@@ -1984,8 +1985,9 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         if (delta) {
             // Error of the start: the previous states are accurate to eps_acc; the
             // change's states are off by at most nbins FP32 inner solves of relative
-            // error rho^inner each, applied to states no larger than sum_t ||dtau dE_t||
-            // <= sqrt(nbins) ||dtau dE|| (the propagators contract).  A worst-case sum
+            // error rho^inner each, applied to states no larger than the sum over bins of
+            // ||dtau (e - e_prev)_t|| <= sqrt(nbins) ||dtau (e - e_prev)|| (the
+            // propagators contract).  A worst-case sum
             // with rho's calibration margin: no extra sweep on top.
             const double eps_in = std::pow(S.rho, S.inner);
             first_nsw.assign(nt, S.acc);
