@@ -1252,14 +1252,40 @@ __global__ void k_add_f32(const float* __restrict__ in, long long is, const doub
     out[(size_t)slot * os + g] = in ? in[(size_t)slot * is + g] + v : v;
 }
 
-// out += (double)in   per slot (an FP32-propagated change added to FP64 states)
-__global__ void k_acc_f32(const float* __restrict__ in, long long is, double* __restrict__ out, long long os, int n,
-                          const int* active) {
-    const int slot = blockIdx.y;
-    if (active && !active[slot]) return;
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    out[(size_t)slot * os + g] += (double)in[(size_t)slot * is + g];
+// out += (double)in per slot (an FP32-propagated change added to FP64 states),
+// and the block's partial sum of in^2 into partials[(slot*nvec + vec)*nb + block]
+// (RED_CHUNK elements a block, 0 for inactive slots).  grid (nb, nslots).
+__global__ __launch_bounds__(RED_THREADS) void k_acc_f32_norm(const float* __restrict__ in, long long is,
+                                                              double* __restrict__ out, long long os, int n,
+                                                              const int* active, int nb, int nvec, int vec,
+                                                              double* __restrict__ partials) {
+    __shared__ double sh[4];
+    const int slot = blockIdx.y, blk = blockIdx.x;
+    const bool on = !active || active[slot];
+    double acc = 0.0;
+    if (on) {
+#pragma unroll
+        for (int e = 0; e < RED_ELEMS; ++e) {
+            const int ix = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
+            if (ix < n) {
+                const double v = (double)in[(size_t)slot * is + ix];
+                out[(size_t)slot * os + ix] += v;
+                acc += v * v;
+            }
+        }
+    }
+    const double bs = block_sum(acc, sh);
+    if (threadIdx.x == 0) partials[((size_t)slot * nvec + vec) * nb + blk] = bs;
+}
+
+// out[slot] = max over vec of the sum over blocks of partials (fixed order).
+__global__ __launch_bounds__(256) void k_max_sum_partials(const double* __restrict__ partials, int nb, int nvec,
+                                                          double* __restrict__ out, int nslots) {
+    const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (slot >= nslots) return;  // uniform over the wave
+    double m = 0.0;
+    for (int v = 0; v < nvec; ++v) m = fmax(m, wave_sum_partials(partials + ((size_t)slot * nvec + v) * nb, nb));
+    if ((threadIdx.x & 63) == 0) out[slot] = m;
 }
 }  // namespace orbit
 
@@ -1648,8 +1674,9 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
 
 // Inexact forced sweep from zero in FP32 (the mixed path's s~, used for ||s|| and
 // as the first right-hand side): result (FP64) in S.r.  acc (forward only,
-// nullable): add every end-of-bin state to acc's state t+1 ([slot][nbins+1][n16]).
-int forced_sweep_f32(Solver& S, double* acc = nullptr) {
+// nullable): add every end-of-bin state to acc's state t+1 ([slot][nbins+1][n16])
+// and leave the largest squared norm of those states per slot in ymax2[slot].
+int forced_sweep_f32(Solver& S, double* acc = nullptr, double* ymax2 = nullptr) {
     hipStream_t st = S.B.ctx->stream;
     const int nbn = S.nbins;
     const size_t n16 = S.n16;
@@ -1666,10 +1693,14 @@ int forced_sweep_f32(Solver& S, double* acc = nullptr) {
             int rc = inner_solve(S.B, t, FWD, SP_F32, S.inner, rhs, vs, y, vs, mask);
             if (rc) return rc;
             if (acc)
-                hipLaunchKernelGGL(k_acc_f32, g, dim3(256), 0, st, y, vs, acc + (size_t)(t + 1) * n16,
-                                   (long long)(nbn + 1) * n16, (int)n16, mask);
+                hipLaunchKernelGGL(k_acc_f32_norm, dim3(S.nb, S.B.nslots), dim3(RED_THREADS), 0, st, y, vs,
+                                   acc + (size_t)(t + 1) * n16, (long long)(nbn + 1) * n16, (int)n16, mask, S.nb,
+                                   nbn, t, S.partials);
             std::swap(x, y);
         }
+        if (acc && ymax2)
+            hipLaunchKernelGGL(k_max_sum_partials, grid_waves(S.B.nslots), dim3(256), 0, st, S.partials, S.nb, nbn,
+                               ymax2, S.B.nslots);
     } else {
         // x = R_{nb-1}; then x = P_t^T x + R_t for t = nb-2..0
         hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.E + (size_t)(nbn - 1) * n16, es, x, vs,
@@ -1952,8 +1983,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                                (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
             hipLaunchKernelGGL(k_ext_to_int_diff, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
                                pd->e_prev, (long long)nbn * op->N, S.E, es_, nbn, op->nz, op->ncol, scale);
-            norms_dev(S, S.E, es_, (int)es_, S.h1);  // h1 is free until GMRES
-            rc = forced_sweep_f32(S, S.orbit);
+            rc = forced_sweep_f32(S, S.orbit, S.h1);  // h1 (free until GMRES): max_t ||y_t||^2
             if (rc) return rc;
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, pd->s_state, vs, 1.0, S.r, vs, (int)n16,
                                nullptr);
@@ -1966,7 +1996,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         if (pd) ORBIT_HIP_CHECK(hipMemcpyAsync(pd->s_state, S.r, (size_t)nt * n16 * 8, hipMemcpyDeviceToDevice, st));
         std::vector<double> cn;
         if (delta) {
-            // ||dtau (e - e_prev)||, ||s~|| and ||c|| with one host round trip
+            // max_t ||y_t|| of the change, ||s~|| and ||c|| with one host round trip
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os_, S.orbit, os_, 0.0, S.c, vs, (int)n16,
                                nullptr);
             norms_dev(S, S.r, vs, (int)n16, S.h1 + nt);
@@ -1983,17 +2013,16 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             if (rc) return rc;
         }
         if (delta) {
-            // Error of the start: the previous states are accurate to eps_acc; the
-            // change's states are off by at most nbins FP32 inner solves of relative
-            // error rho^inner each, applied to states no larger than the sum over bins of
-            // ||dtau (e - e_prev)_t|| <= sqrt(nbins) ||dtau (e - e_prev)|| (the
-            // propagators contract).  A worst-case sum
-            // with rho's calibration margin: no extra sweep on top.
+            // Error of the start: the previous states are accurate to eps_acc; each of
+            // the change's states y_t is off by at most the errors of the FP32 inner
+            // solves so far, rho^inner relative to their outputs, which are no larger
+            // than max_t ||y_t|| (the propagators contract).  A worst-case sum with
+            // rho's calibration margin: no extra sweep on top.
             const double eps_in = std::pow(S.rho, S.inner);
             first_nsw.assign(nt, S.acc);
             for (int s = 0; s < nt; ++s) {
                 if (!(cn[s] > 0.0)) continue;
-                const double rel = S.eps_acc + nbn * eps_in * std::sqrt((double)nbn) * dnorm[s] / cn[s];
+                const double rel = S.eps_acc + nbn * eps_in * dnorm[s] / cn[s];
                 int k = S.acc;
                 if (rel < 1.0) k = (int)std::ceil(std::log(S.eps_acc / rel) / std::log(S.rho));
                 first_nsw[s] = std::min(S.acc, std::max(2, k));
