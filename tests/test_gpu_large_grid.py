@@ -11,8 +11,10 @@ product per bin:
 
 * forward: A_tau x_{tau+1} = x_tau + dtau e_tau for every bin, and x_8 = x_0;
 * adjoint (R_0 = r, R_{tau>0} = 0): A_tau^T (lam_tau - R_tau) = lam_{tau+1} for
-  tau = 1..7, and for tau = 0 to the solve's tolerance (lam_0 is the GMRES
-  solution, as in the reference);
+  tau = 1..7; for tau = 0 lam_0 is the GMRES solution (as in the reference), so
+  there the mismatch is A_0^T times the solve's residual: at most
+  ||A_0^T||_2 * residual * ||s_adj|| with the residual the solve reports
+  (s_adj = r here);
 * mixed precision against the plain FP64 solve at 1e-6 in c0.
 
 These catch what a small grid cannot: 32-bit index products in the launch
@@ -74,9 +76,17 @@ def test_large_grid_adjoint_recursion(big):
         lam = lam.cpu().numpy()[0]
         assert info[0]["converged"], (prec, info[0])
         assert _rel(lam[NB], lam[0]) < 10 * tol, prec
+        assert info[0]["residual"] <= tol, (prec, info[0])
         for tau in range(NB):
-            lhs = wl.A(tau, 1).T @ (lam[tau] - R[0, tau])
-            # lam_0 is the GMRES solution itself (lam_8 = lam_0, orbit_adjoint.py:121-128),
-            # not the swept state, so bin 0 closes only to the solve's tolerance
-            bar = 100 * tol if tau == 0 else 1e-10
-            assert np.linalg.norm(lhs - lam[tau + 1]) <= bar * np.linalg.norm(lam[tau + 1]), (prec, tau)
+            A = wl.A(tau, 1)
+            lhs = A.T @ (lam[tau] - R[0, tau])
+            err = np.linalg.norm(lhs - lam[tau + 1])
+            if tau == 0:
+                # lam_0 is the GMRES solution itself (lam_8 = lam_0, orbit_adjoint.py:121-128),
+                # not the swept state lam_0' = P_0^T lam_1 + R_0: the mismatch is
+                # A_0^T (lam_0 - lam_0'), the residual of the periodic solve
+                a2 = np.sqrt(abs(A).sum(axis=0).max() * abs(A).sum(axis=1).max())  # >= ||A_0||_2
+                bar = 1.01 * a2 * info[0]["residual"] * np.linalg.norm(r) + 1e-10 * np.linalg.norm(lam[1])
+                assert err <= bar, (prec, tau, err, bar)
+            else:
+                assert err <= 1e-10 * np.linalg.norm(lam[tau + 1]), (prec, tau)
