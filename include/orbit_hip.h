@@ -355,15 +355,19 @@ int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, cons
  * FP64 from the previous c0; mixed precision from the previous orbit plus the
  * FP32 propagation of the forcing's change (the solve is linear), so the first
  * accurate check of each later pass needs fewer sweeps than a cold one.  The
- * passes converge to the same tolerance either way.  e_dev (nt, n_bins, N) and ghost_dev (nullable, the
- * same layout: the concentration beyond the lateral edge per bin) are device
- * arrays; orbit_dev (nt, n_bins+1, N) receives the last pass's orbits and
- * info[p*nt + s] the solve information of pass p.  opts as for
- * orbit_solve_forward (rhs_is_s refused).  Returns the worst status of the
+ * passes converge to the same tolerance either way.  e_dev (nt, n_bins, N) and
+ * ghost_dev (nullable, the same layout: the concentration beyond the lateral
+ * edge per bin, the lateral boundary values of orbit/core/lateral_bc.py) are
+ * device arrays; ghost_tracers (nullable, host, nt ints): tracer s reads its
+ * ghost rows only where ghost_tracers[s] != 0, the others keep closed edges
+ * (the reference passes a ghost only for species with boundary values); null
+ * means every tracer has one.  orbit_dev (nt, n_bins+1, N) receives the last
+ * pass's orbits and info[p*nt + s] the solve information of pass p.  opts as
+ * for orbit_solve_forward (rhs_is_s refused).  Returns the worst status of the
  * passes; the caller's stream is not synchronised with torch's. */
 int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids, const int* tracers,
-                          int nt, const double* e_dev, const double* ghost_dev, int n_passes,
-                          const orbit_opts* opts, double* orbit_dev, orbit_info* info);
+                          int nt, const double* e_dev, const double* ghost_dev, const int* ghost_tracers,
+                          int n_passes, const orbit_opts* opts, double* orbit_dev, orbit_info* info);
 
 /* Frozen L_AD from a device-resident grid and a device baseline state. */
 int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,

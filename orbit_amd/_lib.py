@@ -153,7 +153,7 @@ def _declare(lib):
         "orbit_species_diagonals": (_I, [_P, _I, _I, _I, _P, _P, _P, _P]),
         "orbit_fct_source_batch_dev": (_I, [_P, _P, _I, _P, C.c_longlong, _P, _D, _I, _P, C.c_longlong,
                                             _P, C.c_longlong]),
-        "orbit_forward_fct_dev": (_I, [_P, _P, _P, _P, _I, _P, _P, _I, C.POINTER(Opts), _P, _P]),
+        "orbit_forward_fct_dev": (_I, [_P, _P, _P, _P, _I, _P, _P, _P, _I, C.POINTER(Opts), _P, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

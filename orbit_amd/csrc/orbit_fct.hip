@@ -348,7 +348,8 @@ constexpr int YF_W = FT_I + 2, YF_H = FT_J + 3;  // y faces: lo j in [j0-2, j0+F
 constexpr int RR_W = FT_I + 2, RR_H = FT_J + 2;  // R: the tile and its 1-cell ring
 
 __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const double* __restrict__ c, long long cs,
-                                                  const double* __restrict__ cg, long long gs, double dt, int limited,
+                                                  const double* __restrict__ cg, long long gs,
+                                                  const int* __restrict__ gmask, double dt, int limited,
                                                   double* __restrict__ d, long long ds,
                                                   const double* __restrict__ base, long long bs) {
     __shared__ double sx[FX_H][FX_W];
@@ -362,8 +363,9 @@ __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const d
     const int k = blockIdx.z % g.nz, v = blockIdx.z / g.nz;
     const size_t lvl = (size_t)k * ny * nx;
     const double* cv = c + (size_t)v * cs;
-    const double* gv = cg ? cg + (size_t)v * gs : nullptr;
-    const bool has_g = gv != nullptr;
+    // ghost of this vector: cg's row v, unless gmask says it has none (closed edges)
+    const bool has_g = cg != nullptr && (gmask == nullptr || gmask[v] != 0);
+    const double* gv = has_g ? cg + (size_t)v * gs : nullptr;
     for (int t = threadIdx.x; t < FX_H * FX_W; t += blockDim.x) {
         const int jj = t / FX_W, ii = t - jj * FX_W;
         const int j = j0 - FT_H + jj, i = i0 - FT_H + ii;
@@ -729,8 +731,8 @@ int fct_source_impl(orbit_ctx* ctx, const orbit_grid_metrics* gm, bool host, con
         const FctGeo gv = geo_view(geo, gm->nz, gm->ny, gm->nx);
         hipLaunchKernelGGL(k_fct_geometry, g1(N), dim3(256), 0, st, M.fg, (double*)gv.gx, (double*)gv.gy,
                            (double*)gv.invx, (double*)gv.rho);
-        hipLaunchKernelGGL(k_fct_tile, tile_grid(M.fg, 1), dim3(256), 0, st, M.fg, gv, dc, 0LL, dg, 0LL, dt, limited,
-                           dd, 0LL, (const double*)nullptr, 0LL);
+        hipLaunchKernelGGL(k_fct_tile, tile_grid(M.fg, 1), dim3(256), 0, st, M.fg, gv, dc, 0LL, dg, 0LL,
+                           (const int*)nullptr, dt, limited, dd, 0LL, (const double*)nullptr, 0LL);
         if (hipGetLastError() != hipSuccess) rc = fail(ORBIT_ERR_HIP, "fct: kernel launch failed");
     }
     if (!rc && host && hipMemcpyAsync(d, dd, N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -1065,9 +1067,10 @@ extern "C" int orbit_fct_grid_destroy(orbit_fct_grid* G) {
 
 namespace {
 // The batched source with the ghost rows on a stride of their own.
+// gmask_dev (nullable, device, nvec ints): vector v reads the ghost only where gmask_dev[v] != 0.
 int fct_source_batch(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, const double* c_dev, long long c_stride,
                      const double* ghost_dev, long long ghost_stride, double dt, int limited, const double* base_dev,
-                     long long base_stride, double* out_dev, long long out_stride) {
+                     long long base_stride, double* out_dev, long long out_stride, const int* gmask_dev = nullptr) {
     if (!ctx || !G || !c_dev || !out_dev || nvec < 1) return fail(ORBIT_ERR_INVALID, "fct_source_batch: bad argument");
     if (!(dt > 0.0)) return fail(ORBIT_ERR_INVALID, "fct_source_batch: dt must be positive");
     if (nvec > 65535 || (long long)nvec * G->fg.nz > 65535)
@@ -1087,7 +1090,8 @@ int fct_source_batch(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, const double* 
         !(base_dev == out_dev && base_stride == out_stride))
         return fail(ORBIT_ERR_INVALID, "fct_source_batch: base overlaps out other than as out itself");
     hipLaunchKernelGGL(k_fct_tile, tile_grid(G->fg, nvec), dim3(256), 0, st, G->fg, G->geo, c_dev, c_stride, ghost_dev,
-                       ghost_stride, dt, limited, out_dev, out_stride, base_dev, base_stride);
+                       ghost_stride, ghost_dev ? gmask_dev : (const int*)nullptr, dt, limited, out_dev, out_stride,
+                       base_dev, base_stride);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
@@ -1112,7 +1116,8 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
 // sweeps for the FP32 error of that change instead of a cold check.
 extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids,
                                      const int* tracers, int nt, const double* e_dev, const double* ghost_dev,
-                                     int n_passes, const orbit_opts* opts, double* orbit_dev, orbit_info* info) {
+                                     const int* ghost_tracers, int n_passes, const orbit_opts* opts,
+                                     double* orbit_dev, orbit_info* info) {
     if (!ctx || !op || !grids || !tracers || nt < 1 || !e_dev || !orbit_dev || !info)
         return fail(ORBIT_ERR_INVALID, "forward_fct: NULL argument or empty tracer list");
     if (n_passes < 1) return fail(ORBIT_ERR_INVALID, "forward_fct: n_passes must be >= 1");
@@ -1135,15 +1140,21 @@ extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_
     const bool inc = o.precision == ORBIT_PREC_MIXED && n_passes > 1;
     const size_t ne = (size_t)nt * nbn * N, n16 = (size_t)op->n16;
     // scratch: two forcings (this pass's and, mixed, the previous pass's in
-    // turn), c0 (FP64 passes) and s~ (mixed passes)
+    // turn), c0 (FP64 passes), s~ (mixed passes) and the per-tracer ghost flags
     double *ebuf[2] = {nullptr, nullptr}, *c0 = nullptr, *s_state = nullptr;
+    int* gmask = nullptr;
     if (n_passes > 1) {
-        int rc0 = ctx->ensure(8, (2 * ne + (size_t)nt * N + (size_t)nt * n16) * 8);
+        int rc0 = ctx->ensure(8, (2 * ne + (size_t)nt * N + (size_t)nt * n16) * 8 + (size_t)nt * 4);
         if (rc0) return rc0;
         ebuf[0] = ctx->buf<double>(8);
         ebuf[1] = ebuf[0] + ne;
         c0 = ebuf[1] + ne;
         s_state = c0 + (size_t)nt * N;
+        if (ghost_dev && ghost_tracers) {
+            gmask = reinterpret_cast<int*>(s_state + (size_t)nt * n16);
+            ORBIT_HIP_CHECK(hipMemcpyAsync(gmask, ghost_tracers, (size_t)nt * 4, hipMemcpyHostToDevice, st));
+            ORBIT_HIP_CHECK(hipStreamSynchronize(st));  // the caller's array may go away
+        }
     }
     PassDelta pd;
     pd.s_state = s_state;
@@ -1159,7 +1170,7 @@ extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_
         for (int tau = 0; tau < nbn; ++tau) {
             rc = fct_source_batch(ctx, grids[tau], nt, orbit_dev + (size_t)(tau + 1) * N, os,
                                   ghost_dev ? ghost_dev + (size_t)tau * N : nullptr, es, op->dtau, 1,
-                                  e_dev + (size_t)tau * N, es, e_eff + (size_t)tau * N, es);
+                                  e_dev + (size_t)tau * N, es, e_eff + (size_t)tau * N, es, gmask);
             if (rc) return rc;
         }
         if (inc) {

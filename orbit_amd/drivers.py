@@ -39,17 +39,17 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
     warm: "orbit" starts each pass from the whole
     previous orbit (mixed precision), "c0" from its c0 only.  ctx: the context
     (stream) the solves run on, the FCT grids' own (default: the operator set's).
-    native: with warm "c0" and no ghost or one ghost tensor for all tracers, run
-    every pass inside the library (orbit_forward_fct_dev, no return to Python
-    between passes; FP64: the same launches as the loop here; mixed: each later
-    pass starts from the previous orbit plus the FP32 propagation of the
-    forcing's change, fewer accurate sweeps); else, or with native=False, the
-    passes are driven from here.
+    native: with warm "c0", run every pass inside the library
+    (orbit_forward_fct_dev, no return to Python between passes; a ghost list
+    goes in as one tensor with per-tracer flags; FP64: the same launches as
+    the loop here; mixed: each later pass starts from the previous orbit plus
+    the FP32 propagation of the forcing's change, fewer accurate sweeps);
+    with native=False the passes are driven from here.
     Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
     ctx = ctx or op.ctx
     if any(g.ctx is not ctx for g in fct_grids):
         raise ValueError("FCT grids and the solves must share one context (stream)")
-    if native and warm == "c0" and (ghost is None or torch.is_tensor(ghost)):
+    if native and warm == "c0":
         return _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, ghost, maxiter, ctx, **kw)
     kw["ctx"] = ctx
     orbit, info = op.solve_forward(e, tracers, tol=tol, precision=precision, maxiter=maxiter, **kw)
@@ -108,6 +108,18 @@ def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, gho
     if len(fct_grids) != op.n_bins:
         raise ValueError("one FCT grid per bin")
     e = e.contiguous()
+    flags = None
+    if ghost is not None and not torch.is_tensor(ghost):
+        # per-tracer list, None for a tracer without boundary values: one tensor
+        # (zeros where there is none) and the flags that say which rows to read
+        if len(ghost) != nt:
+            raise ValueError("ghost needs one entry per tracer")
+        if all(g is None for g in ghost):
+            ghost = None
+        else:
+            flags = np.ascontiguousarray([g is not None for g in ghost], dtype=np.int32)
+            ghost = torch.stack([torch.zeros_like(e[q]) if g is None else g.to(e.device, torch.float64)
+                                 for q, g in enumerate(ghost)])
     if ghost is not None:
         if tuple(ghost.shape) != tuple(e.shape):
             raise ValueError("ghost must have the shape of e")
@@ -119,7 +131,8 @@ def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, gho
     out = torch.empty((nt, op.n_bins + 1, op.N), dtype=torch.float64, device=e.device)
     torch.cuda.current_stream(e.device).synchronize()  # the library runs on its own stream
     rc = lib().orbit_forward_fct_dev(ctx.handle, op.handle, grids, tr.ctypes.data, nt, e.data_ptr(),
-                                     None if ghost is None else ghost.data_ptr(), int(n_passes),
+                                     None if ghost is None else ghost.data_ptr(),
+                                     None if flags is None else flags.ctypes.data, int(n_passes),
                                      C.byref(opts), out.data_ptr(), info)
     check(rc)
     return out, [[info[p * nt + s].as_dict() for s in range(nt)] for p in range(n_passes)]

@@ -111,3 +111,25 @@ def test_mixed_pass_not_converging_is_a_warning(month):
     a, h = drivers.forward_fct(op, grids, e, n_passes=3, tol=1e-14, precision="mixed", maxiter=1, restart=2)
     assert torch.isfinite(a).all()
     assert any(not i["converged"] for p in h for i in p)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-8)])
+def test_per_tracer_ghosts_run_natively(month, precision, tol):
+    """A ghost for some tracers only (the lateral boundary file of some species):
+    the library takes one tensor with per-tracer flags and reads the ghost rows
+    of the flagged tracers only.  FP64: the same bits as the Python loop's two
+    groups of launches; mixed: the 1e-6 bar against it; the tracers without a
+    ghost get the same bits as in a run with no ghost at all."""
+    wl, op, grids, e, ghost = month
+    gl = [None, ghost[1], None]
+    a, ha = drivers.forward_fct(op, grids, e, n_passes=3, tol=tol, precision=precision, ghost=gl)
+    b, _ = drivers.forward_fct(op, grids, e, n_passes=3, tol=tol, precision=precision, ghost=gl, native=False)
+    plain, _ = drivers.forward_fct(op, grids, e, n_passes=3, tol=tol, precision=precision)
+    assert all(i["converged"] for h in ha for i in h)
+    if precision == "fp64":
+        assert torch.equal(a, b)
+    else:
+        for t in range(3):
+            assert _rel(a[t, 0], b[t, 0]) <= 1e-6, t
+    assert torch.equal(a[0], plain[0]) and torch.equal(a[2], plain[2])
+    assert not torch.equal(a[1], plain[1])
