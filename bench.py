@@ -94,8 +94,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only (profiling runs)")
     ap.add_argument("--tracers", type=int, default=12)
-    ap.add_argument("--plain-tracers", type=int, default=2,
-                    help="tracers timed one at a time with the plain FP64 solve")
+    ap.add_argument("--plain-tracers", type=int, default=12,
+                    help="tracers timed one at a time with the plain FP64 solve (each one's c0 is "
+                         "also the reference its mixed-precision c0 is compared with)")
     ap.add_argument("--roof-reps", type=int, default=400,
                     help="back-to-back half-sweep launches timed for the roofline")
     ap.add_argument("--calib-copy", action="store_true",
@@ -180,6 +181,7 @@ def main():
         "plain_fp64_value": round(plain_value, 3),
         "speedup_vs_plain_fp64": round(value / plain_value, 2),
         "max_rel_err_c0_vs_fp64": max(errs),
+        "rel_err_c0_vs_fp64_per_tracer": [float(f"{x:.3e}") for x in errs],
         "gmres_iters_mixed_per_pass": iters_m[-1] if iters_m else None,
         # the operator set's calibrated sweeps: relaxation omega of the red-black
         # correction, measured contraction (with its 20% margin), sweep counts
