@@ -2179,9 +2179,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, -1.0, S.c, vs, (int)n16, S.B.part_dev);
     }
     // adjoint output convention: state 0 and state nb are the GMRES solution lam0
-    std::vector<double> cnorm;
-    rc = norms(S, S.c, vs, (int)n16, cnorm);
-    if (rc) return rc;
+    norms_dev(S, S.c, vs, (int)n16, S.h1);  // ||c||^2, read back with the negative counts
     if (dir == ADJ)
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, S.orbit + (size_t)nbn * n16,
                            os, 0.0, S.orbit, os, (int)n16, nullptr);
@@ -2192,6 +2190,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         hipLaunchKernelGGL(k_negcount, dim3(nb, nt), dim3(RED_THREADS), 0, st, S.orbit, os, n, nb, S.partials);
         hipLaunchKernelGGL(k_sum_partials, grid_waves(nt), dim3(256), 0, st, S.partials, nb, S.scal, nt);
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.scal, nt * 8, hipMemcpyDeviceToHost, st));
+        ORBIT_HIP_CHECK(hipMemcpyAsync(S.host + nt, S.h1, nt * 8, hipMemcpyDeviceToHost, st));
     }
     hipLaunchKernelGGL(k_int_to_ext, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, S.orbit, os, out_dev,
                        (long long)(nbn + 1) * op->N, nbn + 1, op->nz, op->ncol);
@@ -2206,7 +2205,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         I.zero_source = zero[s];
         I.s_norm = snorm[s];
         I.residual = zero[s] ? 0.0 : last_rel[s];
-        I.periodicity = zero[s] ? 0.0 : (last_rel[s] * snorm[s]) / (cnorm[s] + 1e-30);
+        I.periodicity = zero[s] ? 0.0 : (last_rel[s] * snorm[s]) / (std::sqrt(S.host[nt + s]) + 1e-30);
         I.neg_frac = S.host[s] / ((double)(nbn + 1) * op->N);
         if (!conv[s]) all_conv = false;
     }
