@@ -1890,6 +1890,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.rho = op->rho;
     S.eps_acc = std::min(1e-12, 0.01 * o.tol);
     S.inner = o.inner_sweeps > 0 ? o.inner_sweeps : op->def_inner;
+    const double eps_in = std::pow(S.rho, S.inner);  // relative error of one inexact (FP32) P apply
     S.acc = o.acc_sweeps > 0 ? o.acc_sweeps
                              : std::min(80, std::max(4, (int)std::ceil(std::log(S.eps_acc) / std::log(S.rho))));
     S.m = o.restart;
@@ -2018,7 +2019,6 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             // solves so far, rho^inner relative to their outputs, which are no larger
             // than max_t ||y_t|| (the propagators contract).  A worst-case sum with
             // rho's calibration margin: no extra sweep on top.
-            const double eps_in = std::pow(S.rho, S.inner);
             first_nsw.assign(nt, S.acc);
             for (int s = 0; s < nt; ++s) {
                 if (!(cn[s] > 0.0)) continue;
@@ -2030,7 +2030,6 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             }
         } else if (warm_dev && o.warm_orbit) {
             // whole previous orbit: its states start the first accurate sweep
-            const long long os_ = (long long)(nbn + 1) * n16;
             hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
                                (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
             const double* c0 = dir == FWD ? S.orbit : S.orbit + (size_t)nbn * n16;
@@ -2079,7 +2078,6 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     // cannot fall below the FP32 sweep's error in s~ (about rho^inner relative),
     // so it stops there instead of at 1e-6 (the accurate check then measures).
     bool on_s_tilde = S.prec == ORBIT_PREC_MIXED && !need_check && !S.sv;
-    const double eps_in = std::pow(S.rho, S.inner);
     bool have_orbit = warm_first > 0;
     int next_sweeps = warm_first > 0 ? warm_first : S.acc;
     if (warm_first > 0) {
