@@ -110,3 +110,36 @@ def test_adjoint_driver_gradient_reproduces_the_marginal_driver(case):
         dc, _ = drivers_ref.marginal_fct_gmres(lus, list(de), L_AD, tol=1e-13, max_iters=80, rel_tol=1e-11)
         J = sum(float(np.dot(R[t], dc[t])) for t in range(N_BINS))
         assert J == pytest.approx(float(np.sum(grad * de)), rel=1e-8, abs=1e-12 * np.abs(grad).sum()), trial
+
+
+def test_pass_states_split_into_previous_states_plus_the_propagated_change():
+    """The identity the mixed forward passes start from (orbit_forward_fct_dev):
+    the bin states swept from a fixed c0 are linear in the forcing, so with
+    forcing e they equal the states with e_prev plus the sweep of
+    dtau (e - e_prev) from zero -- and the periodic residual's source part
+    s(e) = s(e_prev) + that sweep's last state.  Checked with SuperLU
+    propagators; a sweep of the change from c0 instead of zero, or the change
+    without dtau, misses it."""
+    wl = synth.make_workload(10, 8, 5, 1, seed=44, convection=True)
+    lus = [orb.factor_one_bin(wl.L(t, 0)) for t in range(8)]
+    rng = np.random.default_rng(45)
+    e_prev = [wl.e[0, t] for t in range(8)]
+    e_new = [x + 0.1 * rng.standard_normal(x.shape) * np.abs(x).max() for x in e_prev]
+    c0 = rng.random(wl.N)
+
+    def sweep(c, e):
+        x = [c]
+        for t in range(8):
+            x.append(lus[t].solve(x[-1] + wl.dtau * e[t]))
+        return x
+
+    old, new = sweep(c0, e_prev), sweep(c0, e_new)
+    y = sweep(np.zeros(wl.N), [a - b for a, b in zip(e_new, e_prev)])
+    for t in range(1, 9):
+        assert _rel(old[t] + y[t], new[t]) < 1e-12, t
+    s_new = orb.compute_s(lus, e_new)
+    assert _rel(orb.compute_s(lus, e_prev) + y[8], s_new) < 1e-12
+    wrong = sweep(c0, [a - b for a, b in zip(e_new, e_prev)])
+    assert _rel(old[8] + wrong[8], new[8]) > 1e-3
+    y_nodt = sweep(np.zeros(wl.N), [(a - b) / wl.dtau for a, b in zip(e_new, e_prev)])
+    assert _rel(old[8] + y_nodt[8], new[8]) > 1e-3
