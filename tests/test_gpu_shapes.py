@@ -1,7 +1,8 @@
 """Edge shapes on the device against the oracle: a single column, a single
 row of columns (ny = 1) and a single column of rows (nx = 1), odd sizes, one
 level (nz = 1) and the largest column (nz = 16), for the propagator, the
-forward and adjoint periodic solves in both precisions, the FCT source and the
+forward and adjoint periodic solves in both precisions, the FCT source, the
+forward FCT passes (mixed, in the library) and the
 transport assembly."""
 import types
 
@@ -64,3 +65,25 @@ def test_assembly_on_edge_shapes(nx, ny, nz):
     ref = asm.transport_stencil(g)
     for k in ref:
         assert np.abs(st[k] - ref[k]).max() <= 1e-12 * max(np.abs(ref[k]).max(), 1e-300), k
+
+
+@pytest.mark.parametrize("nx,ny,nz", [(1, 6, 8), (7, 1, 8), (3, 2, 16), (2, 2, 1)])
+def test_forward_fct_passes_on_edge_shapes(nx, ny, nz):
+    """The library's forward FCT passes (mixed precision: later passes start from
+    the previous orbit plus the propagated change) against the oracle's passes."""
+    import torch
+
+    from oracle import drivers_ref
+    from orbit_amd import drivers
+    wl = synth.make_workload(nx, ny, nz, 2, seed=[11, nx, ny, nz], convection=nz > 1, wind_std=6.0)
+    e = wl.e.copy()
+    e[:, :, : nx * ny] += 1e-6
+    op = OperatorSet.from_workload(wl)
+    grids = drivers.make_fct_grids(wl.grids, op.ctx)
+    got, hist = drivers.forward_fct(op, grids, torch.from_numpy(e).cuda(), n_passes=3, tol=1e-8, precision="mixed")
+    got = got.cpu().numpy()
+    for s in range(2):
+        lus = [orb.factor_one_bin(wl.L(t, s)) for t in range(8)]
+        ref, _ = drivers_ref.forward_fct_passes(lus, list(e[s]), wl.grids, n_passes=3, tol=1e-12, maxiter=500)
+        assert all(h[s]["converged"] for h in hist)
+        assert _rel(got[s, 0], ref[0]) < 1e-6 and _rel(got[s, 5], ref[5]) < 1e-6, s
