@@ -347,11 +347,13 @@ constexpr int XF_W = FT_I + 3, XF_H = FT_J + 2;  // x faces: lo i in [i0-2, i0+F
 constexpr int YF_W = FT_I + 2, YF_H = FT_J + 3;  // y faces: lo j in [j0-2, j0+FT_J], i in [i0-1, i0+FT_I]
 constexpr int RR_W = FT_I + 2, RR_H = FT_J + 2;  // R: the tile and its 1-cell ring
 
-__global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const double* __restrict__ c, long long cs,
-                                                  const double* __restrict__ cg, long long gs,
-                                                  const int* __restrict__ gmask, double dt, int limited,
-                                                  double* __restrict__ d, long long ds,
-                                                  const double* __restrict__ base, long long bs) {
+// One tile (blockIdx.x, blockIdx.y) of level k of vector v, for the kernels below.
+__device__ __forceinline__ void fct_tile(const FctGrid& g, const FctGeo& geo, int k, int v,
+                                         const double* __restrict__ c, long long cs,
+                                         const double* __restrict__ cg, long long gs,
+                                         const int* __restrict__ gmask, double dt, int limited,
+                                         double* __restrict__ d, long long ds,
+                                         const double* __restrict__ base, long long bs) {
     __shared__ double sx[FX_H][FX_W];
     __shared__ double xfA[XF_H][XF_W], xfL[XF_H][XF_W], xfH[XF_H][XF_W];
     __shared__ double yfA[YF_H][YF_W], yfL[YF_H][YF_W], yfH[YF_H][YF_W];
@@ -360,7 +362,6 @@ __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const d
     __shared__ unsigned char xfV[XF_H][XF_W], yfV[YF_H][YF_W];
     const int nx = g.nx, ny = g.ny;
     const int i0 = blockIdx.x * FT_I, j0 = blockIdx.y * FT_J;
-    const int k = blockIdx.z % g.nz, v = blockIdx.z / g.nz;
     const size_t lvl = (size_t)k * ny * nx;
     const double* cv = c + (size_t)v * cs;
     // ghost of this vector: cg's row v, unless gmask says it has none (closed edges)
@@ -518,6 +519,36 @@ __global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const d
         const double val = acc * geo.rho[n];
         d[(size_t)v * ds + n] = base ? base[(size_t)v * bs + n] + val : val;
     }
+}
+
+// One bin: blockIdx.z = v * nz + k.
+__global__ __launch_bounds__(256) void k_fct_tile(FctGrid g, FctGeo geo, const double* __restrict__ c, long long cs,
+                                                  const double* __restrict__ cg, long long gs,
+                                                  const int* __restrict__ gmask, double dt, int limited,
+                                                  double* __restrict__ d, long long ds,
+                                                  const double* __restrict__ base, long long bs) {
+    fct_tile(g, geo, blockIdx.z % g.nz, blockIdx.z / g.nz, c, cs, cg, gs, gmask, dt, limited, d, ds, base, bs);
+}
+
+// Every bin of a pass in one launch (the bins' grids share the shape):
+// blockIdx.z = (bin * nvec + v) * nz + k; the bin's grid and geometry come from
+// bins[bin], and c, cg, d and base advance by their per-bin offsets.
+struct FctBin {
+    FctGrid g;
+    FctGeo geo;
+};
+__global__ __launch_bounds__(256) void k_fct_tile_bins(const FctBin* __restrict__ bins, int nvec, int nz,
+                                                       const double* __restrict__ c, long long cs, long long cb,
+                                                       const double* __restrict__ cg, long long gs, long long gb,
+                                                       const int* __restrict__ gmask, double dt, int limited,
+                                                       double* __restrict__ d, long long ds, long long db,
+                                                       const double* __restrict__ base, long long bs, long long bb) {
+    const int k = blockIdx.z % nz, vb = blockIdx.z / nz;
+    const int bin = vb / nvec, v = vb - bin * nvec;
+    const FctGrid g = bins[bin].g;
+    const FctGeo geo = bins[bin].geo;
+    fct_tile(g, geo, k, v, c + (size_t)bin * cb, cs, cg ? cg + (size_t)bin * gb : nullptr, gs, gmask, dt, limited,
+             d + (size_t)bin * db, ds, base ? base + (size_t)bin * bb : nullptr, bs);
 }
 
 // Frozen gains of L_AD: per x face (nz,ny,nx-1) and y face (nz,ny-1,nx), w_lo and w_hi.
@@ -1143,18 +1174,25 @@ extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_
     // turn), c0 (FP64 passes), s~ (mixed passes) and the per-tracer ghost flags
     double *ebuf[2] = {nullptr, nullptr}, *c0 = nullptr, *s_state = nullptr;
     int* gmask = nullptr;
+    FctBin* bins = nullptr;  // every bin's grid, for one FCT launch per pass
+    const size_t nbin_bytes = ((size_t)nbn * sizeof(FctBin) + 255) / 256 * 256;
+    const bool one_launch = (long long)op->nz * nt * nbn <= 65535;
     if (n_passes > 1) {
-        int rc0 = ctx->ensure(8, (2 * ne + (size_t)nt * N + (size_t)nt * n16) * 8 + (size_t)nt * 4);
+        int rc0 = ctx->ensure(8, nbin_bytes + (2 * ne + (size_t)nt * N + (size_t)nt * n16) * 8 + (size_t)nt * 4);
         if (rc0) return rc0;
-        ebuf[0] = ctx->buf<double>(8);
+        bins = ctx->buf<FctBin>(8);
+        ebuf[0] = reinterpret_cast<double*>(reinterpret_cast<char*>(bins) + nbin_bytes);
         ebuf[1] = ebuf[0] + ne;
         c0 = ebuf[1] + ne;
         s_state = c0 + (size_t)nt * N;
         if (ghost_dev && ghost_tracers) {
             gmask = reinterpret_cast<int*>(s_state + (size_t)nt * n16);
             ORBIT_HIP_CHECK(hipMemcpyAsync(gmask, ghost_tracers, (size_t)nt * 4, hipMemcpyHostToDevice, st));
-            ORBIT_HIP_CHECK(hipStreamSynchronize(st));  // the caller's array may go away
         }
+        std::vector<FctBin> hb(nbn);
+        for (int b = 0; b < nbn; ++b) hb[b] = FctBin{grids[b]->fg, grids[b]->geo};
+        ORBIT_HIP_CHECK(hipMemcpyAsync(bins, hb.data(), (size_t)nbn * sizeof(FctBin), hipMemcpyHostToDevice, st));
+        ORBIT_HIP_CHECK(hipStreamSynchronize(st));  // the host arrays may go away
     }
     PassDelta pd;
     pd.s_state = s_state;
@@ -1167,11 +1205,23 @@ extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_
     const double* e_prev = e_dev;  // the forcing the current orbit was solved with
     for (int p = 1; p < n_passes; ++p) {
         double* e_eff = inc ? ebuf[(p - 1) & 1] : ebuf[0];
-        for (int tau = 0; tau < nbn; ++tau) {
-            rc = fct_source_batch(ctx, grids[tau], nt, orbit_dev + (size_t)(tau + 1) * N, os,
-                                  ghost_dev ? ghost_dev + (size_t)tau * N : nullptr, es, op->dtau, 1,
-                                  e_dev + (size_t)tau * N, es, e_eff + (size_t)tau * N, es, gmask);
-            if (rc) return rc;
+        if (one_launch) {
+            // every bin at once: the same per-(bin, tracer, level) tiles as a launch per bin
+            const FctGrid& g0 = grids[0]->fg;
+            const dim3 grid((unsigned)((g0.nx + FT_I - 1) / FT_I), (unsigned)((g0.ny + FT_J - 1) / FT_J),
+                            (unsigned)(g0.nz * nt * nbn));
+            hipLaunchKernelGGL(k_fct_tile_bins, grid, dim3(256), 0, st, (const FctBin*)bins, nt, g0.nz,
+                               (const double*)(orbit_dev + N), os, (long long)N, ghost_dev, es, (long long)N,
+                               (const int*)(ghost_dev ? gmask : nullptr), op->dtau, 1, e_eff, es, (long long)N,
+                               e_dev, es, (long long)N);
+            ORBIT_HIP_CHECK(hipGetLastError());
+        } else {
+            for (int tau = 0; tau < nbn; ++tau) {
+                rc = fct_source_batch(ctx, grids[tau], nt, orbit_dev + (size_t)(tau + 1) * N, os,
+                                      ghost_dev ? ghost_dev + (size_t)tau * N : nullptr, es, op->dtau, 1,
+                                      e_dev + (size_t)tau * N, es, e_eff + (size_t)tau * N, es, gmask);
+                if (rc) return rc;
+            }
         }
         if (inc) {
             orbit_opts oi = o;
