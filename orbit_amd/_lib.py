@@ -106,8 +106,9 @@ _D = C.c_double
 _PP = C.POINTER(C.c_void_p)
 
 
-def _declare(lib):
-    sig = {
+def signatures():
+    """{symbol: (restype, argtypes)} of every entry point include/orbit_hip.h declares."""
+    return {
         "orbit_last_error": (C.c_char_p, []),
         "orbit_opts_default": (None, [C.POINTER(Opts)]),
         "orbit_ctx_create": (_I, [_I, C.POINTER(_P)]),
@@ -155,7 +156,10 @@ def _declare(lib):
                                             _P, C.c_longlong]),
         "orbit_forward_fct_dev": (_I, [_P, _P, _P, _P, _I, _P, _P, _P, _I, C.POINTER(Opts), _P, _P]),
     }
-    for name, (res, args) in sig.items():
+
+
+def _declare(lib):
+    for name, (res, args) in signatures().items():
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args

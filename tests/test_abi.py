@@ -54,3 +54,26 @@ def test_oracle_is_not_imported_by_the_product():
             if fn.endswith(".py"):
                 txt = open(os.path.join(dirpath, fn)).read()
                 assert not re.search(r"^\s*(from|import)\s+oracle\b", txt, flags=re.M), fn
+
+
+def _param_counts():
+    """Parameter count of every function include/orbit_hip.h declares."""
+    out = {}
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        txt = open(os.path.join(ROOT, "include", fn)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b(orbit_[A-Za-z0-9_]+)\s*\(([^;{]*?)\)\s*;", txt, flags=re.S):
+            params = m.group(2).strip()
+            out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_bindings_pass_as_many_arguments_as_the_header_declares():
+    from orbit_amd import _lib
+    counts = _param_counts()
+    sig = _lib.signatures()
+    assert set(sig) == set(counts)
+    bad = {k: (len(sig[k][1]), counts[k]) for k in sig if len(sig[k][1]) != counts[k]}
+    assert not bad, bad
