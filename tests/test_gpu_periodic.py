@@ -92,3 +92,32 @@ def test_four_column_ring_matches_oracle():
     for prec, tol, bar in (("fp64", 1e-11, 1e-8), ("mixed", 1e-8, 1e-6)):
         orbit, info = op.solve_forward(wl.e, tol=tol, precision=prec)
         assert info[0]["converged"] and _rel(orbit[0, 0], ref[0]) < bar, prec
+
+
+@pytest.mark.parametrize("precision,tol,bar", [("fp64", 1e-11, 1e-8), ("mixed", 1e-8, 1e-6)])
+def test_coupled_species_waves_on_a_periodic_month(precision, tol, bar):
+    """The species layer (orbit/core/orbit.py:213-806) on a periodic-longitude
+    month: 0 -> 1 at the start of each bin, 1 -> 2 at its end, against the
+    oracle's waves over SuperLU factors of the same wrapped operators."""
+    import scipy.sparse as sp
+
+    from oracle import species_ref as spc
+    from orbit_amd import species
+    wl = synth.make_workload(10, 7, 6, 3, seed=47, convection=True, periodic=True)
+    N = wl.N
+    rng = np.random.default_rng(48)
+    K = []
+    for t in range(8):
+        off = sp.random(N, N, density=3.0 / N, random_state=int(rng.integers(1 << 30))) * 2e-6
+        K.append((sp.diags(rng.uniform(0.5, 1.0, N) * 1e-5) + off).tocsr())
+    Ks = [{(1, 0): K[t], (2, 1): 0.7 * K[t]} for t in range(8)]
+    e = np.zeros((8, 3 * N))
+    e[:, :N] = rng.random((8, N)) * (rng.random((8, N)) < 0.4)
+    lus = {s: [orb.factor_one_bin(wl.L(t, s)) for t in range(8)] for s in range(3)}
+    op = OperatorSet.from_workload(wl)
+    kw = dict(tol=tol, maxiter=300, endbin_keys={(2, 1)}, solve_order=(0, 1, 2))
+    ref = spc.solve_all_species(lus, Ks, e, N, n_species=3, **dict(kw, tol=1e-12))
+    got = species.solve_orbit_all_species(None, Ks, e, N, opset=op, verbose=False, precision=precision, **kw)
+    for s in range(3):
+        for j in (0, 4, 8):
+            assert _rel(got["orbits"][s][j], ref["orbits"][s][j]) < bar, (precision, s, j)
