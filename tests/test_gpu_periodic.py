@@ -121,3 +121,25 @@ def test_coupled_species_waves_on_a_periodic_month(precision, tol, bar):
     for s in range(3):
         for j in (0, 4, 8):
             assert _rel(got["orbits"][s][j], ref["orbits"][s][j]) < bar, (precision, s, j)
+
+
+def test_forward_fct_passes_on_a_periodic_month():
+    """The forward FCT passes (orbit/cli.py:1212-1241) on a periodic-longitude
+    month: the transport wraps, the FCT source has no wrap face (as in
+    orbit/core/fct.py:256-357, faces m = 0..nx-2); the library's mixed passes
+    against the oracle's."""
+    import torch
+
+    from oracle import drivers_ref
+    from orbit_amd import drivers
+    wl = synth.make_workload(12, 8, 6, 2, seed=49, convection=True, periodic=True, wind_std=7.0)
+    op = OperatorSet.from_workload(wl)
+    grids = drivers.make_fct_grids(wl.grids, op.ctx)
+    got, hist = drivers.forward_fct(op, grids, torch.from_numpy(wl.e).cuda(), n_passes=3, tol=1e-8,
+                                    precision="mixed")
+    got = got.cpu().numpy()
+    for s in range(2):
+        lus = [orb.factor_one_bin(wl.L(t, s)) for t in range(8)]
+        ref, _ = drivers_ref.forward_fct_passes(lus, list(wl.e[s]), wl.grids, n_passes=3, tol=1e-12, maxiter=500)
+        assert all(h[s]["converged"] for h in hist)
+        assert _rel(got[s, 0], ref[0]) < 1e-6 and _rel(got[s, 3], ref[3]) < 1e-6, s
