@@ -153,3 +153,20 @@ def test_fct_autograd_recompute_equals_stored(setup):
         orbit[:, 0, :20].sum().backward()
         gs.append(et.grad.cpu().numpy())
     assert _rel(gs[1], gs[0]) < 1e-9
+
+
+def test_fct_autograd_mixed_recompute_is_bit_identical(setup):
+    """Mixed precision: the forward passes (later passes started from the
+    previous orbit plus the propagated change) are deterministic, so recomputing
+    them in the backward gives the stored orbit bit for bit, hence the same
+    gradient; and it is within the 1e-6 bar of the FP64 gradient."""
+    wl, op, grids, lus, e = setup
+    gs = []
+    for recompute, precision in ((False, "mixed"), (True, "mixed"), (False, "fp64")):
+        et = torch.from_numpy(e[:1]).cuda().requires_grad_(True)
+        orbit = hag.orbit_solve_fct(et, op, grids, tracers=[0], n_passes=3, tol=1e-8 if precision == "mixed"
+                                    else 1e-11, precision=precision, recompute=recompute)
+        orbit[:, 0, :20].sum().backward()
+        gs.append(et.grad.cpu().numpy())
+    assert np.array_equal(gs[0], gs[1])
+    assert _rel(gs[0], gs[2]) < 1e-6
