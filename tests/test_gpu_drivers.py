@@ -170,3 +170,18 @@ def test_fct_autograd_mixed_recompute_is_bit_identical(setup):
         gs.append(et.grad.cpu().numpy())
     assert np.array_equal(gs[0], gs[1])
     assert _rel(gs[0], gs[2]) < 1e-6
+
+
+def test_marginal_fct_mixed_matches_oracle(setup):
+    """The marginal driver's outer GMRES over mixed-precision periodic solves
+    (each matvec a cold batched solve) against the oracle at the 1e-6 bar."""
+    wl, op, grids, lus, e = setup
+    ed = torch.from_numpy(e).cuda()
+    base, _ = drivers.forward_fct(op, grids, ed, n_passes=2, tol=1e-11, precision="fp64")
+    L = drivers.frozen_operators(op, grids, base[:1])
+    de = 0.01 * e[:1] * np.random.default_rng(0).lognormal(0.0, 1.0, e[:1].shape)
+    dc, info = drivers.marginal_fct(op, L, torch.from_numpy(de).cuda(), tracers=[0], tol=1e-8,
+                                    precision="mixed", max_iters=60, rel_tol=1e-8)
+    Lref = _oracle_LAD(wl, base[0].cpu().numpy())
+    ref, _ = drivers_ref.marginal_fct_gmres(lus[0], list(de[0]), Lref, tol=1e-12, max_iters=60, rel_tol=1e-9)
+    assert _rel(dc[0].cpu().numpy(), ref) < 1e-6
