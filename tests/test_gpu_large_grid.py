@@ -15,7 +15,9 @@ product per bin:
   there the mismatch is A_0^T times the solve's residual: at most
   ||A_0^T||_2 * residual * ||s_adj|| with the residual the solve reports
   (s_adj = r here);
-* mixed precision against the plain FP64 solve at 1e-6 in c0.
+* mixed precision against the plain FP64 solve at 1e-6 in c0;
+* FCT: the library's FP64 passes (one FCT launch per pass over all bins) bit-identical
+  to the per-bin launches of the Python loop, and one bin's source against the oracle.
 
 These catch what a small grid cannot: 32-bit index products in the launch
 grids, permutes and reductions at this many columns."""
@@ -90,3 +92,26 @@ def test_large_grid_adjoint_recursion(big):
                 assert err <= bar, (prec, tau, err, bar)
             else:
                 assert err <= 1e-10 * np.linalg.norm(lam[tau + 1]), (prec, tau)
+
+
+def test_large_grid_fct_passes_and_source(big):
+    """FCT at 768K cells: the library's FP64 passes (one FCT launch per pass over
+    all bins) bit-identical to the Python loop's per-bin launches, and the batched
+    source of one bin against the oracle (orbit/core/fct.py:459-492)."""
+    from oracle import fct_ref
+    from orbit_amd import drivers
+    wl, op = big
+    grids = drivers.make_fct_grids(wl.grids, op.ctx)
+    e = torch.from_numpy(wl.e).cuda()
+    a, ha = drivers.forward_fct(op, grids, e, n_passes=2, tol=1e-10, precision="fp64")
+    b, hb = drivers.forward_fct(op, grids, e, n_passes=2, tol=1e-10, precision="fp64", native=False)
+    assert torch.equal(a, b) and all(i["converged"] for h in ha for i in h)
+    tau = 3
+    c = a[:, tau + 1].contiguous()
+    d = torch.empty_like(c)
+    grids[tau].source_batch(c, d, base=None, dt=wl.dtau)
+    torch.cuda.synchronize()
+    for s in range(wl.n_tracers):
+        ref = fct_ref.compute_horizontal_fct_source(wl.grids[tau], c[s].cpu().numpy(), wl.dtau)
+        got = d[s].cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max(), s
