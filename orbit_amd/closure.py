@@ -33,7 +33,7 @@ class AndersonAccelerator:
 
     def __init__(self, m: int = 3, beta: float = 1.0, rcond: float = 1e-10, safeguard_factor: float = 1.5):
         if m < 1:
-            raise ValueError(f"Anderson memory depth m must be >= 1, got {m}")
+            raise ValueError(f"the Anderson history keeps at least one past iterate; m = {m} asks for none")
         self.m = int(m)
         self.beta = float(beta)
         self.rcond = float(rcond)

@@ -85,8 +85,9 @@ def _write(path, grids, c_bc, species=("SO2",), lat=None, nz=None):
 
 def test_boundary_file_reader_refuses_bad_files(tmp_path):
     """The reader's checks (reference lateral_bc.py:104-127) run before any
-    device work: grid mismatch, shape, non-finite or negative values and
-    unknown species are ValueErrors."""
+    device work: another grid, another c_bc layout, negative or non-finite
+    values, species the run does not carry and missing arrays are
+    BoundaryFileErrors (ValueErrors)."""
     from orbit_amd import lateral_bc
     grids = _bc_grids()
     good = np.ones((1, 8, 3, 4, 5))
@@ -98,22 +99,26 @@ def test_boundary_file_reader_refuses_bad_files(tmp_path):
     for what, kw in cases.items():
         p = tmp_path / f"{what}.npz"
         _write(p, grids, good, **kw)
-        with pytest.raises(ValueError, match="grid mismatch"):
+        with pytest.raises(ValueError, match="on another grid"):
             lateral_bc.load_lateral_bc(str(p), grids, names)
     p = tmp_path / "shape.npz"
     _write(p, grids, np.ones((1, 7, 3, 4, 5)))
-    with pytest.raises(ValueError, match="c_bc shape"):
+    with pytest.raises(ValueError, match="c_bc has layout"):
         lateral_bc.load_lateral_bc(str(p), grids, names)
     for bad in (np.nan, -1.0):
         c = good.copy()
         c[0, 2, 1, 1, 1] = bad
         p = tmp_path / "bad.npz"
         _write(p, grids, c)
-        with pytest.raises(ValueError, match="non-finite or negative"):
+        with pytest.raises(ValueError, match="finite and >= 0"):
             lateral_bc.load_lateral_bc(str(p), grids, names)
+    p = tmp_path / "nokey.npz"
+    np.savez(p, lat=grids[0].lat, lon=grids[0].lon, nz=grids[0].nz, c_bc=good)
+    with pytest.raises(lateral_bc.BoundaryFileError, match="no species"):
+        lateral_bc.load_lateral_bc(str(p), grids, names)
     p = tmp_path / "unknown.npz"
     _write(p, grids, good, species=("XYZ",))
-    with pytest.raises(ValueError, match="unknown species"):
+    with pytest.raises(ValueError, match="not among the run's"):
         lateral_bc.load_lateral_bc(str(p), grids, names)
 
 
