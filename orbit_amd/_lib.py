@@ -201,6 +201,25 @@ def default_opts(**kw) -> Opts:
     return o
 
 
+def dev_f64(t, device: int, name: str, last: int = None, rows: int = None):
+    """t itself when it is a torch float64 tensor on cuda:device (with last
+    dimension `last` and `rows` leading rows when given); else TypeError or
+    ValueError before any pointer reaches a kernel (a host pointer or a
+    narrower dtype would be read out of bounds on the device)."""
+    import torch
+    if not torch.is_tensor(t):
+        raise TypeError(f"{name} must be a torch tensor on cuda:{device}")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64, got {t.dtype}")
+    if not t.is_cuda or t.device.index != device:
+        raise ValueError(f"{name} must live on cuda:{device}, got {t.device}")
+    if last is not None and (t.dim() == 0 or t.shape[-1] != last):
+        raise ValueError(f"{name} must have last dimension {last}, got shape {tuple(t.shape)}")
+    if rows is not None and (t.dim() < 2 or t.shape[0] != rows):
+        raise ValueError(f"{name} must have {rows} rows, got shape {tuple(t.shape)}")
+    return t
+
+
 def ptr(a) -> int:
     """Address of a contiguous numpy array or torch tensor."""
     if hasattr(a, "data_ptr"):

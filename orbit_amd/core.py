@@ -224,14 +224,15 @@ class OperatorSet:
             raise ValueError(f"input must have shape {in_shape}, got {tuple(inp.shape)}")
         if device:
             import torch
-            if inp.dtype != torch.float64:
-                raise TypeError("device inputs must be float64 tensors")
-            inp = inp.contiguous()
+            inp = _lib.dev_f64(inp, ctx.device, "input").contiguous()
             out = torch.empty((nt, self.n_bins + 1, self.N), dtype=torch.float64, device=inp.device)
             wt = None
             if warm is not None:
                 wshape = (nt, self.n_bins + 1, self.N) if opts.warm_orbit else (nt, self.N)
-                wt = warm.to(torch.float64).reshape(wshape).contiguous()  # kept alive for the call
+                if not torch.is_tensor(warm):
+                    warm = torch.as_tensor(warm)
+                # kept alive for the call
+                wt = warm.to(inp.device, torch.float64).reshape(wshape).contiguous()
             # the library runs on its own stream: order it after torch's work
             torch.cuda.current_stream(inp.device).synchronize()
             rc = fn(ctx.handle, self.handle, ptr(tracers), nt, inp.data_ptr(),

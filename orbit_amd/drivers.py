@@ -103,8 +103,9 @@ def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, gho
     from ._lib import check, lib
     tracers = list(range(op.n_tracers)) if tracers is None else list(tracers)
     nt = len(tracers)
-    if tuple(e.shape) != (nt, op.n_bins, op.N) or e.dtype != torch.float64 or not e.is_cuda:
-        raise ValueError(f"e must be a CUDA float64 tensor of shape {(nt, op.n_bins, op.N)}")
+    _lib.dev_f64(e, ctx.device, "e")
+    if tuple(e.shape) != (nt, op.n_bins, op.N):
+        raise ValueError(f"e must have shape {(nt, op.n_bins, op.N)}, got {tuple(e.shape)}")
     if len(fct_grids) != op.n_bins:
         raise ValueError("one FCT grid per bin")
     e = e.contiguous()
@@ -123,7 +124,7 @@ def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, gho
     if ghost is not None:
         if tuple(ghost.shape) != tuple(e.shape):
             raise ValueError("ghost must have the shape of e")
-        ghost = ghost.to(torch.float64).contiguous()
+        ghost = ghost.to(e.device, torch.float64).contiguous()
     opts = op._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps, use_graph, max_rounds, verbose)
     tr = np.ascontiguousarray(tracers, dtype=np.int32)
     grids = (C.c_void_p * op.n_bins)(*[g.handle.value for g in fct_grids])

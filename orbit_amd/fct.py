@@ -153,8 +153,12 @@ class FctGrid:
         sync=False leaves the launch queued on the context's stream (for a
         consumer on that same stream)."""
         nvec = c.shape[0]
-        for t in (c, out) + ((base,) if base is not None else ()) + ((ghost,) if ghost is not None else ()):
-            if t.stride(-1) != 1 or t.shape[-1] != self.N:
+        dev = self.ctx.device
+        for name, t in (("c", c), ("out", out), ("base", base), ("ghost", ghost)):
+            if t is None:
+                continue
+            _lib_dev_f64(t, dev, name, last=self.N, rows=nvec)
+            if t.stride(-1) != 1:
                 raise ValueError("FCT batch views need unit-stride rows of length N")
         if ghost is not None and nvec > 1 and ghost.stride(0) != c.stride(0):
             # the entry point reads ghost rows on c's stride: lay them out so,
@@ -180,6 +184,9 @@ class FctGrid:
         L.N = self.N
         L.shape = (self.N, self.N)
         h = C.c_void_p()
+        c_base = _lib_dev_f64(c_base, self.ctx.device, "c_base", last=self.N).contiguous()
+        if c_base.numel() != self.N:
+            raise ValueError(f"c_base must hold N = {self.N} values")
         _torch_sync(c_base)
         check(lib().orbit_fct_linop_create_dev(self.ctx.handle, self.handle, c_base.data_ptr(), float(dt),
                                                C.byref(h)))
@@ -198,6 +205,11 @@ class FctGrid:
             pass
 
 
+def _lib_dev_f64(t, device, name, last=None, rows=None):
+    from ._lib import dev_f64
+    return dev_f64(t, device, name, last=last, rows=rows)
+
+
 def _torch_sync(t):
     """The library launches on its own stream: finish torch's pending work on t first."""
     import torch
@@ -206,6 +218,10 @@ def _torch_sync(t):
 
 def apply_dev(L: FctLinearOperator, x, out, trans=False):
     """out = L_AD x (or L_AD^T x) for CUDA float64 vectors of length N (synchronous)."""
+    for name, t in (("x", x), ("out", out)):
+        _lib_dev_f64(t, L.ctx.device, name, last=L.N)
+        if t.numel() != L.N or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous vector of N = {L.N} values")
     _torch_sync(x)
     check(lib().orbit_fct_linop_apply_dev(L.ctx.handle, L.handle, int(bool(trans)), x.data_ptr(),
                                           out.data_ptr()))

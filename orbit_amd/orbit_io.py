@@ -28,19 +28,38 @@ def orbit_mean(c_orbit):
 
 
 def save_orbit_npz(path: str, c_orbit, species_order: Optional[Sequence[int]] = None,
-                   info: Optional[Sequence[dict]] = None, month: Optional[int] = None, **extra) -> None:
+                   info: Optional[Sequence[dict]] = None, month: Optional[int] = None,
+                   iteration: int = 0, converged: Optional[bool] = None, **extra) -> None:
     """c_orbit: (S, n_bins+1, N) device tensor or array; info: per-species dicts
-    of the solve (gmres_iters, residual, periodicity)."""
+    of the solve (gmres_iters, residual, periodicity).  Every key the
+    reference's checkpoint reader takes (orbit/cli.py:1387-1407: c_orbit,
+    orbit_species_order, gmres_iters, gmres_resid, periodicity, iter,
+    converged) is always written: without info the per-species arrays hold 0
+    iterations and NaN residuals; ``iteration`` is the outer closure iteration
+    and ``converged`` defaults to every species' converged flag (False
+    without info)."""
     c = np.ascontiguousarray(_host(c_orbit), dtype=np.float64)
     if c.ndim != 3:
         raise ValueError("c_orbit must be (S, n_bins+1, N)")
     S = c.shape[0]
-    save = {"c_orbit": c, "c_mean": orbit_mean(c),
-            "orbit_species_order": np.asarray(range(S) if species_order is None else species_order, dtype=np.int64)}
+    order = np.asarray(range(S) if species_order is None else species_order, dtype=np.int64)
+    if order.shape != (S,):
+        raise ValueError(f"species_order needs {S} entries, got {order.shape}")
+    if info is not None and len(info) != S:
+        raise ValueError(f"info needs one dict per species ({S}), got {len(info)}")
+    save = {"c_orbit": c, "c_mean": orbit_mean(c), "orbit_species_order": order}
     if info is not None:
         save["gmres_iters"] = np.array([i["gmres_iters"] for i in info], dtype=np.int64)
         save["gmres_resid"] = np.array([i["residual"] for i in info], dtype=np.float64)
         save["periodicity"] = np.array([i["periodicity"] for i in info], dtype=np.float64)
+    else:
+        save["gmres_iters"] = np.zeros(S, dtype=np.int64)
+        save["gmres_resid"] = np.full(S, np.nan)
+        save["periodicity"] = np.full(S, np.nan)
+    if converged is None:
+        converged = info is not None and all(bool(i.get("converged", True)) for i in info)
+    save["iter"] = np.array(int(iteration))
+    save["converged"] = np.array(bool(converged))
     if month is not None:
         save["month"] = np.array(month)
     for k, v in extra.items():

@@ -56,6 +56,11 @@ class CouplingBlock:
 
     def axpy(self, x, y, alpha: float, trans: bool = False):
         """y += alpha K x (or K^T x) on the context's stream; x, y CUDA float64 (n,)."""
+        from ._lib import dev_f64
+        for name, t in (("x", x), ("y", y)):
+            dev_f64(t, self.ctx.device, name, last=self.n)
+            if t.numel() != self.n or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous vector of n = {self.n} values")
         check(lib().orbit_spmat_apply_dev(self.ctx.handle, self.handle, int(bool(trans)), float(alpha),
                                           x.data_ptr(), y.data_ptr()))
 
