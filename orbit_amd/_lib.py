@@ -37,6 +37,7 @@ EXPORTS = [
     "orbit_opset_get_shapes", "orbit_opset_set_shapes", "orbit_transport_assemble",
     "orbit_opset_create_grids", "orbit_lut_create", "orbit_lut_query_dev", "orbit_lut_destroy",
     "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation", "orbit_forward_fct_dev",
+    "orbit_solve_forward_dev_noalloc",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
 N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
@@ -124,6 +125,7 @@ def signatures():
         "orbit_solve_forward": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_solve_adjoint": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_solve_forward_dev": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
+        "orbit_solve_forward_dev_noalloc": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_solve_adjoint_dev": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_mass_balance": (_I, [_P, _P, _P, C.POINTER(MbStats)]),
         "orbit_fct_source": (_I, [_P, C.POINTER(GridMetrics), _P, _D, _I, _P, _P]),
@@ -183,6 +185,12 @@ def lib():
             _declare(h)
             _lib = h
     return _lib
+
+
+def last_error() -> str:
+    """Message of the last error on the calling thread."""
+    msg = lib().orbit_last_error()
+    return msg.decode() if msg else ""
 
 
 def check(rc, allow_warn=True):

@@ -58,6 +58,32 @@ int h2d_sync(hipStream_t st, void* dst, const void* src, size_t bytes) {
     return ORBIT_OK;
 }
 
+// Fixed device scratch of orbit_solve_forward_dev_noalloc: one array per GPU,
+// held from the library's load until the process ends.  In host code the
+// array's name is only a stand-in for the symbol; its device address on each
+// GPU comes from fixed_scratch_address and is the only form passed to kernels
+// or copies.
+static __device__ __align__(256) unsigned char scratch[ORBIT_FIXED_SCRATCH_BYTES];
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// once per device and kept: each GPU holds its own copy at its own address.
+int fixed_scratch_address(int device, void** out) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ORBIT_ERR_HIP, std::string("fixed scratch array: hipGetSymbolAddress on device ") +
+                                           std::to_string(device) + ": " + hipGetErrorString(e));
+        it = addr.emplace(device, p).first;
+    }
+    *out = it->second;
+    return ORBIT_OK;
+}
+
 // Per-slot flags and counts (up to 64 slots) travel as kernel arguments: the
 // write is ordered on the stream like any launch and the host array can be
 // dropped at once, so the host does not wait for the device.
@@ -814,6 +840,9 @@ extern "C" int orbit_ctx_create(int device, orbit_ctx** out) {
         delete c;
         return fail(ORBIT_ERR_HIP, "orbit_ctx_create: hipStreamCreate failed");
     }
+    // set-up of the allocation-free solve: this device's scratch address (a
+    // failure here is reported by that entry point, which looks again)
+    if (fixed_scratch_address(device, &c->fixed_scratch) != ORBIT_OK) c->fixed_scratch = nullptr;
     *out = c;
     return ORBIT_OK;
 }
@@ -1860,9 +1889,32 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     return ORBIT_OK;
 }
 
+// Device scratch of one periodic solve in N_SOLVE_PARTS parts: 0 the forcing,
+// 1 the orbit, 2 the Krylov basis, 3 six FP64 work vectors, 4 three FP32 sweep
+// buffers, 5 reduction partials, Hessenberg coefficients and per-slot ints,
+// 6 the GMRES states.  Sizes per slot scale with n16 (the padded vector).
+constexpr int N_SOLVE_PARTS = 7;
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+void solve_scratch_parts(size_t nt, int nbn, size_t n16, int m, size_t* b) {
+    const size_t nb = (n16 + RED_CHUNK - 1) / RED_CHUNK;
+    b[0] = nt * nbn * n16 * 8;
+    b[1] = nt * (nbn + 1) * n16 * 8;
+    b[2] = nt * (m + 1) * n16 * 8;
+    b[3] = nt * 6 * n16 * 8;
+    b[4] = nt * 3 * n16 * 4;
+    b[5] = nt * MAXM1 * nb * 8 * 2 + nt * MAXM1 * 8 * 2 + nt * 8 * 2 + nt * 4 * 6 + 1024;
+    b[6] = nt * sizeof(GState);
+}
+size_t solve_scratch_bytes(size_t nt, int nbn, size_t n16, int m) {
+    size_t b[N_SOLVE_PARTS], t = 0;
+    solve_scratch_parts(nt, nbn, n16, m, b);
+    for (int q = 0; q < N_SOLVE_PARTS; ++q) t += align256(b[q]);
+    return t;
+}
+
 int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, const double* in_dev,
                    const double* warm_dev, const orbit_opts* opts_in, double* out_dev, orbit_info* info,
-                   int dir, PassDelta* pd = nullptr) {
+                   int dir, PassDelta* pd = nullptr, void* arena = nullptr, size_t arena_bytes = 0) {
     if (!ctx || !op || !tracers || nt < 1 || !in_dev || !out_dev || !info)
         return fail(ORBIT_ERR_INVALID, "solve: NULL argument or empty tracer list");
     orbit_opts o;
@@ -1901,31 +1953,44 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     const int nbn = op->nbins;
     S.nb = (int)((n16 + RED_CHUNK - 1) / RED_CHUNK);
 
-    // scratch
-    int rc = ctx->ensure(0, (size_t)nt * nbn * n16 * 8);
-    if (!rc) rc = ctx->ensure(1, (size_t)nt * (nbn + 1) * n16 * 8);
-    if (!rc) rc = ctx->ensure(2, (size_t)nt * (S.m + 1) * n16 * 8);
-    if (!rc) rc = ctx->ensure(3, (size_t)nt * 6 * n16 * 8);
-    if (!rc) rc = ctx->ensure(4, (size_t)nt * 3 * n16 * 4);
-    const size_t small = (size_t)nt * MAXM1 * S.nb * 8 * 2 + (size_t)nt * MAXM1 * 8 * 2 + (size_t)nt * 8 * 2 +
-                         (size_t)nt * 4 * 6 + 1024;
-    if (!rc) rc = ctx->ensure(5, small);
-    if (!rc) rc = ctx->ensure(6, (size_t)nt * sizeof(GState));
-    if (rc) return rc;
-    S.E = ctx->buf<double>(0);
-    S.orbit = ctx->buf<double>(1);
-    S.Vk = ctx->buf<double>(2);
-    double* v5 = ctx->buf<double>(3);
+    // scratch: the context's growable buffers, or carved from a caller's arena
+    size_t part_bytes[N_SOLVE_PARTS];
+    solve_scratch_parts(nt, nbn, n16, S.m, part_bytes);
+    void* part_ptr[N_SOLVE_PARTS];
+    int rc = ORBIT_OK;
+    if (arena) {
+        const size_t need = solve_scratch_bytes(nt, nbn, n16, S.m);
+        if (need > arena_bytes)
+            return fail(ORBIT_ERR_INVALID, "solve: " + std::to_string(nt) + " tracers of this operator set with restart " +
+                                               std::to_string(S.m) + " need " + std::to_string(need) +
+                                               " bytes of scratch; the fixed scratch array holds " +
+                                               std::to_string(arena_bytes));
+        char* a = (char*)arena;
+        for (int q = 0; q < N_SOLVE_PARTS; ++q) {
+            part_ptr[q] = a;
+            a += align256(part_bytes[q]);
+        }
+    } else {
+        for (int q = 0; q < N_SOLVE_PARTS && !rc; ++q) {
+            rc = ctx->ensure(q, part_bytes[q]);
+            part_ptr[q] = ctx->scratch[q].p;
+        }
+        if (rc) return rc;
+    }
+    S.E = (double*)part_ptr[0];
+    S.orbit = (double*)part_ptr[1];
+    S.Vk = (double*)part_ptr[2];
+    double* v5 = (double*)part_ptr[3];
     S.c = v5;
     S.r = v5 + (size_t)nt * n16;
     S.w = v5 + (size_t)2 * nt * n16;
     S.ta = v5 + (size_t)3 * nt * n16;
     S.tb = v5 + (size_t)4 * nt * n16;
     if (o.rhs_is_s) S.sv = v5 + (size_t)5 * nt * n16;
-    S.fa = ctx->buf<float>(4);
+    S.fa = (float*)part_ptr[4];
     S.fb = S.fa + (size_t)nt * n16;
     S.fc = S.fa + (size_t)2 * nt * n16;
-    char* sm = (char*)ctx->scratch[5].p;
+    char* sm = (char*)part_ptr[5];
     S.partials = (double*)sm; sm += (size_t)nt * MAXM1 * S.nb * 8 * 2;
     S.h1 = (double*)sm; sm += (size_t)nt * MAXM1 * 8;
     S.h2 = (double*)sm; sm += (size_t)nt * MAXM1 * 8;
@@ -1936,7 +2001,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.B.active_dev = (int*)sm; sm += (size_t)nt * 4;
     S.B.part_dev = (int*)sm; sm += (size_t)nt * 4;
     S.nsw_dev = (int*)sm;
-    S.gst = ctx->buf<GState>(6);
+    S.gst = (GState*)part_ptr[6];
     S.B.ctx = ctx;
     S.B.op = op;
     S.B.nslots = nt;
@@ -2251,6 +2316,26 @@ extern "C" int orbit_solve_forward_dev(orbit_ctx* ctx, orbit_opset* op, const in
                                        const double* e_dev, const double* c_warm_dev, const orbit_opts* opts,
                                        double* orbit_out_dev, orbit_info* info) {
     return solve_periodic(ctx, op, tracers, nt, e_dev, c_warm_dev, opts, orbit_out_dev, info, FWD);
+}
+
+// The forward solve with no device allocation: scratch from this device's
+// fixed array, no graph (instantiating one allocates).  Calls that overlap on
+// one GPU share the array and race.
+extern "C" int orbit_solve_forward_dev_noalloc(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt,
+                                               const double* e_dev, const double* c_warm_dev,
+                                               const orbit_opts* opts, double* orbit_out_dev, orbit_info* info) {
+    if (!ctx || !op) return fail(ORBIT_ERR_INVALID, "solve (no allocation): NULL context or operator set");
+    hipSetDevice(ctx->device);
+    if (!ctx->fixed_scratch) {
+        const int rc = fixed_scratch_address(ctx->device, &ctx->fixed_scratch);
+        if (rc) return rc;
+    }
+    orbit_opts o;
+    orbit_opts_default(&o);
+    if (opts) o = *opts;
+    o.use_graph = 0;
+    return solve_periodic(ctx, op, tracers, nt, e_dev, c_warm_dev, &o, orbit_out_dev, info, FWD, nullptr,
+                          ctx->fixed_scratch, (size_t)ORBIT_FIXED_SCRATCH_BYTES);
 }
 
 int orbit::solve_forward_pass(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, const double* e_dev,

@@ -58,6 +58,9 @@ struct Ctx {
     // reusable scratch, grown on demand: 0-6 the periodic solve, 7 calibration
     // and benchmarks, 8 the forward FCT driver's emissions, c0 and s~
     DevBuf scratch[9];
+    // this device's copy of the fixed scratch array of the allocation-free
+    // solve (its device address, looked up at context creation)
+    void* fixed_scratch = nullptr;
     int ensure(int slot, size_t bytes);
     template <class T>
     T* buf(int slot) { return reinterpret_cast<T*>(scratch[slot].p); }
