@@ -9,12 +9,18 @@ inside FP64 GMRES, FP64 refinement to 1e-8 relative residual), inputs resident
 in HBM.  value = orbit solves/s = 12 / step time.
 
 Beside it: the plain FP64 GPU solve of the same work (same algorithm in FP64,
-one orbit at a time, no batching, no graphs), the per-tracer c0 difference,
-p50/p99 per-step latency, the same without FCT, the adjoint at the same size
-(50-cell receptors), 144 uneven batched orbits (12 months x 12 tracers over
-24x20 / 48x40 / 64x70), a Poisson serving sweep over those months, the
-roofline of the dominant kernel from live HIP-event timing and a CPU baseline
-(the oracle: SuperLU factors + scipy GMRES) on a bounded sample.
+one orbit at a time, no batching, no graphs) and the same 12 orbits batched in
+FP64 (so the speed-up splits into batching and precision), every tracer's c0
+against its plain FP64 solve and tracers 0 and 7 against the CPU oracle's FCT
+month (tests/golden/fullsize_oracle.npz), p50/p99 per-step latency, the same
+without FCT, the adjoint at the same size (50-cell receptors), 144 uneven
+batched orbits (12 months x 12 tracers over 24x20 / 48x40 / 64x70), a Poisson
+serving sweep over those months admitted by the device's free memory, the
+roofline of the dominant half-sweep variant from the launches' own start/stop
+events over one headline step, and the CPU baseline: the oracle (SuperLU
+factors + scipy GMRES + numpy FCT) as min(cores, 12) single-threaded processes,
+each one complete FCT-month orbit, plus one process alone (single core) and the
+complete 24x20x15 orbit of config 0.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -42,37 +48,79 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, iters_per_pass):
-    """Oracle on the host, one core: SuperLU factors of bins 0-1 of tracer 0, 16
-    back-solves and the FCT source of one bin, extrapolated to one FCT-on orbit
-    (8 factors; per pass 8*(iters+4) solves and 8 FCT sources)."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    from oracle import fct_ref, orbit_ref as orb
+CPU_THREAD_VARS = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+BOX_CPU_SHARE = 16  # host cores a one-GPU box gives a job
+
+
+def _cpu_worker(spec):
+    """One single-threaded oracle orbit (the CPU baseline's unit of work),
+    printed as JSON: "fct-month:<t>" = tracer t of the 64x70x15 FCT month (8
+    SuperLU factors + 6 deferred-correction passes of scipy GMRES(30) to tol
+    1e-8); "single" = the 24x20x15 one-tracer orbit without FCT (tol 1e-8)."""
+    from oracle import drivers_ref, orbit_ref as orb
+    from orbit_amd import synth
+    kind, _, t = spec.partition(":")
+    wl = synth.benchmark_workload(kind)
+    t = int(t or 0)
     t0 = time.perf_counter()
-    lus = [orb.factor_one_bin(wl.L(b, 0)) for b in range(2)]
-    t_fac = (time.perf_counter() - t0) / 2
-    rng = np.random.default_rng(0)
-    vs = [rng.random(wl.N) for _ in range(16)]
-    t0 = time.perf_counter()
-    for k, v in enumerate(vs):
-        lus[k % 2].solve(v)
-    t_sol = (time.perf_counter() - t0) / 16
-    c = rng.random(wl.N)
-    t0 = time.perf_counter()
-    fct_ref.compute_horizontal_fct_source(wl.grids[0], c, wl.dtau, True, None)
-    t_fct = time.perf_counter() - t0
-    n_solves = sum(8 * (it + 4) for it in iters_per_pass)
-    n_pass = len(iters_per_pass)
-    t_orbit = 8 * t_fac + n_solves * t_sol + 8 * n_pass * t_fct
+    lus = [orb.factor_one_bin(wl.L(b, t)) for b in range(8)]
+    t1 = time.perf_counter()
+    if kind == "fct-month":
+        orbit, hist = drivers_ref.forward_fct_passes(lus, list(wl.e[t]), wl.grids, n_passes=N_PASSES, tol=1e-8)
+        iters = [h["gmres_iters"] for h in hist]
+        ok = all(h["converged"] for h in hist)
+    else:
+        orbit, info = orb.solve_orbit_one_species(lus, list(wl.e[t]), tol=1e-8)
+        iters, ok = [info["gmres_iters"]], info["converged"]
+    t2 = time.perf_counter()
+    print(json.dumps({"spec": spec, "factor_s": t1 - t0, "solve_s": t2 - t1, "iters": iters, "converged": bool(ok)}))
+
+
+def _run_workers(specs):
+    import subprocess
+    env = dict(os.environ)
+    for v in CPU_THREAD_VARS:
+        env[v] = "1"
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", sp], env=env,
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for sp in specs]
+    out = []
+    for p in procs:
+        o, e = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"CPU baseline worker failed ({p.returncode}): {e[-2000:]}")
+        out.append(json.loads(o.strip().splitlines()[-1]))
+    return out
+
+
+def cpu_baseline(n_tracers):
+    """The oracle on the host: single-threaded processes, one orbit each.
+    First one process alone (the single-core figure: the 24x20x15 orbit and
+    one complete FCT-month orbit), then min(cores, tracers) processes at once,
+    each the complete FCT-month orbit of its own tracer; throughput = orbits /
+    the slowest process's time, with and without the SuperLU factorisations
+    (the reference factors per solve)."""
+    cores = min(len(os.sched_getaffinity(0)), BOX_CPU_SHARE)
+    P = max(1, min(cores, n_tracers))
+    single = _run_workers(["single"])[0]
+    one = _run_workers(["fct-month:0"])[0]
+    many = _run_workers([f"fct-month:{t}" for t in range(P)])
+    slow_total = max(w["factor_s"] + w["solve_s"] for w in many)
+    slow_solve = max(w["solve_s"] for w in many)
     return {
-        "value": 1.0 / t_orbit, "unit": "orbit solves/s", "cores": 1, "kind": "port",
-        "solve_only_value": 1.0 / (n_solves * t_sol),
-        "sample": (f"oracle (SuperLU + numpy FCT) on tracer 0 at 64x70x15: factor of 2 of 8 bins "
-                   f"({t_fac:.2f} s each), 16 back-solves ({t_sol * 1e3:.1f} ms each), FCT source of "
-                   f"one bin ({t_fct * 1e3:.1f} ms); extrapolated to one orbit with {n_pass} FCT passes: "
-                   f"8 factors + {n_solves} solves (GMRES iterations per pass of the FP64 GPU solve "
-                   f"{iters_per_pass}, +4 for s, restart residual and recovery) + {8 * n_pass} FCT "
-                   f"sources; 1 core"),
+        "value": round(P / slow_total, 5), "unit": "orbit solves/s", "cores": P, "kind": "port",
+        "solve_only_value": round(P / slow_solve, 5),
+        "single_core_value": round(1.0 / (one["factor_s"] + one["solve_s"]), 5),
+        "single_core_solve_only_value": round(1.0 / one["solve_s"], 5),
+        "host_cores_visible": len(os.sched_getaffinity(0)),
+        "config0_single_core": {"orbit_s": round(single["factor_s"] + single["solve_s"], 3),
+                                "factor_s": round(single["factor_s"], 3), "gmres_iters": single["iters"],
+                                "converged": single["converged"]},
+        "workers": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in w.items()} for w in many],
+        "sample": (f"oracle (SuperLU per bin + scipy GMRES(30), numpy FCT) on the GPU box's host: {P} "
+                   f"single-threaded processes at once, each one complete 64x70x15 FCT-month orbit ({N_PASSES} "
+                   "passes, tol 1e-8) of its own tracer, 8 factorisations included (solve_only_value excludes "
+                   "them); single_core_* = tracer 0's orbit in one process alone; config0_single_core = the "
+                   "complete 24x20x15 one-tracer orbit"),
     }
 
 
@@ -101,7 +149,11 @@ def main():
                     help="back-to-back half-sweep launches timed for the roofline")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (scales profiler byte counters)")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        _cpu_worker(args.cpu_worker)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: there is no multi-GPU path here; the orbit "
               "solves of different months and tracers are independent and would shard "
@@ -147,6 +199,23 @@ def main():
     iters_m = [[i["gmres_iters"] for i in h] for h in hist]
     log(f"mixed + FCT: {ms_per_step:.2f} ms/step, {value:.1f} orbit solves/s; iters per pass {iters_m[0]}..{iters_m[-1]}")
 
+    # one more step of the same work with every half-sweep timed by its own
+    # start/stop events (hipExtLaunchKernelGGL on the library's stream; graphs
+    # off, so every launch is seen): the kernel durations behind `roofline`
+    _lib.check(_lib.lib().orbit_timing_start(ctx.handle, 20000))
+    t1 = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    t_instr = time.perf_counter() - t1
+    tab = (_lib.SweepTiming * 32)()
+    nvar, untimed = C.c_int(), C.c_longlong()
+    _lib.check(_lib.lib().orbit_timing_stop(ctx.handle, tab, 32, C.byref(nvar), C.byref(untimed)))
+    variants = [tab[i] for i in range(min(nvar.value, 32))]
+    sweep_ms = sum(v.total_ms for v in variants)
+    sweep_bytes = sum(v.total_bytes for v in variants)
+    log(f"instrumented step: {t_instr * 1e3:.1f} ms, {sum(v.launches for v in variants)} half-sweeps, "
+        f"{sweep_ms:.2f} ms of sweep kernels, {sweep_bytes / 1e9:.1f} GB algorithmic, {untimed.value} untimed")
+
     # plain FP64 with the same FCT passes: one orbit at a time, no graphs
     npl = min(args.plain_tracers, T)
     drivers.forward_fct(op, grids, e_dev[:1], tracers=[0], n_passes=1, tol=1e-8, precision="fp64")  # warm
@@ -164,6 +233,29 @@ def main():
             for s in range(npl)]
     log(f"plain fp64 + FCT: {t_plain * 1e3:.1f} ms/orbit, {plain_value:.2f} orbit solves/s; "
         f"mixed-vs-fp64 c0 rel err {errs}")
+    # the same 12 orbits batched in FP64 (no graphs apply to FP64 sweeps): how
+    # much of the speed-up is batching and how much precision
+    drivers.forward_fct(op, grids, e_dev, n_passes=1, tol=1e-8, precision="fp64")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    orbit_b64, _ = drivers.forward_fct(op, grids, e_dev, n_passes=N_PASSES, tol=1e-8, precision="fp64")
+    torch.cuda.synchronize()
+    batched_fp64_value = T / (time.perf_counter() - t1)
+    log(f"batched fp64 + FCT: {batched_fp64_value:.1f} orbit solves/s")
+    # c0 against the CPU oracle's FCT month (SuperLU + GMRES to 1e-12,
+    # tests/golden/fullsize_oracle.npz) for the tracers it holds
+    oracle_err = None
+    gold_path = os.path.join(ROOT, "tests", "golden", "fullsize_oracle.npz")
+    if os.path.exists(gold_path):
+        gold = np.load(gold_path, allow_pickle=False)
+        oracle_err = {}
+        for t in [int(x) for x in gold["tracers"] if int(x) < T]:
+            ref = gold[f"forward_c0_{t}"]
+            rel = lambda a: float(np.abs(a.cpu().numpy() - ref).max() / np.abs(ref).max())  # noqa: E731
+            oracle_err[str(t)] = {"mixed": float(f"{rel(orbit_mixed[t, 0]):.3e}"),
+                                  "plain_fp64": float(f"{rel(plain[t][0][0, 0]):.3e}") if t < npl else None,
+                                  "batched_fp64": float(f"{rel(orbit_b64[t, 0]):.3e}")}
+        log(f"c0 vs the CPU oracle: {oracle_err}")
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "orbit solves/s", "n_gpus": 1,
@@ -180,6 +272,10 @@ def main():
         "p99_ms": round(float(np.percentile(np.array(lat) * 1e3, 99)), 3),
         "plain_fp64_value": round(plain_value, 3),
         "speedup_vs_plain_fp64": round(value / plain_value, 2),
+        "batched_fp64_value": round(batched_fp64_value, 3),
+        "speedup_from_batching": round(batched_fp64_value / plain_value, 2),
+        "speedup_from_precision": round(value / batched_fp64_value, 2),
+        "rel_err_c0_vs_cpu_oracle": oracle_err,
         "max_rel_err_c0_vs_fp64": max(errs),
         "rel_err_c0_vs_fp64_per_tracer": [float(f"{x:.3e}") for x in errs],
         "gmres_iters_mixed_per_pass": iters_m[-1] if iters_m else None,
@@ -282,7 +378,7 @@ def main():
         # serving: Poisson sweep over resident month operator sets
         ops = {(m, sizes[m % 3]): jobs[m][0] for m in range(12)}
         sweeps = []
-        srv = serve.OrbitServer(ops, n_workers=4, max_slots=16, hbm_budget=64e9)
+        srv = serve.OrbitServer(ops, n_workers=4, max_slots=16)  # budget: free HBM (hipMemGetInfo)
         try:
             for rate, nq in ((5.0, 25), (50.0, 60), (200.0, 120)):
                 rq = np.random.default_rng(int(rate))
@@ -303,32 +399,66 @@ def main():
         finally:
             srv.close()
         out["serving"] = sweeps
+        out["serving_admission"] = {"budget_gb": round(srv.budget / 1e9, 2), "source": srv.budget_source,
+                                    "free_at_start_gb": round(srv.free_at_start / 1e9, 2)}
 
-    # roofline of the two sweep kernels of the mixed solve (all slots, bin 0), live
-    # HIP events on the context's stream: the FP32/BF16 inner sweep and the
-    # FP64/BF16 accurate sweep; the dominant one (more device time per step in the
-    # kernel-trace profile of this run) is reported as `roofline`.
-    roofs = {}
-    for name, prec, kern in (("inner", _lib.ORBIT_PREC_MIXED, "k_halfsweep<float,float,bf16,float>"),
-                             ("accurate", _lib.ORBIT_SWEEP_ACC_MIXED, "k_halfsweep<double,double,bf16,double>")):
+    # roofline: the half-sweep variant with the most device time in the
+    # instrumented headline step; achieved = its algorithmic bytes / its
+    # launches' own durations (start/stop events around each launch)
+    def vname(v):
+        base = {_lib.ORBIT_PREC_FP64: "fp64", _lib.ORBIT_SWEEP_ACC_MIXED: "accurate",
+                _lib.ORBIT_PREC_MIXED: "inner"}[v.precision]
+        return base + ("_ownzero" if v.own_zero else "") + ("_nbrzero" if v.nbr_zero else "")
+
+    def kname(v):
+        xt = {_lib.ORBIT_PREC_FP64: "double,double,double,double", _lib.ORBIT_SWEEP_ACC_MIXED:
+              "double,double,unsigned short,double", _lib.ORBIT_PREC_MIXED: "float,float,unsigned short,float"}
+        return (f"k_halfsweep<{xt[v.precision]},{1 if v.slots_per_wave == 4 else 0},"
+                f"{'true' if v.own_zero else 'false'},{'true' if v.nbr_zero else 'false'}>")
+
+    def roof(v):
+        us = 1e3 * v.total_ms / v.launches
+        bpl = v.total_bytes / v.launches
+        achieved = bpl / (us * 1e-6) / 1e9
+        tr = traffic_from_profile(vname(v))
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr,
+                "kernel": f"{kname(v)} ({T} slots, 64x70x15, one colour)", "variant": vname(v),
+                "us_per_launch": round(us, 3), "algorithmic_bytes_per_launch": round(bpl),
+                "launches_per_step": int(v.launches), "ms_per_step": round(v.total_ms, 3),
+                "traffic_over_algorithmic": round(tr / bpl, 3) if tr else None,
+                "timing": "kernel start/stop events (hipExtLaunchKernelGGL) on the library's stream over one "
+                          "headline step (the timed work; graphs off so every launch is timed)",
+                "residency": "a bin's working set (40-60 MB) fits the 256 MB Infinity Cache: bytes leaving L2, "
+                             "not necessarily HBM"}
+
+    out["roofline"] = roof(variants[0])
+    out["roofline_other"] = [roof(v) for v in variants[1:4]]
+    out["step_sweeps"] = {"half_sweeps": int(sum(v.launches for v in variants)),
+                          "kernel_ms": round(sweep_ms, 3), "algorithmic_gb": round(sweep_bytes / 1e9, 3),
+                          "achieved_gbs_over_step": round(sweep_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                          "frac_over_step": round(sweep_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    # the isolated figure: `roof_reps` back-to-back launches on one bin (HIP
+    # events around the whole run, gaps between dispatches included)
+    iso = {}
+    for name, prec in (("inner", _lib.ORBIT_PREC_MIXED), ("accurate", _lib.ORBIT_SWEEP_ACC_MIXED)):
         ms = C.c_double()
         by = C.c_double()
         _lib.check(_lib.lib().orbit_bench_halfsweep(ctx.handle, op.handle, 0, prec, T, args.roof_reps,
                                                     C.byref(ms), C.byref(by)))
-        achieved = by.value / (ms.value * 1e-3) / 1e9
-        log(f"halfsweep {name}: {ms.value * 1e3:.2f} us/launch, {by.value / 1e6:.2f} MB, {achieved:.0f} GB/s")
-        roofs[name] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(name),
-                       "kernel": f"{kern} ({T} slots, 64x70x15, one colour)",
-                       "us_per_launch": round(ms.value * 1e3, 3), "algorithmic_bytes_per_launch": by.value}
-    out["roofline"] = roofs[DOMINANT]
-    out["roofline_other"] = roofs["inner" if DOMINANT == "accurate" else "accurate"]
+        iso[name] = {"us_per_launch": round(ms.value * 1e3, 3), "algorithmic_bytes": by.value,
+                     "achieved_gbs": round(by.value / (ms.value * 1e-3) / 1e9, 1)}
+    out["isolated_back_to_back"] = iso
+    log(f"roofline {out['roofline']['variant']}: {out['roofline']['us_per_launch']} us/launch, "
+        f"{out['roofline']['achieved']} GB/s ({out['roofline']['frac']}); isolated {iso}")
 
     cpu = None
-    if not args.no_cpu:
-        ipp = [int(h[0]["gmres_iters"]) for h in plain[0][1]] if plain else [6] * N_PASSES
-        cpu = cpu_baseline(wl, ipp)
-        log(f"cpu baseline: {cpu['value']:.4f} orbit solves/s (solve only {cpu['solve_only_value']:.3f})")
+    if not args.no_cpu and not args.quick:
+        del orbit_mixed, plain, orbit_b64
+        cpu = cpu_baseline(T)
+        log(f"cpu baseline: {cpu['value']:.4f} orbit solves/s on {cpu['cores']} cores (solve only "
+            f"{cpu['solve_only_value']:.4f}); one core {cpu['single_core_value']:.4f}; config 0 "
+            f"{cpu['config0_single_core']['orbit_s']} s per orbit")
     out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 

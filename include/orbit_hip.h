@@ -421,6 +421,27 @@ int orbit_opset_check(orbit_ctx* ctx, orbit_opset* op, long long* counts);
 int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, int precision, int nslots,
                           int reps, double* ms_per_launch, double* bytes_per_launch);
 
+/* Kernel timing of the half-sweeps (diagnostic, no counterpart in the
+ * reference): between orbit_timing_start and orbit_timing_stop every
+ * half-sweep the context launches records its own start and stop events
+ * (hipExtLaunchKernelGGL; events made by orbit_timing_start for up to
+ * max_launches launches, later ones run untimed), and solves launch their
+ * inexact sweeps directly instead of replaying captured graphs.  Stop returns
+ * per kernel variant (sorted by total time) the launch count, the sum of the
+ * launches' own durations and of their algorithmic bytes (as
+ * orbit_bench_halfsweep counts them, less what the variant does not read). */
+typedef struct {
+    int precision;       /* ORBIT_PREC_FP64, ORBIT_SWEEP_ACC_MIXED or ORBIT_PREC_MIXED (FP32 inner) */
+    int slots_per_wave;  /* 1 or 4 */
+    int own_zero;        /* 1: x of the colour known zero (not read) */
+    int nbr_zero;        /* 1: x of the other colour known zero (not read) */
+    long long launches;
+    double total_ms;
+    double total_bytes;
+} orbit_sweep_timing;
+int orbit_timing_start(orbit_ctx* ctx, int max_launches);
+int orbit_timing_stop(orbit_ctx* ctx, orbit_sweep_timing* out, int max_out, int* n_out, long long* untimed);
+
 /* Launch shape of the red-black half-sweep (the column SpMV + block apply of
  * every inner solve): slots_per_wave 1 (a wave is 4 columns of one tracer slot)
  * or 4 (one column x 4 slots sharing the column's block loads), threads per

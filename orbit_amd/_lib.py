@@ -37,7 +37,7 @@ EXPORTS = [
     "orbit_opset_get_shapes", "orbit_opset_set_shapes", "orbit_transport_assemble",
     "orbit_opset_create_grids", "orbit_lut_create", "orbit_lut_query_dev", "orbit_lut_destroy",
     "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation", "orbit_forward_fct_dev",
-    "orbit_solve_forward_dev_noalloc",
+    "orbit_solve_forward_dev_noalloc", "orbit_timing_start", "orbit_timing_stop",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
 N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
@@ -67,6 +67,12 @@ class Info(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SweepTiming(C.Structure):
+    _fields_ = [("precision", C.c_int), ("slots_per_wave", C.c_int), ("own_zero", C.c_int),
+                ("nbr_zero", C.c_int), ("launches", C.c_longlong), ("total_ms", C.c_double),
+                ("total_bytes", C.c_double)]
 
 
 class OpsetStats(C.Structure):
@@ -126,6 +132,8 @@ def signatures():
         "orbit_solve_adjoint": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_solve_forward_dev": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_solve_forward_dev_noalloc": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
+        "orbit_timing_start": (_I, [_P, _I]),
+        "orbit_timing_stop": (_I, [_P, _P, _I, _P, _P]),
         "orbit_solve_adjoint_dev": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_mass_balance": (_I, [_P, _P, _P, C.POINTER(MbStats)]),
         "orbit_fct_source": (_I, [_P, C.POINTER(GridMetrics), _P, _D, _I, _P, _P]),

@@ -24,6 +24,8 @@
 #include <memory>
 #include <numeric>
 
+#include <hip/hip_ext.h>
+
 #include "orbit_internal.h"
 
 namespace orbit {
@@ -1332,8 +1334,21 @@ struct Batch {
     int* part_dev;     // slots taking part in the current round
 };
 
+// Algorithmic bytes of one half-sweep launch: per column of the colour the
+// shared block V (unless x of this colour is known zero) and stencil H (unless
+// the other colour's x is), per slot the diagonal d (unless own x is zero),
+// the inverse block P, the rhs b, own x read (unless zero) and written, and
+// the other colour's x once (unless zero).
+double halfsweep_bytes(int ncols, int nslots, size_t ae, size_t pe, size_t xe, bool own_zero, bool nbr_zero) {
+    const double shared = (own_zero ? 0.0 : 256.0 * ae) + (nbr_zero ? 0.0 : 64.0 * ae);
+    const double per_slot = (own_zero ? 0.0 : 16.0 * ae) + 256.0 * pe + 16.0 * xe + (own_zero ? 0.0 : 16.0 * xe) +
+                            16.0 * xe + (nbr_zero ? 0.0 : 16.0 * xe);
+    return (double)ncols * (shared + nslots * per_slot);
+}
+
 template <class XT, class AT, class PT, class CT>
-int launch_sweep_kernel(SweepArgs a, const SweepShape& sh, bool own_zero, bool nbr_zero, hipStream_t st) {
+int launch_sweep_kernel(orbit_ctx* ctx, int prec, SweepArgs a, const SweepShape& sh, bool own_zero, bool nbr_zero,
+                        hipStream_t st) {
     const int cs = sh.cs, W = sh.threads;
     const int tc = cs ? W / 64 : W / 16;  // columns per tile
     const int ntiles = (a.ncols + tc - 1) / tc;
@@ -1342,18 +1357,40 @@ int launch_sweep_kernel(SweepArgs a, const SweepShape& sh, bool own_zero, bool n
     a.tiles_per_xcd = (ntiles + 7) / 8;
     const dim3 grid(sh.xcd ? 8 * a.tiles_per_xcd * nsg : ntiles * nsg), block(W);
     const size_t lds = (cs ? (size_t)(W / 64) * 256 * sizeof(AT) : 0) + 2 * (size_t)W * sizeof(CT);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (SweepTiming* tm = ctx->timing) {
+        if (tm->next + 2 <= tm->ev.size()) {
+            e0 = tm->ev[tm->next];
+            e1 = tm->ev[tm->next + 1];
+            tm->next += 2;
+            tm->recs.push_back({prec * 8 + cs * 4 + (own_zero ? 2 : 0) + (nbr_zero ? 1 : 0),
+                                halfsweep_bytes(a.ncols, a.nslots, sizeof(AT), sizeof(PT), sizeof(XT), own_zero,
+                                                nbr_zero)});
+        } else {
+            ++tm->dropped;
+        }
+    }
+#define ORBIT_SWEEP_LAUNCH(CSV, OZV, NZV)                                                                    \
+    do {                                                                                                     \
+        if (e0)                                                                                              \
+            hipExtLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, CSV, OZV, NZV>), grid, block, lds, st, e0, e1, \
+                                  0, a);                                                                     \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, CSV, OZV, NZV>), grid, block, lds, st, a);        \
+    } while (0)
     // (own_zero, nbr_zero) = (true, false) is the second colour of a cold first sweep
     if (cs) {
-        if (own_zero && nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, true, true>), grid, block, lds, st, a);
-        else if (own_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, true, false>), grid, block, lds, st, a);
-        else if (nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, false, true>), grid, block, lds, st, a);
-        else hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, false, false>), grid, block, lds, st, a);
+        if (own_zero && nbr_zero) ORBIT_SWEEP_LAUNCH(1, true, true);
+        else if (own_zero) ORBIT_SWEEP_LAUNCH(1, true, false);
+        else if (nbr_zero) ORBIT_SWEEP_LAUNCH(1, false, true);
+        else ORBIT_SWEEP_LAUNCH(1, false, false);
     } else {
-        if (own_zero && nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, true, true>), grid, block, lds, st, a);
-        else if (own_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, true, false>), grid, block, lds, st, a);
-        else if (nbr_zero) hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, false, true>), grid, block, lds, st, a);
-        else hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 0, false, false>), grid, block, lds, st, a);
+        if (own_zero && nbr_zero) ORBIT_SWEEP_LAUNCH(0, true, true);
+        else if (own_zero) ORBIT_SWEEP_LAUNCH(0, true, false);
+        else if (nbr_zero) ORBIT_SWEEP_LAUNCH(0, false, true);
+        else ORBIT_SWEEP_LAUNCH(0, false, false);
     }
+#undef ORBIT_SWEEP_LAUNCH
     return ORBIT_OK;
 }
 
@@ -1393,15 +1430,15 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     if (prec == SP_F64) {
         a.V = T ? O.VT64 : O.V64; a.Vt = T ? O.V64 : O.VT64;
         a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT64 : O.P64;
-        return launch_sweep_kernel<double, double, double, double>(a, sh, own_zero, nbr_zero, B.ctx->stream);
+        return launch_sweep_kernel<double, double, double, double>(B.ctx, prec, a, sh, own_zero, nbr_zero, B.ctx->stream);
     } else if (prec == SP_ACC_MIXED) {
         a.V = T ? O.VT64 : O.V64; a.Vt = T ? O.V64 : O.VT64;
         a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT16 : O.P16;
-        return launch_sweep_kernel<double, double, uint16_t, double>(a, sh, own_zero, nbr_zero, B.ctx->stream);
+        return launch_sweep_kernel<double, double, uint16_t, double>(B.ctx, prec, a, sh, own_zero, nbr_zero, B.ctx->stream);
     }
     a.V = T ? O.VT32 : O.V32; a.Vt = T ? O.V32 : O.VT32;
     a.H = T ? O.HT32 : O.H32; a.d = O.d32; a.P = T ? O.PT16 : O.P16;
-    return launch_sweep_kernel<float, float, uint16_t, float>(a, sh, own_zero, nbr_zero, B.ctx->stream);
+    return launch_sweep_kernel<float, float, uint16_t, float>(B.ctx, prec, a, sh, own_zero, nbr_zero, B.ctx->stream);
 }
 
 // x = P_bin(rhs) approximately: `sweeps` red-black sweeps starting from x = 0.
@@ -1946,7 +1983,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.acc = o.acc_sweeps > 0 ? o.acc_sweeps
                              : std::min(80, std::max(4, (int)std::ceil(std::log(S.eps_acc) / std::log(S.rho))));
     S.m = o.restart;
-    S.use_graph = o.use_graph != 0 && S.prec == ORBIT_PREC_MIXED;
+    S.use_graph = o.use_graph != 0 && S.prec == ORBIT_PREC_MIXED && !ctx->timing;
     S.n16 = op->n16;
     S.nbins = op->nbins;
     const size_t n16 = S.n16;
@@ -2481,6 +2518,72 @@ extern "C" int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, i
     return ORBIT_OK;
 }
 
+
+extern "C" int orbit_timing_start(orbit_ctx* ctx, int max_launches) {
+    if (!ctx) return fail(ORBIT_ERR_INVALID, "timing_start: NULL context");
+    if (ctx->timing) return fail(ORBIT_ERR_INVALID, "timing_start: kernel timing is already on for this context");
+    if (max_launches < 1 || max_launches > 1000000)
+        return fail(ORBIT_ERR_INVALID, "timing_start: max_launches must be in [1, 1000000]");
+    hipSetDevice(ctx->device);
+    auto* tm = new SweepTiming();
+    tm->ev.resize(2 * (size_t)max_launches, nullptr);
+    for (auto& e : tm->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            for (auto& f : tm->ev)
+                if (f) hipEventDestroy(f);
+            delete tm;
+            return fail(ORBIT_ERR_HIP, "timing_start: hipEventCreate failed");
+        }
+    }
+    tm->recs.reserve(max_launches);
+    ctx->timing = tm;
+    return ORBIT_OK;
+}
+
+extern "C" int orbit_timing_stop(orbit_ctx* ctx, orbit_sweep_timing* out, int max_out, int* n_out,
+                                 long long* untimed) {
+    if (!ctx || !ctx->timing) return fail(ORBIT_ERR_INVALID, "timing_stop: kernel timing is not on");
+    hipSetDevice(ctx->device);
+    SweepTiming* tm = ctx->timing;
+    ctx->timing = nullptr;
+    int rc = ORBIT_OK;
+    std::map<int, orbit_sweep_timing> agg;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ORBIT_ERR_HIP, "timing_stop: stream sync failed");
+    for (size_t i = 0; i < tm->recs.size() && rc == ORBIT_OK; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, tm->ev[2 * i], tm->ev[2 * i + 1]) != hipSuccess) {
+            rc = fail(ORBIT_ERR_HIP, "timing_stop: hipEventElapsedTime failed");
+            break;
+        }
+        const int v = tm->recs[i].variant;
+        auto it = agg.find(v);
+        if (it == agg.end()) {
+            const int p = v / 8;  // internal sweep precision -> ABI constant
+            orbit_sweep_timing t{};
+            t.precision = p == SP_F64 ? ORBIT_PREC_FP64 : p == SP_ACC_MIXED ? ORBIT_SWEEP_ACC_MIXED : ORBIT_PREC_MIXED;
+            t.slots_per_wave = (v & 4) ? 4 : 1;
+            t.own_zero = (v & 2) ? 1 : 0;
+            t.nbr_zero = (v & 1) ? 1 : 0;
+            it = agg.emplace(v, t).first;
+        }
+        it->second.launches += 1;
+        it->second.total_ms += ms;
+        it->second.total_bytes += tm->recs[i].bytes;
+    }
+    const long long dropped = tm->dropped;
+    for (auto& e : tm->ev) hipEventDestroy(e);
+    delete tm;
+    if (rc) return rc;
+    std::vector<orbit_sweep_timing> v;
+    for (auto& kv : agg) v.push_back(kv.second);
+    std::stable_sort(v.begin(), v.end(),
+                     [](const orbit_sweep_timing& a, const orbit_sweep_timing& b) { return a.total_ms > b.total_ms; });
+    const int n = std::min<int>((int)v.size(), std::max(0, max_out));
+    for (int i = 0; i < n && out; ++i) out[i] = v[i];
+    if (n_out) *n_out = (int)v.size();
+    if (untimed) *untimed = dropped;
+    return ORBIT_OK;
+}
 
 namespace {
 

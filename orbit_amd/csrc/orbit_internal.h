@@ -51,6 +51,21 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Per-launch kernel timing of the half-sweeps (a diagnostic): while `on`,
+// every half-sweep launch on the context's stream records its own start and
+// stop events (hipExtLaunchKernelGGL) from a pool made beforehand, with its
+// variant and algorithmic bytes; solves do not replay captured graphs then.
+struct SweepTiming {
+    struct Rec {
+        int variant;   // prec * 8 + cs * 4 + own_zero * 2 + nbr_zero
+        double bytes;  // algorithmic bytes of the launch
+    };
+    std::vector<hipEvent_t> ev;  // 2 per launch
+    size_t next = 0;
+    long long dropped = 0;  // launches past the pool (not timed)
+    std::vector<Rec> recs;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -61,6 +76,7 @@ struct Ctx {
     // this device's copy of the fixed scratch array of the allocation-free
     // solve (its device address, looked up at context creation)
     void* fixed_scratch = nullptr;
+    SweepTiming* timing = nullptr;  // non-null while kernel timing is on
     int ensure(int slot, size_t bytes);
     template <class T>
     T* buf(int slot) { return reinterpret_cast<T*>(scratch[slot].p); }

@@ -11,7 +11,9 @@ queries.  Every month's operator set stays resident on the device
   adjoint periodic solve giving dJ/de for every bin).
 * Admission: a query is admitted when the device memory its batch needs
   (scratch of the solve + its inputs/outputs) fits in the free part of the HBM
-  budget; otherwise it waits.
+  budget; otherwise it waits.  The budget is the device's free memory
+  (hipMemGetInfo) once the month operator sets are resident, less a reserve,
+  unless the caller gives one.
 * Continuous batching: each of n_workers host threads owns a HIP stream
   (Context).  A free worker takes the oldest waiting query and every other
   waiting query with the same (month, size, mode), up to max_slots tracers and
@@ -96,15 +98,33 @@ class ShapeTuner:
         return out
 
 
+def device_free_bytes(device: int = 0) -> int:
+    """Free device memory now (hipMemGetInfo), after whatever is resident."""
+    import torch
+    return int(torch.cuda.mem_get_info(device)[0])
+
+
 class OrbitServer:
+    """hbm_budget: device bytes the in-flight batches may use.  None (the
+    default) takes it from the device itself: the free memory reported by
+    hipMemGetInfo once the month operator sets are resident, less a reserve
+    (``reserve_frac`` of it, for torch's allocator and the libraries)."""
+
     def __init__(self, opsets: Dict[Tuple[int, str], OperatorSet], n_workers: int = 4,
-                 max_slots: int = 16, hbm_budget: float = 32e9, precision: str = "mixed",
-                 device: int = 0, tuner: Optional[ShapeTuner] = None):
+                 max_slots: int = 16, hbm_budget: Optional[float] = None, precision: str = "mixed",
+                 device: int = 0, tuner: Optional[ShapeTuner] = None, reserve_frac: float = 0.1):
         if tuner is not None:  # before any worker launches on these sets
             for op in opsets.values():
                 tuner.apply(op)
         self.opsets = opsets
         self.max_slots = max_slots
+        if hbm_budget is None:
+            self.free_at_start = device_free_bytes(device)
+            hbm_budget = (1.0 - reserve_frac) * self.free_at_start
+            self.budget_source = "hipMemGetInfo"
+        else:
+            self.free_at_start = None
+            self.budget_source = "caller"
         self.budget = float(hbm_budget)
         self.in_use = 0.0
         self.precision = precision
@@ -212,26 +232,32 @@ class OrbitServer:
                     self.cv.notify_all()
 
 
-def make_query(op: OperatorSet, month: int, size: str, tracers, mode: str, rng, base_e=None,
-               tol: float = 1e-6):
-    """Synthetic query: forward = 1% lognormal perturbation of the month's
-    emissions of the chosen tracers; adjoint = a receptor on 10-100 random
-    surface cells with U(0,1) weights at c_0."""
-    import torch
+def query_payload(n_bins: int, N: int, ncol: int, tracers, mode: str, rng, base_e=None) -> np.ndarray:
+    """Synthetic query input (nt, n_bins, N), host float64: "forward" = a 1 %
+    lognormal perturbation of the month's emissions of the chosen tracers
+    (the marginal forcing de); "adjoint" = a receptor on 10-100 random surface
+    cells with U(0,1) weights at c_0."""
     nt = len(tracers)
     if mode == "forward":
-        de = 0.01 * base_e[tracers] * rng.lognormal(0.0, 1.0, base_e[tracers].shape)
-        payload = de
-    else:
-        R = np.zeros((nt, op.n_bins, op.N))
-        ncol = op.nx * op.ny
-        for i in range(nt):
-            m = int(rng.integers(10, 101))
-            cells = rng.choice(ncol, m, replace=False)
-            R[i, 0, cells] = rng.random(m)
-        payload = R
+        e = np.asarray(base_e)[list(tracers)]
+        return 0.01 * e * rng.lognormal(0.0, 1.0, e.shape)
+    if mode != "adjoint":
+        raise ValueError("mode must be 'forward' or 'adjoint'")
+    R = np.zeros((nt, n_bins, N))
+    for i in range(nt):
+        m = int(rng.integers(10, 101))
+        cells = rng.choice(ncol, m, replace=False)
+        R[i, 0, cells] = rng.random(m)
+    return R
+
+
+def make_query(op: OperatorSet, month: int, size: str, tracers, mode: str, rng, base_e=None,
+               tol: float = 1e-6):
+    """A Query whose payload (``query_payload``) is on the operator set's device."""
+    import torch
+    payload = query_payload(op.n_bins, op.N, op.nx * op.ny, tracers, mode, rng, base_e)
     return Query(month=month, size=size, tracers=list(tracers), mode=mode,
-                 payload=torch.from_numpy(np.ascontiguousarray(payload)).to("cuda:0"), tol=tol)
+                 payload=torch.from_numpy(np.ascontiguousarray(payload)).to(f"cuda:{op.ctx.device}"), tol=tol)
 
 
 def run_poisson(server: OrbitServer, rate: float, n_queries: int, make, rng) -> dict:
@@ -265,7 +291,7 @@ def solve_many(jobs, n_workers: int = 4, precision: str = "mixed", tol: float = 
     from `tuner` when given).  Returns (results, wall seconds)."""
     opsets = {(i, "job"): j[0] for i, j in enumerate(jobs)}
     srv = OrbitServer(opsets, n_workers=n_workers, max_slots=max(len(j[2]) for j in jobs),
-                      hbm_budget=1e12, precision=precision, tuner=tuner)
+                      precision=precision, tuner=tuner)
     try:
         t0 = time.perf_counter()
         futs = [srv.submit(Query(month=i, size="job", tracers=list(j[2]), mode="forward", payload=j[1], tol=tol))
