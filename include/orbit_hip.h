@@ -119,7 +119,8 @@ void* orbit_ctx_stream(orbit_ctx* ctx);
  * One CSR of L_{bin,t} per (bin, tracer), index [bin*n_tracers + t].  A is
  * built as I + dtau*L.  The pattern must couple each cell only to cells of its
  * own column (any levels) and to its 4 horizontal neighbours on its own level;
- * anything else is ORBIT_ERR_STRUCTURE.  nz <= 16. */
+ * anything else is ORBIT_ERR_STRUCTURE; couplings across a periodic x wrap face
+ * (i = 0 with i = nx - 1, any nx) are accepted.  nz <= 16. */
 int orbit_opset_create_csr(orbit_ctx* ctx, int nz, int ny, int nx, int n_bins, int n_tracers,
                            double dtau, const int32_t* const* rowptr,
                            const int32_t* const* colidx, const double* const* vals,
@@ -255,8 +256,8 @@ int orbit_lateral_inflow(orbit_ctx* ctx, const orbit_grid_fields* g, double* rat
  * k-major) is added to it: L_{bin,t} = T_bin + diag(D_{bin,t})
  * (reference assemble_single_species_L, orbit/core/operator.py:356-368), with
  * no host CSR.  The result is the same opset orbit_opset_create_shared builds
- * from the CSR of T.  A periodic longitude needs an even nx (the red-black
- * colouring of the sweeps; ORBIT_ERR_STRUCTURE otherwise). */
+ * from the CSR of T.  Any nx is accepted with a periodic longitude (with an
+ * odd one the sweeps give the last column of each row two colours of its own). */
 int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracers, double dtau,
                              const orbit_grid_fields* grids, const double* D, orbit_opset** out);
 

@@ -555,11 +555,7 @@ extern "C" int orbit_opset_create_grids(orbit_ctx* ctx, int n_bins, int n_tracer
         if ((grids[b].periodic_lon != 0) != (grids[0].periodic_lon != 0))
             return fail(ORBIT_ERR_INVALID, "create_grids: every bin needs the same periodic_lon");
     }
-    // the sweeps' red-black colouring pairs columns 0 and nx-1 of a row only with an even nx
     const int wrap = grids[0].periodic_lon && grids[0].nx > 1;
-    if (wrap && grids[0].nx % 2)
-        return fail(ORBIT_ERR_STRUCTURE, "create_grids: a periodic longitude needs an even nx (red-black "
-                                         "colouring); nx = " + std::to_string(grids[0].nx));
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     const int nz = grids[0].nz, ny = grids[0].ny, nx = grids[0].nx, ncol = nx * ny, N = nz * ncol;

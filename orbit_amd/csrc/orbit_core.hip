@@ -215,6 +215,7 @@ __global__ void k_transpose_stencil(const T* __restrict__ H, T* __restrict__ HT,
 struct SweepArgs {
     int ncols;
     int nx, ny;
+    int nxc;            // width of the red-black part of a row (nx, or nx - 1 with four colours)
     const void* V;
     long long vstride;  // per tracer (0 when the block is shared)
     const void* Vt;     // transpose of V (read by the CS mapping, which needs vstride 0)
@@ -238,7 +239,7 @@ struct SweepArgs {
     const int* counts;  // nullable: per-slot sweep count; slots past theirs skip
     int sweep_idx;
     int colour;         // 0 red ((i + j) even), 1 black
-    int px;             // periodic x (wrap couplings; nx even keeps the colouring)
+    int px;             // periodic x (wrap couplings)
     float omega;        // relaxation of the correction (1: Gauss-Seidel)
 };
 
@@ -259,17 +260,25 @@ __device__ __forceinline__ void linear_tile_slot(int L, int nslots, int& tile, i
     slot = L - tile * nslots;
 }
 
-// Column id of the idx-th column of colour c in row-major order.  Rows 2p and
-// 2p+1 together hold exactly nx columns of each colour; row 2p's start at i = c.
-__device__ __forceinline__ void colour_col(int idx, int c, int nx, int& col, int& i, int& j) {
-    const int p = idx / nx, r = idx - p * nx;
-    const int n0 = (nx + 1 - c) >> 1;
-    if (r < n0) {
-        j = 2 * p;
-        i = c + 2 * r;
+// Column id of the idx-th column of colour c in row-major order.  Colours 0 / 1
+// (red / black) cover the first w columns of every row: rows 2p and 2p+1
+// together hold exactly w columns of each, row 2p's start at i = c.  Colours
+// 2 / 3 are the last column of the even / odd rows (a periodic x with an odd
+// nx, where w = nx - 1).
+__device__ __forceinline__ void colour_col(int idx, int c, int w, int nx, int& col, int& i, int& j) {
+    if (c >= 2) {
+        j = 2 * idx + (c - 2);
+        i = nx - 1;
     } else {
-        j = 2 * p + 1;
-        i = (1 - c) + 2 * (r - n0);
+        const int p = idx / w, r = idx - p * w;
+        const int n0 = (w + 1 - c) >> 1;
+        if (r < n0) {
+            j = 2 * p;
+            i = c + 2 * r;
+        } else {
+            j = 2 * p + 1;
+            i = (1 - c) + 2 * (r - n0);
+        }
     }
     col = j * nx + i;
 }
@@ -320,7 +329,7 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     const bool valid = idx < a.ncols && slot < a.nslots;
     const int ls = slot < a.nslots ? slot : a.nslots - 1;
     int col, i, j;
-    colour_col(idx < a.ncols ? idx : 0, a.colour, a.nx, col, i, j);
+    colour_col(idx < a.ncols ? idx : 0, a.colour, a.nxc, a.nx, col, i, j);
     const size_t base = (size_t)col * 16 + lane;
     const XT* __restrict__ b = (const XT*)a.b + (size_t)ls * a.bstride;
     XT* __restrict__ x = (XT*)a.x + (size_t)ls * a.xstride;
@@ -949,10 +958,16 @@ void default_shapes(orbit_opset* op);
 
 // Colour counts, default launch shapes, then the sweep calibration.
 int finalize_opset(orbit_ctx* ctx, orbit_opset* op) {
-    // columns of each colour ((i + j) even / odd); the sweeps map a colour index
-    // to its column arithmetically
-    op->ncols_c[0] = (op->ncol + 1) / 2;
-    op->ncols_c[1] = op->ncol / 2;
+    // columns of each colour; the sweeps map a colour index to its column
+    // arithmetically.  A periodic x with an odd nx (>= 3) keeps the last column
+    // of every row out of the red-black pattern (colours 2 and 3).
+    const bool four = op->px && op->nx >= 3 && (op->nx % 2);
+    op->ncolours = four ? 4 : 2;
+    op->nxc = four ? op->nx - 1 : op->nx;
+    for (int c = 0; c < 4; ++c) op->ncols_c[c] = 0;
+    for (int j = 0; j < op->ny; ++j)
+        for (int i = 0; i < op->nx; ++i)
+            op->ncols_c[i < op->nxc ? (i + j) % 2 : 2 + j % 2] += 1;
     default_shapes(op);
     ORBIT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return calibrate_opset(ctx, op, 0.0);
@@ -1044,16 +1059,6 @@ int finish_opset(orbit_ctx* ctx, orbit_opset* op, std::vector<HostBin>& hb) {
     return finalize_opset(ctx, op);
 }
 
-// A periodic x direction keeps the red-black colouring only with an even nx:
-// with an odd one, columns 0 and nx - 1 of a row have one colour and would be
-// updated together while coupled.
-int check_wrap(int wrap, int nx) {
-    if (wrap && nx % 2)
-        return fail(ORBIT_ERR_STRUCTURE, "couplings across a periodic x wrap face need an even nx (red-black "
-                                         "colouring); nx = " + std::to_string(nx));
-    return ORBIT_OK;
-}
-
 int check_dims(int nz, int ny, int nx, int nbins, int ntr) {
     if (nz < 1 || nz > NZP) return fail(ORBIT_ERR_INVALID, "nz must be in [1, 16]");
     if (ny < 1 || nx < 1) return fail(ORBIT_ERR_INVALID, "ny and nx must be positive");
@@ -1113,8 +1118,6 @@ extern "C" int orbit_opset_create_csr(orbit_ctx* ctx, int nz, int ny, int nx, in
             }
         }
     }
-    rc = check_wrap(wrap, nx);
-    if (rc) return rc;
     auto* op = new orbit_opset();
     op->ctx = ctx;
     op->nz = nz; op->ny = ny; op->nx = nx; op->nbins = n_bins; op->ntr = n_tracers;
@@ -1159,8 +1162,6 @@ extern "C" int orbit_opset_create_shared(orbit_ctx* ctx, int nz, int ny, int nx,
             }
         }
     }
-    rc = check_wrap(wrap, nx);
-    if (rc) return rc;
     auto* op = new orbit_opset();
     op->ctx = ctx;
     op->nz = nz; op->ny = ny; op->nx = nx; op->nbins = n_bins; op->ntr = n_tracers;
@@ -1270,6 +1271,18 @@ __global__ void k_zero(double* __restrict__ x, long long xslot, int n) {
     const int slot = blockIdx.y;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) x[(size_t)slot * xslot + g] = 0.0;
+}
+
+// x[col*16 + k] = 0 for the last column of every row (i = nx - 1) of every
+// slot the mask keeps (all when null); grid (ceil(ny*16/256), nslots).
+template <class T>
+__global__ void k_zero_last_col(T* __restrict__ x, long long xslot, int nx, int ny, const int* active) {
+    const int slot = blockIdx.y;
+    if (active && !active[slot]) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ny * 16) return;
+    const int j = g >> 4, k = g & 15;
+    x[(size_t)slot * xslot + ((size_t)j * nx + nx - 1) * 16 + k] = T(0);
 }
 
 // out = (in ? in : 0) + (float)e   per slot (FP32 forced sweeps)
@@ -1404,6 +1417,7 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     if (a.ncols == 0) return ORBIT_OK;
     a.nx = op->nx;
     a.ny = op->ny;
+    a.nxc = op->nxc;
     a.px = op->px;
     a.omega = (float)op->omega;
     a.vstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 256;
@@ -1442,14 +1456,37 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
 }
 
 // x = P_bin(rhs) approximately: `sweeps` red-black sweeps starting from x = 0.
+// One full sweep: every colour in turn.  first: the sweep starts from x = 0
+// (x is not read: own-zero for every colour, neighbour-zero for the first).
+// With four colours the last column of each row (colours 2 and 3) is set to
+// zero first, so that colours 1 and 2, whose neighbours include columns not
+// yet written in this sweep, read zeros there; slots the mask leaves out are
+// not touched.
+int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long long bstride, void* x,
+               long long xstride, bool first, const int* mask, const int* counts, int sweep_idx) {
+    orbit_opset* op = B.op;
+    if (first && op->ncolours == 4) {
+        const dim3 g((unsigned)((op->ny * 16 + 255) / 256), B.nslots);
+        if (prec == SP_F32)
+            hipLaunchKernelGGL(k_zero_last_col<float>, g, dim3(256), 0, B.ctx->stream, (float*)x, xstride, op->nx,
+                               op->ny, mask);
+        else
+            hipLaunchKernelGGL(k_zero_last_col<double>, g, dim3(256), 0, B.ctx->stream, (double*)x, xstride, op->nx,
+                               op->ny, mask);
+    }
+    for (int c = 0; c < op->ncolours; ++c) {
+        const int rc = launch_halfsweep(B, bin, dir, prec, c, rhs, bstride, x, xstride, first, first && c == 0, mask,
+                                        counts, sweep_idx);
+        if (rc) return rc;
+    }
+    return ORBIT_OK;
+}
+
 int inner_solve(const Batch& B, int bin, int dir, int prec, int sweeps, const void* rhs,
                 long long bstride, void* x, long long xstride, const int* mask, bool warm = false,
                 const int* counts = nullptr) {
     for (int s = 0; s < sweeps; ++s) {
-        const bool z = s == 0 && !warm;
-        int rc = launch_halfsweep(B, bin, dir, prec, 0, rhs, bstride, x, xstride, z, z, mask, counts, s);
-        if (rc) return rc;
-        rc = launch_halfsweep(B, bin, dir, prec, 1, rhs, bstride, x, xstride, z, false, mask, counts, s);
+        const int rc = sweep_once(B, bin, dir, prec, rhs, bstride, x, xstride, s == 0 && !warm, mask, counts, s);
         if (rc) return rc;
     }
     return ORBIT_OK;
@@ -1476,8 +1513,7 @@ int measure_contraction(orbit_ctx* ctx, orbit_opset* op, int ns, int* tr, const 
     for (int bin = 0; bin < nbins; ++bin) {
         const BinOps& O = op->bins[bin];
         for (int k = 1; k <= 8; ++k) {
-            int rc = launch_halfsweep(B, bin, FWD, SP_F64, 0, b, 0, x, xs, k == 1, k == 1, nullptr);
-            if (!rc) rc = launch_halfsweep(B, bin, FWD, SP_F64, 1, b, 0, x, xs, k == 1, false, nullptr);
+            const int rc = sweep_once(B, bin, FWD, SP_F64, b, 0, x, xs, k == 1, nullptr, nullptr, 0);
             if (rc) return rc;
             if (k == 3 || k == 8) {
                 hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, ns), dim3(256), 0, st, O.V64, vstride, O.H64,
@@ -2440,8 +2476,7 @@ extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int trace
     while (total < 200) {
         const int chunk = 4;
         for (int s = 0; s < chunk; ++s) {
-            rc = launch_halfsweep(B, bin, dir, SP_F64, 0, rhs, (long long)n16, xs, (long long)n16, first, first, nullptr);
-            if (!rc) rc = launch_halfsweep(B, bin, dir, SP_F64, 1, rhs, (long long)n16, xs, (long long)n16, first, false, nullptr);
+            rc = sweep_once(B, bin, dir, SP_F64, rhs, (long long)n16, xs, (long long)n16, first, nullptr, nullptr, 0);
             if (rc) return rc;
             first = false;
         }

@@ -101,7 +101,7 @@ struct BinOps {
 
 // West / east neighbour column of column `col` at x index i: the adjacent
 // column, the column across the wrap face when x is periodic (px: couplings
-// between i = 0 and i = nx - 1, nx even), or -1 at an open edge.  With nx = 2
+// between i = 0 and i = nx - 1), or -1 at an open edge.  With nx = 2
 // both neighbours of a column are the other column.
 __host__ __device__ inline int west_col(int col, int i, int nx, int px) {
     return i > 0 ? col - 1 : (px ? col + nx - 1 : -1);
@@ -140,14 +140,21 @@ struct Opset {
     Ctx* ctx = nullptr;
     int nz = 0, ny = 0, nx = 0, nbins = 0, ntr = 0, ngroups = 1;
     int ncol = 0, n16 = 0, N = 0;
-    int px = 0;  // periodic x: the operators couple i = 0 and i = nx - 1 (nx even)
+    int px = 0;  // periodic x: the operators couple i = 0 and i = nx - 1
     double dtau = 0.0;
     bool has_tdiag = false;
     int def_inner = 6, def_acc = 14;
     double rho = 0.1;  // calibrated contraction of one red-black sweep (with margin)
     double omega = 1.0;  // relaxation of the sweeps' correction (chosen by the calibration)
     std::vector<BinOps> bins;
-    int ncols_c[2] = {0, 0};
+    // Column colours of the sweeps: red / black ((i + j) even / odd) over the
+    // first nxc columns of every row; with a periodic x and an odd nx the wrap
+    // face would join two columns of one colour, so the last column of each
+    // row (i = nx - 1) is left out of the red-black pattern (nxc = nx - 1) and
+    // takes colours 2 (j even) and 3 (j odd), neither of which touches itself.
+    int ncolours = 2;
+    int nxc = 0;
+    int ncols_c[4] = {0, 0, 0, 0};
     std::vector<void*> owned;
     double bytes = 0.0;
     SweepShape shapes[N_SWEEP_PRECS][N_SLOT_BUCKETS];  // set at creation, by autotune or by the caller

@@ -141,14 +141,30 @@ def test_periodic_month_from_grids():
     np.testing.assert_allclose(mb["top_p90"], ref["top_p90"], rtol=1e-9)
 
 
-def test_periodic_odd_nx_stencil_but_no_opset():
-    grids, D, _ = synth.grid_workload(11, 6, 4, 1, seed=32, periodic=True)
+def test_periodic_odd_nx_month_from_grids():
+    """A periodic longitude with an odd nx from grid fields: the device stencil
+    equals the reference's golden-checked assembly, the grid-built opset equals
+    the CSR-built one bit for bit, and its orbits match the oracle."""
+    grids, D, e = synth.grid_workload(11, 6, 4, 2, seed=32, periodic=True)
     A = assembly.assemble_transport_block(grids[2])
     B = asm.assemble_transport_block(grids[2])
     assert abs(A - B).max() <= 1e-12 * abs(B).max()
-    with pytest.raises(_lib.OrbitError, match="even nx") as ei:
-        OperatorSet.from_grids(grids, D)
-    assert ei.value.code == _lib.ORBIT_ERR_STRUCTURE
+    op_g = OperatorSet.from_grids(grids, D)
+    To = [asm.assemble_transport_block(g) for g in grids]
+    op_c = OperatorSet.from_shared([assembly.assemble_transport_block(g) for g in grids], D, 4, 6, 11)
+    assert op_g.stats()["periodic_x"] == 1
+    for prec in ("fp64", "mixed"):
+        og, ig = op_g.solve_forward(e, tol=1e-10, precision=prec)
+        oc, _ = op_c.solve_forward(e, tol=1e-10, precision=prec)
+        assert np.array_equal(og, oc), prec
+        assert all(i["converged"] for i in ig)
+    og, _ = op_g.solve_forward(e, tol=1e-11, precision="fp64")
+    lg, _ = op_g.solve_adjoint(e, tol=1e-11, precision="fp64")
+    for s in range(e.shape[0]):
+        lus = [orb.factor_one_bin((To[t] + sp.diags(D[t, s])).tocsr()) for t in range(8)]
+        ref, _ = orb.solve_orbit_one_species(lus, list(e[s]), tol=1e-12, maxiter=500)
+        lref, _ = orb.solve_orbit_adjoint_one_species(lus, list(e[s]), tol=1e-12, maxiter=500)
+        assert _rel(og[s, 0], ref[0]) < 1e-8 and _rel(lg[s, 0], lref[0]) < 1e-8, s
 
 
 def test_fct_forward_month_on_grid_built_operators_matches_the_oracle():

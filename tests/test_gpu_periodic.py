@@ -5,9 +5,9 @@ The CSR import puts the wrap couplings in the W / E stencil slots of the edge
 columns and the sweeps, operator applies, transposes and the mass-balance row
 sums follow them.  Checked against the CPU oracle (splu on the same CSR) at
 the bars of test_gpu_orbit.py: the LU shim both ways, forward and adjoint
-orbits in FP64 and mixed precision, the air-mass statistics; an odd nx is
-refused (the red-black colouring would pair two coupled columns of one
-colour); the smallest ring (nx = 4) matches too."""
+orbits in FP64 and mixed precision, the air-mass statistics; odd nx (the
+last column of each row sweeps in two colours of its own) and the smallest
+rings (nx = 3, 4) match too."""
 import numpy as np
 import pytest
 
@@ -15,7 +15,6 @@ pytestmark = pytest.mark.gpu
 
 from oracle import mass_ref, orbit_ref as orb
 from orbit_amd import synth
-from orbit_amd._lib import ORBIT_ERR_STRUCTURE, OrbitError
 from orbit_amd.core import OperatorSet
 
 NB = 8
@@ -74,11 +73,34 @@ def test_periodic_mass_balance_matches_oracle():
     assert got["interior_max"] < 1e-9  # the wrap face conserves air mass like an interior face
 
 
-def test_periodic_odd_nx_is_refused():
-    wl = synth.make_workload(9, 6, 4, 1, seed=2, periodic=True)
-    with pytest.raises(OrbitError, match="even nx") as ei:
-        OperatorSet.from_workload(wl)
-    assert ei.value.code == ORBIT_ERR_STRUCTURE
+@pytest.mark.parametrize("nx", [9, 3, 13])
+def test_periodic_odd_nx_matches_oracle(nx):
+    """An odd nx on a periodic longitude (the reference accepts any nx,
+    grid_data.py:427): the last column of each row leaves the red-black pattern
+    and takes two colours of its own, so no two coupled columns share a colour.
+    LU shim both ways 1e-12, forward and adjoint orbits against the oracle in
+    FP64 (1e-8) and mixed precision (1e-6), nx = 3 the smallest such ring."""
+    wl = synth.make_workload(nx, 6, 4, 2, seed=2 + nx, convection=True, periodic=True)
+    op = OperatorSet.from_workload(wl)
+    assert op.stats()["periodic_x"] == 1
+    lus = [[orb.factor_one_bin(wl.L(t, s)) for t in range(NB)] for s in range(wl.n_tracers)]
+    rng = np.random.default_rng(nx)
+    b = rng.random(wl.N)
+    for tau in (0, 6):
+        for trans in ("N", "T"):
+            assert _rel(op.apply_P(tau, 1, b, trans=trans == "T"), lus[1][tau].solve(b, trans=trans)) < 1e-12
+    R = rng.random((wl.n_tracers, NB, wl.N))
+    for prec, tol, bar in (("fp64", 1e-11, 1e-8), ("mixed", 1e-8, 1e-6)):
+        orbit, info = op.solve_forward(wl.e, tol=tol, precision=prec)
+        lam, ainfo = op.solve_adjoint(R, tol=tol, precision=prec)
+        for s in range(wl.n_tracers):
+            ref, _ = orb.solve_orbit_one_species(lus[s], list(wl.e[s]), tol=1e-12, maxiter=500)
+            aref, _ = orb.solve_orbit_adjoint_one_species(lus[s], list(R[s]), tol=1e-12, maxiter=500)
+            assert info[s]["converged"] and ainfo[s]["converged"], (prec, s)
+            for k in (0, 3, 8):
+                assert _rel(orbit[s, k], ref[k]) < bar, (nx, prec, s, k)
+            for k in (0, 4, 7):
+                assert _rel(lam[s, k], aref[k]) < bar, (nx, prec, s, k)
 
 
 def test_four_column_ring_matches_oracle():
