@@ -387,6 +387,16 @@ int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const
                           int nt, const double* e_dev, const double* ghost_dev, const int* ghost_tracers,
                           int n_passes, const orbit_opts* opts, double* orbit_dev, orbit_info* info);
 
+/* The same passes, also writing the orbit every pass before the last ends
+ * with into pass_orbits_dev (n_passes - 1, nt, n_bins + 1, N), device: the
+ * baseline states of each pass's FCT source, which the backward pass of the
+ * FCT orbit through the passes freezes its operators at.  Extends
+ * orbit_forward_fct_dev (the reference keeps only the last orbit). */
+int orbit_forward_fct_passes_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids,
+                                 const int* tracers, int nt, const double* e_dev, const double* ghost_dev,
+                                 const int* ghost_tracers, int n_passes, const orbit_opts* opts,
+                                 double* orbit_dev, orbit_info* info, double* pass_orbits_dev);
+
 /* Frozen L_AD from a device-resident grid and a device baseline state. */
 int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,
                                orbit_fct** out);

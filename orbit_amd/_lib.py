@@ -38,6 +38,7 @@ EXPORTS = [
     "orbit_opset_create_grids", "orbit_lut_create", "orbit_lut_query_dev", "orbit_lut_destroy",
     "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation", "orbit_forward_fct_dev",
     "orbit_solve_forward_dev_noalloc", "orbit_timing_start", "orbit_timing_stop",
+    "orbit_forward_fct_passes_dev",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
 N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
@@ -165,6 +166,7 @@ def signatures():
         "orbit_fct_source_batch_dev": (_I, [_P, _P, _I, _P, C.c_longlong, _P, _D, _I, _P, C.c_longlong,
                                             _P, C.c_longlong]),
         "orbit_forward_fct_dev": (_I, [_P, _P, _P, _P, _I, _P, _P, _P, _I, C.POINTER(Opts), _P, _P]),
+        "orbit_forward_fct_passes_dev": (_I, [_P, _P, _P, _P, _I, _P, _P, _P, _I, C.POINTER(Opts), _P, _P, _P]),
     }
 
 

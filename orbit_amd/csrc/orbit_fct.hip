@@ -1145,10 +1145,12 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
 // from the previous orbit plus the FP32 propagation of the change of the
 // forcing (solve_forward_pass), so its first accurate check needs only the
 // sweeps for the FP32 error of that change instead of a cold check.
-extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids,
-                                     const int* tracers, int nt, const double* e_dev, const double* ghost_dev,
-                                     const int* ghost_tracers, int n_passes, const orbit_opts* opts,
-                                     double* orbit_dev, orbit_info* info) {
+// The forward passes; pass_orbits (nullable) receives the orbits of passes
+// 0 .. n_passes-2 in turn, (n_passes - 1, nt, n_bins + 1, N).
+static int forward_fct_passes(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids, const int* tracers,
+                              int nt, const double* e_dev, const double* ghost_dev, const int* ghost_tracers,
+                              int n_passes, const orbit_opts* opts, double* orbit_dev, orbit_info* info,
+                              double* pass_orbits) {
     if (!ctx || !op || !grids || !tracers || nt < 1 || !e_dev || !orbit_dev || !info)
         return fail(ORBIT_ERR_INVALID, "forward_fct: NULL argument or empty tracer list");
     if (n_passes < 1) return fail(ORBIT_ERR_INVALID, "forward_fct: n_passes must be >= 1");
@@ -1204,6 +1206,9 @@ extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_
     const long long os = (long long)(nbn + 1) * N, es = (long long)nbn * N;
     const double* e_prev = e_dev;  // the forcing the current orbit was solved with
     for (int p = 1; p < n_passes; ++p) {
+        if (pass_orbits)
+            ORBIT_HIP_CHECK(hipMemcpyAsync(pass_orbits + (size_t)(p - 1) * nt * os, orbit_dev, (size_t)nt * os * 8,
+                                           hipMemcpyDeviceToDevice, st));
         double* e_eff = inc ? ebuf[(p - 1) & 1] : ebuf[0];
         if (one_launch) {
             // every bin at once: the same per-(bin, tracer, level) tiles as a launch per bin
@@ -1239,6 +1244,23 @@ extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_
         worst = std::max(worst, rc);
     }
     return worst;
+}
+
+extern "C" int orbit_forward_fct_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids,
+                                     const int* tracers, int nt, const double* e_dev, const double* ghost_dev,
+                                     const int* ghost_tracers, int n_passes, const orbit_opts* opts,
+                                     double* orbit_dev, orbit_info* info) {
+    return forward_fct_passes(ctx, op, grids, tracers, nt, e_dev, ghost_dev, ghost_tracers, n_passes, opts,
+                              orbit_dev, info, nullptr);
+}
+
+extern "C" int orbit_forward_fct_passes_dev(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids,
+                                            const int* tracers, int nt, const double* e_dev, const double* ghost_dev,
+                                            const int* ghost_tracers, int n_passes, const orbit_opts* opts,
+                                            double* orbit_dev, orbit_info* info, double* pass_orbits_dev) {
+    if (n_passes > 1 && !pass_orbits_dev) return fail(ORBIT_ERR_INVALID, "forward_fct_passes: NULL pass_orbits");
+    return forward_fct_passes(ctx, op, grids, tracers, nt, e_dev, ghost_dev, ghost_tracers, n_passes, opts,
+                              orbit_dev, info, pass_orbits_dev);
 }
 
 extern "C" int orbit_fct_linop_create_dev(orbit_ctx* ctx, orbit_fct_grid* G, const double* c_base_dev, double dt,

@@ -39,6 +39,9 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
     warm: "orbit" starts each pass from the whole
     previous orbit (mixed precision), "c0" from its c0 only.  ctx: the context
     (stream) the solves run on, the FCT grids' own (default: the operator set's).
+    keep_passes (native only): also return the orbits of passes 0 .. n_passes-2,
+    (n_passes - 1, nt, n_bins+1, N), the states each later pass's FCT source
+    was evaluated at: (orbit, infos, passes).
     native: with warm "c0", run every pass inside the library
     (orbit_forward_fct_dev, no return to Python between passes; a ghost list
     goes in as one tensor with per-tracer flags; FP64: the same launches as
@@ -51,6 +54,8 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
         raise ValueError("FCT grids and the solves must share one context (stream)")
     if native and warm == "c0":
         return _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, ghost, maxiter, ctx, **kw)
+    if kw.pop("keep_passes", False):
+        raise ValueError("keep_passes needs the native passes (native=True, warm='c0')")
     kw["ctx"] = ctx
     orbit, info = op.solve_forward(e, tracers, tol=tol, precision=precision, maxiter=maxiter, **kw)
     hist = [info]
@@ -96,7 +101,8 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
 
 
 def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, ghost, maxiter, ctx,
-                        restart=30, inner_sweeps=0, acc_sweeps=0, use_graph=None, max_rounds=0, verbose=False):
+                        restart=30, inner_sweeps=0, acc_sweeps=0, use_graph=None, max_rounds=0, verbose=False,
+                        keep_passes=False):
     import ctypes as C
     import numpy as np
     from . import _lib
@@ -131,12 +137,17 @@ def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, gho
     info = (_lib.Info * (n_passes * nt))()
     out = torch.empty((nt, op.n_bins + 1, op.N), dtype=torch.float64, device=e.device)
     torch.cuda.current_stream(e.device).synchronize()  # the library runs on its own stream
-    rc = lib().orbit_forward_fct_dev(ctx.handle, op.handle, grids, tr.ctypes.data, nt, e.data_ptr(),
-                                     None if ghost is None else ghost.data_ptr(),
-                                     None if flags is None else flags.ctypes.data, int(n_passes),
-                                     C.byref(opts), out.data_ptr(), info)
+    args = (ctx.handle, op.handle, grids, tr.ctypes.data, nt, e.data_ptr(), None if ghost is None else ghost.data_ptr(),
+            None if flags is None else flags.ctypes.data, int(n_passes), C.byref(opts), out.data_ptr(), info)
+    passes = None
+    if keep_passes:
+        passes = torch.empty((max(n_passes - 1, 0), nt, op.n_bins + 1, op.N), dtype=torch.float64, device=e.device)
+        rc = lib().orbit_forward_fct_passes_dev(*args, passes.data_ptr() if n_passes > 1 else None)
+    else:
+        rc = lib().orbit_forward_fct_dev(*args)
     check(rc)
-    return out, [[info[p * nt + s].as_dict() for s in range(nt)] for p in range(n_passes)]
+    hist = [[info[p * nt + s].as_dict() for s in range(nt)] for p in range(n_passes)]
+    return (out, hist, passes) if keep_passes else (out, hist)
 
 
 def _ghost_groups(ghost, e):

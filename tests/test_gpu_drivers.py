@@ -204,3 +204,65 @@ def test_adjoint_fct_mixed_matches_oracle(setup):
     lr, Rr, _ = drivers_ref.adjoint_fct_gmres(lus[0], list(R[0]), Lref, tol=1e-12, max_iters=60, rel_tol=1e-9)
     gr = np.stack([wl.dtau * (lr[t] - Rr[t]) for t in range(N_BINS)])
     assert _rel(grad, gr) < 1e-6
+
+
+def _frozen_tangent(op, grids, passes, v, tol):
+    """dc of the n-pass recursion with the FCT coefficients frozen at each
+    pass's input orbit: dc^(0) = S(v), dc^(p) = S(v - L_AD^(p-1) dc^(p-1)[tau+1])."""
+    from orbit_amd.fct import apply_dev
+    dc, _ = op.solve_forward(v, [0], tol=tol, precision="fp64")
+    tmp = torch.empty(op.N, dtype=torch.float64, device="cuda")
+    for p in range(1, passes.shape[0] + 1):
+        L = drivers.frozen_operators(op, grids, passes[p - 1])
+        rhs = v.clone()
+        for tau in range(N_BINS):
+            apply_dev(L[tau][0], dc[0, tau + 1].contiguous(), tmp)
+            rhs[0, tau] -= tmp
+        dc, _ = op.solve_forward(rhs, [0], tol=tol, precision="fp64")
+    return dc
+
+
+def test_fct_autograd_is_the_derivative_of_its_passes(setup):
+    """orbit_solve_fct's backward ("passes") is the derivative of the forward it
+    ran: (1) a central finite difference of the true forward along e itself
+    (the passes are homogeneous of degree one in e, so the limiter's
+    switching does not enter) equals <grad, e> and J(e); (2) along a random
+    direction the gradient is the exact adjoint of the frozen-coefficient
+    tangent of the same passes (duality to 1e-8).  The converged-fixed-point
+    gradient ("fixed_point", the reference's outer GMRES) is another quantity
+    (the derivative of the limit of the passes); it runs and is printed
+    beside the others."""
+    wl, op, grids, lus, e = setup
+    n = 4
+    rng = np.random.default_rng(17)
+    gmask = torch.from_numpy(rng.random((1, N_BINS + 1, wl.N))).cuda()
+    et = torch.from_numpy(e[:1].copy()).cuda().requires_grad_(True)
+    orbit = hag.orbit_solve_fct(et, op, grids, tracers=[0], n_passes=n, tol=1e-12)
+    J = float((gmask * orbit.detach()).sum())
+    (gmask * orbit).sum().backward()
+    grad = et.grad.clone()
+    ed = torch.from_numpy(e[:1].copy()).cuda()
+    eps = 1e-3
+    fwd = lambda x: float((gmask * drivers.forward_fct(op, grids, x, tracers=[0], n_passes=n, tol=1e-12,  # noqa
+                                                       precision="fp64")[0]).sum())
+    fd = (fwd(ed * (1 + eps)) - fwd(ed * (1 - eps))) / (2 * eps)
+    ge = float((grad * ed).sum())
+    print(f"J {J:.12e}  FD along e {fd:.12e}  <grad, e> {ge:.12e}", flush=True)
+    assert abs(fd - J) <= 1e-9 * abs(J)
+    assert abs(ge - fd) <= 1e-8 * abs(fd)
+    # duality with the frozen tangent along a random direction
+    _, _, passes = drivers.forward_fct(op, grids, ed, tracers=[0], n_passes=n, tol=1e-12, precision="fp64",
+                                       keep_passes=True)
+    v = torch.from_numpy(rng.standard_normal((1, N_BINS, wl.N)) * np.abs(e[:1]).mean()).cuda()
+    dc = _frozen_tangent(op, grids, passes, v, 1e-12)
+    lhs, rhs = float((gmask * dc).sum()), float((grad * v).sum())
+    print(f"<g, dc(v)> {lhs:.12e}  <grad, v> {rhs:.12e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-8 * max(abs(lhs), float((gmask.abs() * dc.abs()).sum()))
+    # the fixed-point gradient: the derivative of the limit of the passes
+    et2 = torch.from_numpy(e[:1].copy()).cuda().requires_grad_(True)
+    o2 = hag.orbit_solve_fct(et2, op, grids, tracers=[0], n_passes=n, tol=1e-12, mode="fixed_point",
+                             rel_tol=1e-10, max_iters=80)
+    (gmask * o2).sum().backward()
+    gf = float((et2.grad * ed).sum())
+    print(f"fixed-point <grad, e> {gf:.12e} (rel. to the passes' {abs(gf - fd) / abs(fd):.2e})", flush=True)
+    assert np.isfinite(gf)

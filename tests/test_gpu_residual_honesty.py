@@ -47,3 +47,34 @@ def test_adjoint_residual_is_the_true_one(month, precision, tol):
         true = np.linalg.norm(sa - (l0 - orb.apply_monodromy_adjoint(l0, lus[s]))) / np.linalg.norm(sa)
         assert info[s]["converged"] and true <= 1.05 * tol, (precision, s, true)
         assert abs(info[s]["residual"] - true) <= 0.05 * tol, (precision, s, info[s]["residual"], true)
+
+
+@pytest.mark.parametrize("n_passes,tol", [(2, 1e-8), (3, 1e-8), (3, 1e-6)])
+def test_mixed_fct_pass_residual_is_the_true_one(month, n_passes, tol):
+    """The library's mixed forward FCT passes start each later pass from the
+    previous orbit plus the FP32 propagation of the forcing's change, with the
+    sweeps of its first accurate check set by an error bound.  For the last
+    pass the oracle rebuilds the forcing e + d_AD(previous pass's end-of-bin
+    states) (reference orbit/cli.py:1212-1241, fct.py:459-492) and recomputes
+    the residual of the returned c0 with SuperLU propagators: within the
+    tolerance, and the reported one agrees with it to 5 % of the tolerance."""
+    import torch
+
+    from oracle import fct_ref
+    from orbit_amd import drivers
+    wl, op, lus = month
+    grids = drivers.make_fct_grids(wl.grids, op.ctx)
+    e = torch.from_numpy(wl.e).cuda()
+    prev, _ = drivers.forward_fct(op, grids, e, n_passes=n_passes - 1, tol=tol, precision="mixed")
+    last, hist = drivers.forward_fct(op, grids, e, n_passes=n_passes, tol=tol, precision="mixed")
+    prev = prev.cpu().numpy()
+    last = last.cpu().numpy()
+    for s in range(wl.n_tracers):
+        e_eff = [wl.e[s, t] + fct_ref.compute_horizontal_fct_source(wl.grids[t], prev[s, t + 1], wl.dtau)
+                 for t in range(8)]
+        sv = orb.compute_s(lus[s], e_eff)
+        c0 = last[s, 0]
+        true = np.linalg.norm(sv - (c0 - orb.apply_monodromy(c0, lus[s]))) / np.linalg.norm(sv)
+        info = hist[-1][s]
+        assert info["converged"] and true <= 1.05 * tol, (n_passes, s, true)
+        assert abs(info["residual"] - true) <= 0.05 * tol, (n_passes, s, info["residual"], true)
