@@ -319,6 +319,22 @@ int orbit_lut_create(orbit_ctx* ctx, int ndim, const int* axis_len, const double
 int orbit_lut_query_dev(orbit_ctx* ctx, orbit_lut* lut, long long npts, const double* const* coords_dev,
                         const double* clamp_lo, const double* clamp_hi, double* const* out_dev);
 int orbit_lut_destroy(orbit_lut* lut);
+/* ISORROPIA partitioning update on the device (reference partitioning_per_bin
+ * and the blend of update_grid_partitioning, orbit/core/isorropia_closure.py:
+ * 75-478) over a 7-axis table whose fields field_nh4 / field_no3 are f_nh4 and
+ * f_no3.  Per bin b < n_bins and cell n < ncell, from the end-of-bin states
+ * state_dev[0..2] = pSO4, TotalNH, TotalNO3 (clipped at 0; bin b at
+ * + b*state_stride) and met_dev[0..3] = T (K), RH (percent), dust_fine and
+ * sea_salt_fine (ug/m3, nullable: 0; bin b at + b*met_stride): out_dev[f]
+ * (bin b at + b*ncell) receives f = f_nh_eq, f_nh_marg, f_no3_eq, f_no3_marg
+ * and, with cross != 0, f_nh_dno3, f_no3_dnh, f_nh_dso4, f_no3_dso4; with
+ * prev_dev (same layout) and 0 < alpha < 1 each output is
+ * alpha*new + (1 - alpha)*prev.  Device arrays of doubles on the context's
+ * stream; out must not overlap the inputs, except out[f] == prev[f]. */
+int orbit_lut_partitioning_dev(orbit_ctx* ctx, orbit_lut* lut, int field_nh4, int field_no3, int n_bins,
+                               long long ncell, const double* const* state_dev, long long state_stride,
+                               const double* const* met_dev, long long met_stride, int cross, double alpha,
+                               const double* const* prev_dev, double* const* out_dev);
 
 /* Air-mass-balance pre-flight over the shared transport blocks of an opset
  * made with orbit_opset_create_shared (reference orbit/core/operator.py:598-652);
