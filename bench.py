@@ -124,13 +124,14 @@ def cpu_baseline(n_tracers):
     }
 
 
-def traffic_from_profile(name):
-    """HBM bytes per launch of a sweep kernel from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE scaled by a calibration copy), or None."""
+def from_profile(name, what="traffic_bytes_per_launch"):
+    """A figure of a sweep kernel variant from the committed rocprofv3 summary
+    (HBM bytes per launch from FETCH_SIZE + WRITE_SIZE scaled by a calibration
+    kernel, or the kernel trace's average duration), or None."""
     path = os.path.join(ROOT, "profiles", "halfsweep_64x70x15_12tracers.md")
     if not os.path.exists(path):
         return None
-    m = re.search(name + r"_traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(name + "_" + what + r":\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -149,6 +150,9 @@ def main():
                     help="back-to-back half-sweep launches timed for the roofline")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 256 MiB device copy first (scales profiler byte counters)")
+    ap.add_argument("--mfma", action="store_true",
+                    help="column-block products of the reduced-precision sweeps on the matrix cores instead of "
+                         "the VALU loop (the default) for the headline")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:
@@ -170,6 +174,8 @@ def main():
     ctx = Context(0)
     op = OperatorSet.from_workload(wl, ctx)
     grids = drivers.make_fct_grids(wl.grids, ctx)
+    if args.mfma:
+        op.set_mfma(True)
     log(f"headline: 64x70x15 x {T} tracers, operator set {op.stats()['device_bytes'] / 1e9:.2f} GB "
         f"on device, setup {time.perf_counter() - t0:.1f}s")
     e_dev = torch.from_numpy(wl.e).to("cuda:0")
@@ -215,6 +221,45 @@ def main():
     sweep_bytes = sum(v.total_bytes for v in variants)
     log(f"instrumented step: {t_instr * 1e3:.1f} ms, {sum(v.launches for v in variants)} half-sweeps, "
         f"{sweep_ms:.2f} ms of sweep kernels, {sweep_bytes / 1e9:.1f} GB algorithmic, {untimed.value} untimed")
+
+    def timed_variants(fn):
+        _lib.check(_lib.lib().orbit_timing_start(ctx.handle, 20000))
+        fn()
+        torch.cuda.synchronize()
+        t = (_lib.SweepTiming * 32)()
+        n, u = C.c_int(), C.c_longlong()
+        _lib.check(_lib.lib().orbit_timing_stop(ctx.handle, t, 32, C.byref(n), C.byref(u)))
+        return {f"prec{t[i].precision}_cs{t[i].slots_per_wave}_mf{t[i].mfma}_oz{t[i].own_zero}_nz{t[i].nbr_zero}":
+                round(1e3 * t[i].total_ms / t[i].launches, 3) for i in range(min(n.value, 32))}
+
+    # the other block product (matrix cores against the VALU loop): the same
+    # step with the setting flipped, timed the same way
+    mfma_ab = None
+    if not args.quick:
+        # alternating rounds (5 steps after one untimed step each), medians
+        ms = {False: [], True: []}
+        for _ in range(3):
+            for on in (not args.mfma, args.mfma):
+                op.set_mfma(on)
+                step()
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                for _ in range(5):
+                    step()
+                torch.cuda.synchronize()
+                ms[on].append(1e3 * (time.perf_counter() - t1) / 5)
+        us = {}
+        for on in (not args.mfma, args.mfma):
+            op.set_mfma(on)
+            step()
+            us[on] = timed_variants(step)
+        op.set_mfma(args.mfma)
+        step()
+        torch.cuda.synchronize()
+        mfma_ab = {("mfma" if on else "valu"): {"ms_per_step_rounds": [round(x, 3) for x in ms[on]],
+                                                "ms_per_step_median": round(float(np.median(ms[on])), 3),
+                                                "us_per_launch": us[on]} for on in (False, True)}
+        log(f"block product A/B: {mfma_ab}")
 
     # plain FP64 with the same FCT passes: one orbit at a time, no graphs
     npl = min(args.plain_tracers, T)
@@ -276,13 +321,14 @@ def main():
         "speedup_from_batching": round(batched_fp64_value / plain_value, 2),
         "speedup_from_precision": round(value / batched_fp64_value, 2),
         "rel_err_c0_vs_cpu_oracle": oracle_err,
+        "block_product_ab": mfma_ab,
         "max_rel_err_c0_vs_fp64": max(errs),
         "rel_err_c0_vs_fp64_per_tracer": [float(f"{x:.3e}") for x in errs],
         "gmres_iters_mixed_per_pass": iters_m[-1] if iters_m else None,
         # the operator set's calibrated sweeps: relaxation omega of the red-black
         # correction, measured contraction (with its 20% margin), sweep counts
         "sweeps": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in op.stats().items()
-                   if k in ("relaxation", "sweep_contraction", "default_acc_sweeps", "default_inner_sweeps")},
+                   if k in ("relaxation", "sweep_contraction", "default_acc_sweeps", "default_inner_sweeps", "mfma")},
     }
 
     if not args.quick:
@@ -420,13 +466,16 @@ def main():
         us = 1e3 * v.total_ms / v.launches
         bpl = v.total_bytes / v.launches
         achieved = bpl / (us * 1e-6) / 1e9
-        tr = traffic_from_profile(vname(v))
+        tr = from_profile(vname(v))
+        prof_us = from_profile(vname(v), "rocprof_us_per_launch")
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr,
                 "kernel": f"{kname(v)} ({T} slots, 64x70x15, one colour)", "variant": vname(v),
                 "us_per_launch": round(us, 3), "algorithmic_bytes_per_launch": round(bpl),
                 "launches_per_step": int(v.launches), "ms_per_step": round(v.total_ms, 3),
                 "traffic_over_algorithmic": round(tr / bpl, 3) if tr else None,
+                "rocprof_us_per_launch": prof_us,
+                "frac_from_rocprof_duration": round(bpl / (prof_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if prof_us else None,
                 "timing": "kernel start/stop events (hipExtLaunchKernelGGL) on the library's stream over one "
                           "headline step (the timed work; graphs off so every launch is timed)",
                 "residency": "a bin's working set (40-60 MB) fits the 256 MB Infinity Cache: bytes leaving L2, "

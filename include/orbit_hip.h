@@ -76,6 +76,7 @@ typedef struct {
     double sweep_contraction;  /* measured error contraction of one red-black sweep (with margin) */
     int periodic_x;  /* 1 when the operators couple i = 0 and i = nx-1 (periodic longitude) */
     double relaxation;  /* omega of the sweeps' correction x += omega P (b - A x) */
+    int mfma;           /* 1: BF16-inverse sweeps multiply the shared column block on the matrix cores */
 } orbit_opset_stats;
 
 /* Air-mass-balance statistics (reference orbit/core/operator.py:598-652). */
@@ -143,6 +144,14 @@ int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st);
  * reference (its direct factorisation has no iteration).  Not to be called
  * while a solve on the opset runs. */
 int orbit_opset_set_relaxation(orbit_ctx* ctx, orbit_opset* op, double omega);
+/* Block product of the reduced-precision sweeps (FP32 inner, FP64-vector
+ * accurate; both with BF16 inverse blocks) of a shared-transport set: on = 1
+ * multiplies the shared 16x16 column block by the four slots of a wave on the
+ * matrix cores (v_mfma 16x16x4, FP32 or FP64); 0, the default, uses the VALU
+ * loop (measured 2-3 % faster per launch at 64x70x15 with 12 slots).  Either way a tracer's result does not depend on its batch; the
+ * two settings differ in the last bits.  No counterpart in the reference.  Not
+ * to be called while a solve on the opset runs. */
+int orbit_opset_set_mfma(orbit_opset* op, int on);
 /* Replace the per-tracer diagonals (D laid out [bin][t][N]) and refactor the
  * column blocks in place: the outer chemistry loop changes only loss rates. */
 int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const double* D);
@@ -460,6 +469,7 @@ int orbit_bench_halfsweep(orbit_ctx* ctx, orbit_opset* op, int bin, int precisio
 typedef struct {
     int precision;       /* ORBIT_PREC_FP64, ORBIT_SWEEP_ACC_MIXED or ORBIT_PREC_MIXED (FP32 inner) */
     int slots_per_wave;  /* 1 or 4 */
+    int mfma;            /* 1: the shared column block's product on the matrix cores */
     int own_zero;        /* 1: x of the colour known zero (not read) */
     int nbr_zero;        /* 1: x of the other colour known zero (not read) */
     long long launches;

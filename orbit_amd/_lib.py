@@ -38,7 +38,7 @@ EXPORTS = [
     "orbit_opset_create_grids", "orbit_lut_create", "orbit_lut_query_dev", "orbit_lut_destroy",
     "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation", "orbit_forward_fct_dev",
     "orbit_solve_forward_dev_noalloc", "orbit_timing_start", "orbit_timing_stop",
-    "orbit_forward_fct_passes_dev", "orbit_lut_partitioning_dev",
+    "orbit_forward_fct_passes_dev", "orbit_lut_partitioning_dev", "orbit_opset_set_mfma",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
 N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
@@ -71,7 +71,7 @@ class Info(C.Structure):
 
 
 class SweepTiming(C.Structure):
-    _fields_ = [("precision", C.c_int), ("slots_per_wave", C.c_int), ("own_zero", C.c_int),
+    _fields_ = [("precision", C.c_int), ("slots_per_wave", C.c_int), ("mfma", C.c_int), ("own_zero", C.c_int),
                 ("nbr_zero", C.c_int), ("launches", C.c_longlong), ("total_ms", C.c_double),
                 ("total_bytes", C.c_double)]
 
@@ -81,7 +81,7 @@ class OpsetStats(C.Structure):
                 ("n_tracers", C.c_int), ("n_groups", C.c_int),
                 ("default_inner_sweeps", C.c_int), ("default_acc_sweeps", C.c_int),
                 ("dtau", C.c_double), ("device_bytes", C.c_double), ("sweep_contraction", C.c_double),
-                ("periodic_x", C.c_int), ("relaxation", C.c_double)]
+                ("periodic_x", C.c_int), ("relaxation", C.c_double), ("mfma", C.c_int)]
 
 
 class MbStats(C.Structure):
@@ -135,6 +135,7 @@ def signatures():
         "orbit_solve_forward_dev_noalloc": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_lut_partitioning_dev": (_I, [_P, _P, _I, _I, _I, C.c_longlong, _P, C.c_longlong, _P, C.c_longlong,
                                             _I, C.c_double, _P, _P]),
+        "orbit_opset_set_mfma": (_I, [_P, _I]),
         "orbit_timing_start": (_I, [_P, _I]),
         "orbit_timing_stop": (_I, [_P, _P, _I, _P, _P]),
         "orbit_solve_adjoint_dev": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),

@@ -155,6 +155,13 @@ class OperatorSet:
         check(lib().orbit_opset_set_relaxation(self.ctx.handle, self.handle, float(omega)), allow_warn=False)
         return self.stats()
 
+    def set_mfma(self, on: bool = True) -> None:
+        """Block product of the reduced-precision sweeps on the matrix cores
+        (on) or on the VALU (the default, 2-3 % faster per launch at
+        64x70x15); results are batch-invariant either way and differ between
+        the two in the last bits."""
+        check(lib().orbit_opset_set_mfma(self.handle, int(bool(on))))
+
     def stats(self) -> dict:
         s = _lib.OpsetStats()
         check(lib().orbit_opset_stats_get(self.handle, C.byref(s)))

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 
 #include <hip/hip_ext.h>
 
@@ -304,7 +305,23 @@ __device__ __forceinline__ void wave_sync() {
 // replaced by zero with a select, so stale values there never enter; inactive
 // slots and padding columns compute but do not store.  The per-lane arithmetic
 // does not depend on CS, so a slot's result is the same in any batch.
-template <class XT, class AT, class PT, class CT, int CS, bool OZ, bool NZ>
+// Matrix-core product of the shared column block with the four slots' x
+// (CS = 1): D (16x16) = A (16x4) B (4x16) accumulated over four k-slices,
+// lane l holding A[l % 16][l / 16], B[l / 16][l % 16] and D[i][l % 16] with
+// i = 4 (l / 16) + r (FP32) or l / 16 + 4 r (FP64), r = 0..3 (measured).
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+template <class T> struct MfAcc;
+template <> struct MfAcc<float> { using V = mf_f4; };
+template <> struct MfAcc<double> { using V = mf_d4; };
+__device__ __forceinline__ void mfma_16x16x4(float a, float b, mf_f4& d) {
+    d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
+}
+__device__ __forceinline__ void mfma_16x16x4(double a, double b, mf_d4& d) {
+    d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
+}
+
+template <class XT, class AT, class PT, class CT, int CS, bool OZ, bool NZ, bool MF = false>
 __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     AT* sV = (AT*)smem;  // CS = 1 only: blockDim.x / 64 blocks of 256
@@ -363,7 +380,13 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     AT vrow[CS ? 4 : 16];
     CT dv = 0;
     if (!OZ) {
-        if (CS) {
+        if (CS && MF) {
+            // the matrix cores' A operand of k-slice q: lane l takes V[l % 16][4q + l / 16],
+            // i.e. element 64q + l of the transposed block (four coalesced 64-lane rows)
+            const AT* Vt = (const AT*)a.Vt + (size_t)col * 256 + (threadIdx.x & 63);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) vrow[q] = Vt[64 * q];
+        } else if (CS) {
             const AT* Vt = (const AT*)a.Vt + (size_t)col * 256 + (threadIdx.x & 63) * 4;
 #pragma unroll
             for (int q = 0; q < 4; ++q) vrow[q] = Vt[q];
@@ -396,7 +419,29 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     if (!OZ) {
         sx[threadIdx.x] = xo;
         CT acc = dv * xo;
-        if (CS) {
+        if (CS && MF) {
+            // V [x_0 x_1 x_2 x_3] on the matrix cores: B's column j < 4 is slot j's
+            // x (the wave's four 16-lane groups), the other columns are zero; the
+            // product's column j goes back to slot j's lanes through LDS
+            const int wl = threadIdx.x & 63, jc = wl & 15, kk = wl >> 4;
+            const CT* swx = sx + (threadIdx.x - wl);
+            wave_sync();
+            typename MfAcc<CT>::V dacc = {CT(0), CT(0), CT(0), CT(0)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const CT bq = jc < 4 ? swx[jc * 16 + 4 * q + kk] : CT(0);
+                mfma_16x16x4((CT)vrow[q], bq, dacc);
+            }
+            // D rows of lane l: 4 (l / 16) + r for the FP32 product, l / 16 + 4 r for FP64
+            constexpr bool f64 = sizeof(CT) == 8;
+            CT* so = (CT*)(sV + (threadIdx.x >> 6) * 256);
+            if (jc < 4) {
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) so[jc * 16 + (f64 ? kk + 4 * rr : 4 * kk + rr)] = dacc[rr];
+            }
+            wave_sync();
+            acc += so[wl];
+        } else if (CS) {
             AT* sv = sV + (threadIdx.x >> 6) * 256;
 #pragma unroll
             for (int q = 0; q < 4; ++q) sv[(threadIdx.x & 63) * 4 + q] = vrow[q];
@@ -1197,6 +1242,7 @@ extern "C" int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st) {
     st->dtau = op->dtau; st->device_bytes = op->bytes; st->sweep_contraction = op->rho;
     st->periodic_x = op->px;
     st->relaxation = op->omega;
+    st->mfma = op->mfma ? 1 : 0;
     return ORBIT_OK;
 }
 
@@ -1361,8 +1407,11 @@ double halfsweep_bytes(int ncols, int nslots, size_t ae, size_t pe, size_t xe, b
 
 template <class XT, class AT, class PT, class CT>
 int launch_sweep_kernel(orbit_ctx* ctx, int prec, SweepArgs a, const SweepShape& sh, bool own_zero, bool nbr_zero,
-                        hipStream_t st) {
-    const int cs = sh.cs, W = sh.threads;
+                        hipStream_t st, bool mf = false) {
+    // the matrix-core block product exists for the BF16-inverse sweeps (four-slot mapping)
+    constexpr bool kMfOk = std::is_same<PT, uint16_t>::value;
+    mf = mf && kMfOk;
+    const int cs = mf ? 1 : sh.cs, W = sh.threads;
     const int tc = cs ? W / 64 : W / 16;  // columns per tile
     const int ntiles = (a.ncols + tc - 1) / tc;
     const int nsg = cs ? (a.nslots + 3) / 4 : a.nslots;
@@ -1376,7 +1425,7 @@ int launch_sweep_kernel(orbit_ctx* ctx, int prec, SweepArgs a, const SweepShape&
             e0 = tm->ev[tm->next];
             e1 = tm->ev[tm->next + 1];
             tm->next += 2;
-            tm->recs.push_back({prec * 8 + cs * 4 + (own_zero ? 2 : 0) + (nbr_zero ? 1 : 0),
+            tm->recs.push_back({prec * 16 + (mf ? 8 : 0) + cs * 4 + (own_zero ? 2 : 0) + (nbr_zero ? 1 : 0),
                                 halfsweep_bytes(a.ncols, a.nslots, sizeof(AT), sizeof(PT), sizeof(XT), own_zero,
                                                 nbr_zero)});
         } else {
@@ -1391,8 +1440,23 @@ int launch_sweep_kernel(orbit_ctx* ctx, int prec, SweepArgs a, const SweepShape&
         else                                                                                                 \
             hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, CSV, OZV, NZV>), grid, block, lds, st, a);        \
     } while (0)
+#define ORBIT_SWEEP_LAUNCH_MF(OZV, NZV)                                                                         \
+    do {                                                                                                         \
+        if (e0)                                                                                                  \
+            hipExtLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, OZV, NZV, true>), grid, block, lds, st, e0, e1, \
+                                  0, a);                                                                         \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, OZV, NZV, true>), grid, block, lds, st, a);        \
+    } while (0)
     // (own_zero, nbr_zero) = (true, false) is the second colour of a cold first sweep
-    if (cs) {
+    if (mf) {
+        if constexpr (kMfOk) {
+            if (own_zero && nbr_zero) ORBIT_SWEEP_LAUNCH_MF(true, true);
+            else if (own_zero) ORBIT_SWEEP_LAUNCH_MF(true, false);
+            else if (nbr_zero) ORBIT_SWEEP_LAUNCH_MF(false, true);
+            else ORBIT_SWEEP_LAUNCH_MF(false, false);
+        }
+    } else if (cs) {
         if (own_zero && nbr_zero) ORBIT_SWEEP_LAUNCH(1, true, true);
         else if (own_zero) ORBIT_SWEEP_LAUNCH(1, true, false);
         else if (nbr_zero) ORBIT_SWEEP_LAUNCH(1, false, true);
@@ -1404,6 +1468,7 @@ int launch_sweep_kernel(orbit_ctx* ctx, int prec, SweepArgs a, const SweepShape&
         else ORBIT_SWEEP_LAUNCH(0, false, false);
     }
 #undef ORBIT_SWEEP_LAUNCH
+#undef ORBIT_SWEEP_LAUNCH_MF
     return ORBIT_OK;
 }
 
@@ -1438,9 +1503,13 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     a.nslots = B.nslots;
     const bool T = dir == ADJ;
     // launch shape of this precision and slot count (the four-slot mapping reads
-    // one shared column block, so it needs a single transport group)
+    // one shared column block, so it needs a single transport group); the
+    // BF16-inverse sweeps of a shared-transport set run the block product on the
+    // matrix cores, always in the four-slot mapping, so a slot's arithmetic is
+    // the same in every batch
     SweepShape sh = op->shapes[prec][slot_bucket(B.nslots)];
     if (op->ngroups != 1) sh.cs = 0;
+    const bool mf = op->mfma && prec != SP_F64 && op->ngroups == 1;
     if (prec == SP_F64) {
         a.V = T ? O.VT64 : O.V64; a.Vt = T ? O.V64 : O.VT64;
         a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT64 : O.P64;
@@ -1448,11 +1517,12 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     } else if (prec == SP_ACC_MIXED) {
         a.V = T ? O.VT64 : O.V64; a.Vt = T ? O.V64 : O.VT64;
         a.H = T ? O.HT64 : O.H64; a.d = O.d64; a.P = T ? O.PT16 : O.P16;
-        return launch_sweep_kernel<double, double, uint16_t, double>(B.ctx, prec, a, sh, own_zero, nbr_zero, B.ctx->stream);
+        return launch_sweep_kernel<double, double, uint16_t, double>(B.ctx, prec, a, sh, own_zero, nbr_zero, B.ctx->stream,
+                                                                     mf);
     }
     a.V = T ? O.VT32 : O.V32; a.Vt = T ? O.V32 : O.VT32;
     a.H = T ? O.HT32 : O.H32; a.d = O.d32; a.P = T ? O.PT16 : O.P16;
-    return launch_sweep_kernel<float, float, uint16_t, float>(B.ctx, prec, a, sh, own_zero, nbr_zero, B.ctx->stream);
+    return launch_sweep_kernel<float, float, uint16_t, float>(B.ctx, prec, a, sh, own_zero, nbr_zero, B.ctx->stream, mf);
 }
 
 // x = P_bin(rhs) approximately: `sweeps` red-black sweeps starting from x = 0.
@@ -2593,9 +2663,10 @@ extern "C" int orbit_timing_stop(orbit_ctx* ctx, orbit_sweep_timing* out, int ma
         const int v = tm->recs[i].variant;
         auto it = agg.find(v);
         if (it == agg.end()) {
-            const int p = v / 8;  // internal sweep precision -> ABI constant
+            const int p = v / 16;  // internal sweep precision -> ABI constant
             orbit_sweep_timing t{};
             t.precision = p == SP_F64 ? ORBIT_PREC_FP64 : p == SP_ACC_MIXED ? ORBIT_SWEEP_ACC_MIXED : ORBIT_PREC_MIXED;
+            t.mfma = (v & 8) ? 1 : 0;
             t.slots_per_wave = (v & 4) ? 4 : 1;
             t.own_zero = (v & 2) ? 1 : 0;
             t.nbr_zero = (v & 1) ? 1 : 0;
@@ -2634,6 +2705,13 @@ void drop_graphs(orbit_opset* op) {
 }
 
 }  // namespace
+
+extern "C" int orbit_opset_set_mfma(orbit_opset* op, int on) {
+    if (!op) return fail(ORBIT_ERR_INVALID, "set_mfma: NULL operator set");
+    op->mfma = on != 0;
+    drop_graphs(op);  // captured sweeps carry the old kernels
+    return ORBIT_OK;
+}
 
 extern "C" int orbit_opset_get_shapes(orbit_opset* op, orbit_sweep_shape* table) {
     if (!op || !table) return fail(ORBIT_ERR_INVALID, "get_shapes: NULL argument");

@@ -57,7 +57,7 @@ struct DevBuf {
 // variant and algorithmic bytes; solves do not replay captured graphs then.
 struct SweepTiming {
     struct Rec {
-        int variant;   // prec * 8 + cs * 4 + own_zero * 2 + nbr_zero
+        int variant;   // prec * 16 + mfma * 8 + cs * 4 + own_zero * 2 + nbr_zero
         double bytes;  // algorithmic bytes of the launch
     };
     std::vector<hipEvent_t> ev;  // 2 per launch
@@ -146,6 +146,7 @@ struct Opset {
     int def_inner = 6, def_acc = 14;
     double rho = 0.1;  // calibrated contraction of one red-black sweep (with margin)
     double omega = 1.0;  // relaxation of the sweeps' correction (chosen by the calibration)
+    bool mfma = false;   // block product of the BF16-inverse sweeps on the matrix cores (opt-in)
     std::vector<BinOps> bins;
     // Column colours of the sweeps: red / black ((i + j) even / odd) over the
     // first nxc columns of every row; with a periodic x and an odd nx the wrap
