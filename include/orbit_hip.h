@@ -201,6 +201,22 @@ int orbit_solve_forward_dev_noalloc(orbit_ctx* ctx, orbit_opset* op, const int* 
                                     const double* e_dev, const double* c_warm_dev,
                                     const orbit_opts* opts, double* orbit_out_dev, orbit_info* info);
 
+/* Periodic solves of slots from several operator sets (of any grid sizes) as
+ * one batch: every sweep, vector and reduction launch covers all slots (the
+ * uneven-orbit batch of many months and sub-domains; no counterpart in the
+ * reference, whose species and months are solved one LU at a time).  Slot s
+ * solves tracer tracers[s] of sets[slot_set[s]]: forward (adjoint = 0) for the
+ * emissions in_dev[s] or adjoint (1) for the receptors in_dev[s], each a
+ * device array (n_bins, N) of its set, into out_dev[s] (n_bins + 1, N), info[s]
+ * as orbit_solve_forward_dev.  Cold starts only (rhs_is_s and warm_orbit
+ * refused).  Every set needs one shared transport per bin and two sweep
+ * colours (not a periodic longitude with an odd nx), the same n_bins and
+ * dtau.  A slot's result is bit-identical to its set solved alone (with the
+ * matrix-core block product when every set has it on, else the VALU loop). */
+int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* const* sets, int nt, const int* slot_set,
+                          const int* tracers, const double* const* in_dev, int adjoint, const orbit_opts* opts,
+                          double* const* out_dev, orbit_info* info);
+
 /* Grid fields of one bin for the device-side transport assembly (the GridData
  * fields read by orbit/core/operator.py:281-353 and the assemblers it calls).
  * (nz,ny,nx) arrays are k-major; NULL marks an absent optional field. */

@@ -242,6 +242,26 @@ struct SweepArgs {
     int colour;         // 0 red ((i + j) even), 1 black
     int px;             // periodic x (wrap couplings)
     float omega;        // relaxation of the correction (1: Gauss-Seidel)
+    // group launches (slots of several operator sets, four-slot mapping): the
+    // slot groups' operators and geometry, and per block its (group, tile)
+    const struct GroupOp* gops;
+    const int* btab;    // group << 16 | tile, -1 for a block with no work
+};
+
+// One slot group of a group launch: up to four consecutive slots of one
+// operator set, with that set's operator arrays for the launch's bin and sweep
+// precision (tracer t's diagonal and inverse blocks at + t * stride) and grid.
+struct GroupOp {
+    const void* V;
+    const void* Vt;
+    const void* H;
+    const void* d;
+    const void* P;
+    long long dstride, pstride;
+    int ncols[4];       // columns of each colour
+    int nx, ny, nxc, px;
+    int first, count;   // slots first .. first + count - 1
+    float omega;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
@@ -321,32 +341,59 @@ __device__ __forceinline__ void mfma_16x16x4(double a, double b, mf_d4& d) {
     d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
 }
 
-template <class XT, class AT, class PT, class CT, int CS, bool OZ, bool NZ, bool MF = false>
+template <class XT, class AT, class PT, class CT, int CS, bool OZ, bool NZ, bool MF = false, bool GRP = false>
 __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     AT* sV = (AT*)smem;  // CS = 1 only: blockDim.x / 64 blocks of 256
     CT* sx = (CT*)(smem + (CS ? (blockDim.x >> 6) * 256 * sizeof(AT) : 0));
     CT* sr = sx + blockDim.x;
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    const int nsg = CS ? (a.nslots + 3) >> 2 : a.nslots;
     const int tc = CS ? (int)(blockDim.x >> 6) : (int)(blockDim.x >> 4);  // columns per tile
-    int tile, sg, slot, idx;
-    if (a.xcd_map)
-        xcd_tile_slot(blockIdx.x, nsg, a.tiles_per_xcd, tile, sg);
-    else
-        linear_tile_slot(blockIdx.x, nsg, tile, sg);
-    if (tile * tc >= a.ncols) return;  // uniform over the block
-    if (CS) {
-        slot = sg * 4 + (grp & 3);
+    // this block's operator and grid: the launch's, or (group launch) its slot group's
+    int ncols, nx, ny, nxc, px, tile, slot, idx, ls;
+    bool slot_ok;
+    float omega;
+    const void *Vb, *Vtb, *Hb, *db, *Pb;
+    long long vstride, hstride, dstride, pstride;
+    if constexpr (GRP) {  // four-slot mapping only
+        const int e = a.btab[blockIdx.x];
+        if (e < 0) return;  // uniform over the block
+        const GroupOp& go = a.gops[e >> 16];
+        tile = e & 0xffff;
+        ncols = go.ncols[a.colour];
+        nx = go.nx; ny = go.ny; nxc = go.nxc; px = go.px; omega = go.omega;
+        Vb = go.V; Vtb = go.Vt; Hb = go.H; db = go.d; Pb = go.P;
+        vstride = 0; hstride = 0; dstride = go.dstride; pstride = go.pstride;
+        const int q = grp & 3;
+        slot = go.first + q;
+        slot_ok = q < go.count;
+        ls = go.first + (q < go.count ? q : go.count - 1);
         idx = tile * tc + (grp >> 2);
     } else {
-        slot = sg;
-        idx = tile * tc + grp;
+        const int nsg = CS ? (a.nslots + 3) >> 2 : a.nslots;
+        int sg;
+        if (a.xcd_map)
+            xcd_tile_slot(blockIdx.x, nsg, a.tiles_per_xcd, tile, sg);
+        else
+            linear_tile_slot(blockIdx.x, nsg, tile, sg);
+        ncols = a.ncols;
+        if (tile * tc >= ncols) return;  // uniform over the block
+        nx = a.nx; ny = a.ny; nxc = a.nxc; px = a.px; omega = a.omega;
+        Vb = a.V; Vtb = a.Vt; Hb = a.H; db = a.d; Pb = a.P;
+        vstride = a.vstride; hstride = a.hstride; dstride = a.dstride; pstride = a.pstride;
+        if (CS) {
+            slot = sg * 4 + (grp & 3);
+            idx = tile * tc + (grp >> 2);
+        } else {
+            slot = sg;
+            idx = tile * tc + grp;
+        }
+        slot_ok = slot < a.nslots;
+        ls = slot < a.nslots ? slot : a.nslots - 1;
     }
-    const bool valid = idx < a.ncols && slot < a.nslots;
-    const int ls = slot < a.nslots ? slot : a.nslots - 1;
+    const bool valid = idx < ncols && slot_ok;
     int col, i, j;
-    colour_col(idx < a.ncols ? idx : 0, a.colour, a.nxc, a.nx, col, i, j);
+    colour_col(idx < ncols ? idx : 0, a.colour, nxc, nx, col, i, j);
     const size_t base = (size_t)col * 16 + lane;
     const XT* __restrict__ b = (const XT*)a.b + (size_t)ls * a.bstride;
     XT* __restrict__ x = (XT*)a.x + (size_t)ls * a.xstride;
@@ -359,18 +406,18 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     const CT bv = (CT)b[base];
     const CT xo = OZ ? CT(0) : (CT)x[base];
     CT xw = 0, xe = 0, xs = 0, xn = 0;
-    const int cw = west_col(col, i, a.nx, a.px), ce = east_col(col, i, a.nx, a.px);
+    const int cw = west_col(col, i, nx, px), ce = east_col(col, i, nx, px);
     if (!NZ) {
         xw = (CT)x[(size_t)(cw >= 0 ? cw : col) * 16 + lane];
         xe = (CT)x[(size_t)(ce >= 0 ? ce : col) * 16 + lane];
-        xs = (CT)x[(size_t)(j > 0 ? col - a.nx : col) * 16 + lane];
-        xn = (CT)x[(size_t)(j < a.ny - 1 ? col + a.nx : col) * 16 + lane];
+        xs = (CT)x[(size_t)(j > 0 ? col - nx : col) * 16 + lane];
+        xn = (CT)x[(size_t)(j < ny - 1 ? col + nx : col) * 16 + lane];
     }
     // per-tracer operator rows
-    const AT* __restrict__ V = (const AT*)a.V + (size_t)tr * a.vstride;
-    const AT* __restrict__ H = (const AT*)a.H + (size_t)tr * a.hstride;
-    const AT* __restrict__ d = (const AT*)a.d + (size_t)tr * a.dstride;
-    const PT* __restrict__ P = (const PT*)a.P + (size_t)tr * a.pstride;
+    const AT* __restrict__ V = (const AT*)Vb + (size_t)tr * vstride;
+    const AT* __restrict__ H = (const AT*)Hb + (size_t)tr * hstride;
+    const AT* __restrict__ d = (const AT*)db + (size_t)tr * dstride;
+    const PT* __restrict__ P = (const PT*)Pb + (size_t)tr * pstride;
     const PT* Pr = P + (size_t)col * 256 + lane * 16;
     PT prow[16];
 #pragma unroll
@@ -383,11 +430,11 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
         if (CS && MF) {
             // the matrix cores' A operand of k-slice q: lane l takes V[l % 16][4q + l / 16],
             // i.e. element 64q + l of the transposed block (four coalesced 64-lane rows)
-            const AT* Vt = (const AT*)a.Vt + (size_t)col * 256 + (threadIdx.x & 63);
+            const AT* Vt = (const AT*)Vtb + (size_t)col * 256 + (threadIdx.x & 63);
 #pragma unroll
             for (int q = 0; q < 4; ++q) vrow[q] = Vt[64 * q];
         } else if (CS) {
-            const AT* Vt = (const AT*)a.Vt + (size_t)col * 256 + (threadIdx.x & 63) * 4;
+            const AT* Vt = (const AT*)Vtb + (size_t)col * 256 + (threadIdx.x & 63) * 4;
 #pragma unroll
             for (int q = 0; q < 4; ++q) vrow[q] = Vt[q];
         } else {
@@ -413,7 +460,7 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
         hx += hw * (cw >= 0 ? xw : CT(0));
         hx += he * (ce >= 0 ? xe : CT(0));
         hx += hs * (j > 0 ? xs : CT(0));
-        hx += hn * (j < a.ny - 1 ? xn : CT(0));
+        hx += hn * (j < ny - 1 ? xn : CT(0));
         r -= hx;
     }
     if (!OZ) {
@@ -460,7 +507,7 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     CT dx = 0;
 #pragma unroll
     for (int m = 0; m < 16; ++m) dx += (CT)ldv(prow[m]) * sr[(grp << 4) + m];
-    if (on) x[base] = (XT)(xo + (CT)a.omega * dx);
+    if (on) x[base] = (XT)(xo + (CT)omega * dx);
 }
 
 // y = A x (one operator apply, FP64), used for residual checks of the inner solves.
@@ -579,26 +626,90 @@ __global__ __launch_bounds__(256) void k_int_to_ext(const double* __restrict__ s
     }
 }
 
+// Group solves: slot s's k-major (nstates, N_s) input at src[s] -> its padded
+// rows [s][state][n16max] (its ncol_s columns only), times scale; grid
+// (ceil(max ncol / 64), nslots * nstates).
+__global__ __launch_bounds__(256) void k_ext_to_int_g(const double* const* __restrict__ src, const int* __restrict__ sncol,
+                                                      const int* __restrict__ snz, double* __restrict__ dst,
+                                                      long long dstride, long long n16max, int nstates, double scale) {
+    __shared__ double tile[PT_COLS * PT_LD];
+    const int s = blockIdx.y;
+    const int slot = s / nstates, st = s - slot * nstates;
+    const int ncol = sncol[slot], nz = snz[slot];
+    const int col0 = blockIdx.x * PT_COLS;
+    if (col0 >= ncol) return;  // uniform over the block
+    const double* in = src[slot] + (size_t)st * (size_t)nz * ncol;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int k = e / PT_COLS, c = e - k * PT_COLS;
+        double v = 0.0;
+        if (k < nz && col0 + c < ncol) v = scale * in[(size_t)k * ncol + col0 + c];
+        tile[c * PT_LD + k] = v;
+    }
+    __syncthreads();
+    double* out = dst + (size_t)slot * dstride + (size_t)st * n16max + (size_t)col0 * 16;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int c = e >> 4, k = e & 15;
+        if (col0 + c < ncol) out[e] = tile[c * PT_LD + k];
+    }
+}
+
+// Group solves: padded rows [s][state][n16max] -> slot s's k-major
+// (nstates, N_s) output at dst[s]; same grid.
+__global__ __launch_bounds__(256) void k_int_to_ext_g(const double* __restrict__ src, long long sstride,
+                                                      long long n16max, double* const* __restrict__ dst,
+                                                      const int* __restrict__ sncol, const int* __restrict__ snz,
+                                                      int nstates) {
+    __shared__ double tile[PT_COLS * PT_LD];
+    const int s = blockIdx.y;
+    const int slot = s / nstates, st = s - slot * nstates;
+    const int ncol = sncol[slot], nz = snz[slot];
+    const int col0 = blockIdx.x * PT_COLS;
+    if (col0 >= ncol) return;  // uniform over the block
+    const double* in = src + (size_t)slot * sstride + (size_t)st * n16max + (size_t)col0 * 16;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int c = e >> 4, k = e & 15;
+        if (col0 + c < ncol) tile[c * PT_LD + k] = in[e];
+    }
+    __syncthreads();
+    double* out = dst[slot] + (size_t)st * (size_t)nz * ncol;
+    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
+        const int k = e / PT_COLS, c = e - k * PT_COLS;
+        if (k < nz && col0 + c < ncol) out[(size_t)k * ncol + col0 + c] = tile[c * PT_LD + k];
+    }
+}
+
 inline dim3 grid_permute(int ncol, int nstreams) { return dim3((unsigned)((ncol + PT_COLS - 1) / PT_COLS), nstreams); }
+
+// Per-slot vector lengths of a group solve (slots of operator sets of
+// different sizes in buffers of the largest): element g of slot s counts only
+// where g % seg < len[s]; the rest is padding, never read or written.  A null
+// len (single-set solves) makes every element count.
+struct SlotLen {
+    const int* len = nullptr;
+    int seg = 1;
+};
+__device__ __forceinline__ bool in_slot(const SlotLen& L, int slot, long long g) {
+    return !L.len || (int)(g % L.seg) < L.len[slot];
+}
 
 // out[slot] = a[slot] + alpha * b[slot]   (strides in elements)
 __global__ void k_axpby(const double* __restrict__ a, long long as, const double* __restrict__ b,
                         long long bs, double alpha, double* __restrict__ out, long long os, int n,
-                        const int* active) {
+                        const int* active, SlotLen sl) {
     const int slot = blockIdx.y;
     if (active && !active[slot]) return;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+    if (g >= n || !in_slot(sl, slot, g)) return;
     out[(size_t)slot * os + g] = a[(size_t)slot * as + g] + alpha * b[(size_t)slot * bs + g];
 }
 
 template <class S, class D>
 __global__ void k_convert_slots(const S* __restrict__ src, long long ss, D* __restrict__ dst,
-                                long long ds, int n, const int* active) {
+                                long long ds, int n, const int* active, SlotLen sl) {
     const int slot = blockIdx.y;
     if (active && !active[slot]) return;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+    if (g >= n || !in_slot(sl, slot, g)) return;
     dst[(size_t)slot * ds + g] = (D)src[(size_t)slot * ss + g];
 }
 
@@ -606,11 +717,11 @@ __global__ void k_convert_slots(const S* __restrict__ src, long long ss, D* __re
 template <class U>
 __global__ void k_sub_from(const double* __restrict__ v, long long vs, const U* __restrict__ u,
                            long long us, double* __restrict__ w, long long ws, int n,
-                           const int* active) {
+                           const int* active, SlotLen sl) {
     const int slot = blockIdx.y;
     if (active && !active[slot]) return;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+    if (g >= n || !in_slot(sl, slot, g)) return;
     w[(size_t)slot * ws + g] = v[(size_t)slot * vs + g] - (double)u[(size_t)slot * us + g];
 }
 
@@ -638,10 +749,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_mdot(const double* __restrict__
                                                       int nvec, const double* __restrict__ w,
                                                       long long wslot, int n, int nb,
                                                       double* __restrict__ partials,
-                                                      const int* active) {
+                                                      const int* active, SlotLen sl) {
     __shared__ double sh[4];
     const int slot = blockIdx.y, blk = blockIdx.x;
     if (active && !active[slot]) return;
+    const int nn = sl.len ? min(n, sl.len[slot]) : n;  // (single vectors: seg = n)
     const double* ws = w + (size_t)slot * wslot;
     const double* Vs = Vb + (size_t)slot * vslot;
     double wv[RED_ELEMS];
@@ -649,14 +761,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_mdot(const double* __restrict__
 #pragma unroll
     for (int e = 0; e < RED_ELEMS; ++e) {
         idx[e] = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
-        wv[e] = idx[e] < n ? ws[idx[e]] : 0.0;
+        wv[e] = idx[e] < nn ? ws[idx[e]] : 0.0;
     }
     for (int i = 0; i < nvec; ++i) {
         const double* vi = Vs + (size_t)i * n;
         double acc = 0.0;
 #pragma unroll
         for (int e = 0; e < RED_ELEMS; ++e)
-            if (idx[e] < n) acc += vi[idx[e]] * wv[e];
+            if (idx[e] < nn) acc += vi[idx[e]] * wv[e];
         double s = block_sum(acc, sh);
         if (threadIdx.x == 0) partials[((size_t)slot * nvec + i) * nb + blk] = s;
     }
@@ -691,10 +803,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_mupdate(const double* __restric
                                                          double sign, double* __restrict__ w,
                                                          long long wslot, int n, int nb,
                                                          double* __restrict__ npartials,
-                                                         const int* active, const int* counts) {
+                                                         const int* active, const int* counts, SlotLen sl) {
     __shared__ double sh[4];
     const int slot = blockIdx.y, blk = blockIdx.x;
     if (active && !active[slot]) return;
+    const int nn = sl.len ? min(n, sl.len[slot]) : n;
     if (counts) nvec = counts[slot];  // only the slot's own valid basis vectors
     double* ws = w + (size_t)slot * wslot;
     const double* Vs = Vb + (size_t)slot * vslot;
@@ -703,7 +816,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_mupdate(const double* __restric
 #pragma unroll
     for (int e = 0; e < RED_ELEMS; ++e) {
         int ix = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
-        if (ix < n) {
+        if (ix < nn) {
             double v = ws[ix];
             for (int i = 0; i < nvec; ++i) v -= sign * c[i] * Vs[(size_t)i * n + ix];
             ws[ix] = v;
@@ -718,15 +831,16 @@ __global__ __launch_bounds__(RED_THREADS) void k_mupdate(const double* __restric
 
 // partials of ||x||^2 per slot
 __global__ __launch_bounds__(RED_THREADS) void k_norm2(const double* __restrict__ x, long long xslot,
-                                                       int n, int nb, double* __restrict__ partials) {
+                                                       int n, int nb, double* __restrict__ partials, SlotLen sl) {
     __shared__ double sh[4];
     const int slot = blockIdx.y, blk = blockIdx.x;
+    const int nn = sl.len ? min(n, sl.len[slot]) : n;
     const double* xs = x + (size_t)slot * xslot;
     double acc = 0.0;
 #pragma unroll
     for (int e = 0; e < RED_ELEMS; ++e) {
         int ix = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
-        if (ix < n) acc += xs[ix] * xs[ix];
+        if (ix < nn) acc += xs[ix] * xs[ix];
     }
     double s = block_sum(acc, sh);
     if (threadIdx.x == 0) partials[(size_t)slot * nb + blk] = s;
@@ -734,7 +848,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_norm2(const double* __restrict_
 
 // partial counts of negative entries per slot over nstates states
 __global__ __launch_bounds__(RED_THREADS) void k_negcount(const double* __restrict__ x, long long xslot,
-                                                          int n, int nb, double* __restrict__ partials) {
+                                                          int n, int nb, double* __restrict__ partials, SlotLen sl) {
     __shared__ double sh[4];
     const int slot = blockIdx.y, blk = blockIdx.x;
     const double* xs = x + (size_t)slot * xslot;
@@ -742,7 +856,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_negcount(const double* __restri
 #pragma unroll
     for (int e = 0; e < RED_ELEMS; ++e) {
         int ix = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
-        if (ix < n && xs[ix] < 0.0) acc += 1.0;
+        if (ix < n && in_slot(sl, slot, ix) && xs[ix] < 0.0) acc += 1.0;
     }
     double s = block_sum(acc, sh);
     if (threadIdx.x == 0) partials[(size_t)slot * nb + blk] = s;
@@ -790,11 +904,11 @@ __global__ void k_gmres_start(GState* st, const double* norm2, int nslots, const
 
 // out[slot] = in[slot] * (1/scale[slot]) where scale is read from the state
 __global__ void k_scale_by(const double* __restrict__ in, long long is, double* __restrict__ out,
-                           long long os, const GState* st, int which, int n, const int* active) {
+                           long long os, const GState* st, int which, int n, const int* active, SlotLen sl) {
     const int slot = blockIdx.y;
     if (active && !active[slot]) return;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+    if (g >= n || !in_slot(sl, slot, g)) return;
     double s = which == 0 ? st[slot].beta : st[slot].hnext;
     double inv = s > 0.0 ? 1.0 / s : 0.0;
     out[(size_t)slot * os + g] = in[(size_t)slot * is + g] * inv;
@@ -1313,10 +1427,10 @@ extern "C" int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const do
 // solver
 // ------------------------------------------------------------------------- //
 namespace orbit {
-__global__ void k_zero(double* __restrict__ x, long long xslot, int n) {
+__global__ void k_zero(double* __restrict__ x, long long xslot, int n, SlotLen sl) {
     const int slot = blockIdx.y;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n) x[(size_t)slot * xslot + g] = 0.0;
+    if (g < n && in_slot(sl, slot, g)) x[(size_t)slot * xslot + g] = 0.0;
 }
 
 // x[col*16 + k] = 0 for the last column of every row (i = nx - 1) of every
@@ -1333,11 +1447,11 @@ __global__ void k_zero_last_col(T* __restrict__ x, long long xslot, int nx, int 
 
 // out = (in ? in : 0) + (float)e   per slot (FP32 forced sweeps)
 __global__ void k_add_f32(const float* __restrict__ in, long long is, const double* __restrict__ e, long long es,
-                          float* __restrict__ out, long long os, int n, const int* active) {
+                          float* __restrict__ out, long long os, int n, const int* active, SlotLen sl) {
     const int slot = blockIdx.y;
     if (active && !active[slot]) return;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+    if (g >= n || !in_slot(sl, slot, g)) return;
     const float v = (float)e[(size_t)slot * es + g];
     out[(size_t)slot * os + g] = in ? in[(size_t)slot * is + g] + v : v;
 }
@@ -1348,16 +1462,17 @@ __global__ void k_add_f32(const float* __restrict__ in, long long is, const doub
 __global__ __launch_bounds__(RED_THREADS) void k_acc_f32_norm(const float* __restrict__ in, long long is,
                                                               double* __restrict__ out, long long os, int n,
                                                               const int* active, int nb, int nvec, int vec,
-                                                              double* __restrict__ partials) {
+                                                              double* __restrict__ partials, SlotLen sl) {
     __shared__ double sh[4];
     const int slot = blockIdx.y, blk = blockIdx.x;
     const bool on = !active || active[slot];
+    const int nn = sl.len ? min(n, sl.len[slot]) : n;
     double acc = 0.0;
     if (on) {
 #pragma unroll
         for (int e = 0; e < RED_ELEMS; ++e) {
             const int ix = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
-            if (ix < n) {
+            if (ix < nn) {
                 const double v = (double)in[(size_t)slot * is + ix];
                 out[(size_t)slot * os + ix] += v;
                 acc += v * v;
@@ -1384,6 +1499,30 @@ namespace {
 enum Dir { FWD = 0, ADJ = 1 };
 enum SweepPrec { SP_F64 = 0, SP_ACC_MIXED = 1, SP_F32 = 2 };
 
+// A group solve: slots of several operator sets (any grid sizes) in one batch.
+// Slots are ordered by set; up to four consecutive slots of one set form a slot
+// group, the unit of the four-slot sweep mapping.  Device tables: GroupOps per
+// (bin, sweep precision, group), per colour the (group, tile) of every block
+// (XCD-contiguous), per slot the vector length, grid and in/out pointers.
+struct GroupSpec {
+    int ngroups = 0;
+    size_t n16max = 0;
+    int ncolmax = 0;
+    bool mf = false;
+    std::vector<orbit_opset*> slot_op;  // per slot (internal order)
+    const GroupOp* gops = nullptr;      // [bin][prec][group]
+    const int* btab[2] = {nullptr, nullptr};
+    int nblocks[2] = {0, 0};
+    double bytes[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // algorithmic bytes of a full launch, per precision and colour
+    const int* slen = nullptr;          // n16 of each slot's set
+    const int* sncol = nullptr;
+    const int* snz = nullptr;
+    const double* const* in_ptrs = nullptr;  // per slot: (n_bins, N) input
+    double* const* out_ptrs = nullptr;       // per slot: (n_bins + 1, N) output
+    int* inner_cnt = nullptr;           // per slot sweep counts of its set (filled by the solve)
+    int* acc_cnt = nullptr;
+};
+
 struct Batch {
     orbit_ctx* ctx;
     orbit_opset* op;
@@ -1391,6 +1530,7 @@ struct Batch {
     int* tmap_dev;
     int* active_dev;   // slots still iterating inside a GMRES cycle
     int* part_dev;     // slots taking part in the current round
+    const GroupSpec* grp = nullptr;  // a group solve's tables, else null
 };
 
 // Algorithmic bytes of one half-sweep launch: per column of the colour the
@@ -1472,9 +1612,79 @@ int launch_sweep_kernel(orbit_ctx* ctx, int prec, SweepArgs a, const SweepShape&
     return ORBIT_OK;
 }
 
+// A half-sweep of a group solve: every slot group's tiles of this colour in
+// one launch (four-slot mapping, 256 threads, the block table's order).
+template <class XT, class AT, class PT, class CT>
+int launch_group_kernel(const Batch& B, int bin, int prec, SweepArgs a, bool own_zero, bool nbr_zero) {
+    const GroupSpec& G = *B.grp;
+    const int colour = a.colour;
+    if (G.nblocks[colour] == 0) return ORBIT_OK;
+    a.gops = G.gops + ((size_t)bin * N_SWEEP_PRECS + prec) * G.ngroups;
+    a.btab = G.btab[colour];
+    const dim3 grid(G.nblocks[colour]), block(256);
+    const size_t lds = (size_t)4 * 256 * sizeof(AT) + 2 * 256 * sizeof(CT);
+    constexpr bool kMfOk = std::is_same<PT, uint16_t>::value;
+    const bool mf = G.mf && kMfOk;
+    hipStream_t st = B.ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (SweepTiming* tm = B.ctx->timing) {
+        if (tm->next + 2 <= tm->ev.size()) {
+            e0 = tm->ev[tm->next];
+            e1 = tm->ev[tm->next + 1];
+            tm->next += 2;
+            tm->recs.push_back({prec * 16 + (mf ? 8 : 0) + 4 + (own_zero ? 2 : 0) + (nbr_zero ? 1 : 0),
+                                G.bytes[prec][colour]});
+        } else {
+            ++tm->dropped;
+        }
+    }
+#define ORBIT_GROUP_LAUNCH(MFV, OZV, NZV)                                                                        \
+    do {                                                                                                         \
+        if (e0)                                                                                                  \
+            hipExtLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, OZV, NZV, MFV, true>), grid, block, lds, st, e0, \
+                                  e1, 0, a);                                                                     \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_halfsweep<XT, AT, PT, CT, 1, OZV, NZV, MFV, true>), grid, block, lds, st, a);   \
+    } while (0)
+    if (mf) {
+        if constexpr (kMfOk) {
+            if (own_zero && nbr_zero) ORBIT_GROUP_LAUNCH(true, true, true);
+            else if (own_zero) ORBIT_GROUP_LAUNCH(true, true, false);
+            else if (nbr_zero) ORBIT_GROUP_LAUNCH(true, false, true);
+            else ORBIT_GROUP_LAUNCH(true, false, false);
+        }
+    } else {
+        if (own_zero && nbr_zero) ORBIT_GROUP_LAUNCH(false, true, true);
+        else if (own_zero) ORBIT_GROUP_LAUNCH(false, true, false);
+        else if (nbr_zero) ORBIT_GROUP_LAUNCH(false, false, true);
+        else ORBIT_GROUP_LAUNCH(false, false, false);
+    }
+#undef ORBIT_GROUP_LAUNCH
+    return ORBIT_OK;
+}
+
 int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, const void* rhs,
                      long long bstride, void* x, long long xstride, bool own_zero, bool nbr_zero,
                      const int* mask, const int* counts = nullptr, int sweep_idx = 0) {
+    if (B.grp) {  // operators per slot group; the direction is the tables'
+        SweepArgs a{};
+        a.b = rhs;
+        a.bstride = bstride;
+        a.x = x;
+        a.xstride = xstride;
+        a.tmap = B.tmap_dev;
+        a.active = mask;
+        a.own_zero = own_zero;
+        a.nbr_zero = nbr_zero;
+        a.counts = counts;
+        a.sweep_idx = sweep_idx;
+        a.colour = colour;
+        a.nslots = B.nslots;
+        if (prec == SP_F64) return launch_group_kernel<double, double, double, double>(B, bin, prec, a, own_zero, nbr_zero);
+        if (prec == SP_ACC_MIXED)
+            return launch_group_kernel<double, double, uint16_t, double>(B, bin, prec, a, own_zero, nbr_zero);
+        return launch_group_kernel<float, float, uint16_t, float>(B, bin, prec, a, own_zero, nbr_zero);
+    }
     orbit_opset* op = B.op;
     const BinOps& O = op->bins[bin];
     SweepArgs a;
@@ -1535,7 +1745,8 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
 int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long long bstride, void* x,
                long long xstride, bool first, const int* mask, const int* counts, int sweep_idx) {
     orbit_opset* op = B.op;
-    if (first && op->ncolours == 4) {
+    const int ncolours = B.grp ? 2 : op->ncolours;
+    if (!B.grp && first && op->ncolours == 4) {
         const dim3 g((unsigned)((op->ny * 16 + 255) / 256), B.nslots);
         if (prec == SP_F32)
             hipLaunchKernelGGL(k_zero_last_col<float>, g, dim3(256), 0, B.ctx->stream, (float*)x, xstride, op->nx,
@@ -1544,7 +1755,7 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
             hipLaunchKernelGGL(k_zero_last_col<double>, g, dim3(256), 0, B.ctx->stream, (double*)x, xstride, op->nx,
                                op->ny, mask);
     }
-    for (int c = 0; c < op->ncolours; ++c) {
+    for (int c = 0; c < ncolours; ++c) {
         const int rc = launch_halfsweep(B, bin, dir, prec, c, rhs, bstride, x, xstride, first, first && c == 0, mask,
                                         counts, sweep_idx);
         if (rc) return rc;
@@ -1589,8 +1800,8 @@ int measure_contraction(orbit_ctx* ctx, orbit_opset* op, int ns, int* tr, const 
                 hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, ns), dim3(256), 0, st, O.V64, vstride, O.H64,
                                    hstride, O.d64, (long long)n16, x, y, xs, tr, op->ncol, op->nx, op->ny, op->px);
                 hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, ns), dim3(256), 0, st, b, 0LL, y, xs, -1.0, y, xs,
-                                   (int)n16, nullptr);
-                hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, y, xs, (int)n16, nb, red);
+                                   (int)n16, nullptr, SlotLen{});
+                hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, y, xs, (int)n16, nb, red, SlotLen{});
                 hipLaunchKernelGGL(k_sum_partials, grid_waves(ns), dim3(256), 0, st, red, nb,
                                    out + (size_t)(2 * bin + (k == 8)) * ns, ns);
             }
@@ -1656,7 +1867,7 @@ int calibration_tracers(orbit_ctx* ctx, orbit_opset* op, int ns, std::vector<int
     double* part = ctx->buf<double>(7);
     double* sums = part + (size_t)T * nb;
     hipLaunchKernelGGL(k_norm2, dim3(nb, T), dim3(RED_THREADS), 0, st, op->bins[0].d64, (long long)n16, (int)n16, nb,
-                       part);
+                       part, SlotLen{});
     hipLaunchKernelGGL(k_sum_partials, grid_waves(T), dim3(256), 0, st, part, nb, sums, T);
     ORBIT_HIP_CHECK(hipGetLastError());
     std::vector<double> hs(T);
@@ -1770,12 +1981,15 @@ struct Solver {
     int* kcols = nullptr;
     double* host = nullptr;   // pinned-free host staging (plain new[])
     int nb = 1;
+    SlotLen sl;               // per-slot lengths (group solves), else none
+    const int* inner_cnt = nullptr;  // per-slot sweep counts (group solves), else the uniform ones
+    const int* acc_cnt = nullptr;
 };
 
 int norms(Solver& S, const double* x, long long xslot, int n, std::vector<double>& out) {
     hipStream_t st = S.B.ctx->stream;
     const int nb = (n + RED_CHUNK - 1) / RED_CHUNK;
-    hipLaunchKernelGGL(k_norm2, dim3(nb, S.B.nslots), dim3(RED_THREADS), 0, st, x, xslot, n, nb, S.partials);
+    hipLaunchKernelGGL(k_norm2, dim3(nb, S.B.nslots), dim3(RED_THREADS), 0, st, x, xslot, n, nb, S.partials, S.sl);
     hipLaunchKernelGGL(k_sum_partials, grid_waves(S.B.nslots), dim3(256), 0, st, S.partials, nb, S.scal, S.B.nslots);
     ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.scal, S.B.nslots * 8, hipMemcpyDeviceToHost, st));
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
@@ -1789,7 +2003,7 @@ int norms(Solver& S, const double* x, long long xslot, int n, std::vector<double
 void norms_dev(Solver& S, const double* x, long long xslot, int n, double* dst) {
     hipStream_t st = S.B.ctx->stream;
     const int nb = (n + RED_CHUNK - 1) / RED_CHUNK;
-    hipLaunchKernelGGL(k_norm2, dim3(nb, S.B.nslots), dim3(RED_THREADS), 0, st, x, xslot, n, nb, S.partials);
+    hipLaunchKernelGGL(k_norm2, dim3(nb, S.B.nslots), dim3(RED_THREADS), 0, st, x, xslot, n, nb, S.partials, S.sl);
     hipLaunchKernelGGL(k_sum_partials, grid_waves(S.B.nslots), dim3(256), 0, st, S.partials, nb, dst, S.B.nslots);
 }
 
@@ -1807,38 +2021,38 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
     const int* mask = S.B.part_dev;
     if (S.dir == FWD) {
         if (zero_start)
-            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.orbit, os, (int)n16);
+            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.orbit, os, (int)n16, S.sl);
         else
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, 0.0, S.orbit, os, (int)n16, mask);
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, 0.0, S.orbit, os, (int)n16, mask, S.sl);
         for (int t = 0; t < nbn; ++t) {
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)t * n16, os,
-                               S.E + (size_t)t * n16, es, 1.0, S.ta, vs, (int)n16, mask);
+                               S.E + (size_t)t * n16, es, 1.0, S.ta, vs, (int)n16, mask, S.sl);
             int rc = inner_solve(S.B, t, FWD, sp, sweeps, S.ta, vs, S.orbit + (size_t)(t + 1) * n16, os, mask, warm,
-                                 warm ? S.nsw_dev : nullptr);
+                                 warm ? S.nsw_dev : S.acc_cnt);
             if (rc) return rc;
         }
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, S.orbit, os,
-                           -1.0, S.r, vs, (int)n16, mask);
-        if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask);
+                           -1.0, S.r, vs, (int)n16, mask, S.sl);
+        if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask, S.sl);
     } else {
         if (zero_start)
-            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, (int)n16);
+            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, (int)n16, S.sl);
         else
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, 0.0,
-                               S.orbit + (size_t)nbn * n16, os, (int)n16, mask);
+                               S.orbit + (size_t)nbn * n16, os, (int)n16, mask, S.sl);
         for (int t = nbn - 1; t >= 0; --t) {
             if (warm)  // previous P_t^T output = orbit[t] - R_t
                 hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)t * n16, os, S.E + (size_t)t * n16,
-                                   es, -1.0, S.ta, vs, (int)n16, mask);
+                                   es, -1.0, S.ta, vs, (int)n16, mask, S.sl);
             int rc = inner_solve(S.B, t, ADJ, sp, sweeps, S.orbit + (size_t)(t + 1) * n16, os, S.ta, vs, mask, warm,
-                                 warm ? S.nsw_dev : nullptr);
+                                 warm ? S.nsw_dev : S.acc_cnt);
             if (rc) return rc;
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.ta, vs, S.E + (size_t)t * n16, es, 1.0,
-                               S.orbit + (size_t)t * n16, os, (int)n16, mask);
+                               S.orbit + (size_t)t * n16, os, (int)n16, mask, S.sl);
         }
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os, S.orbit + (size_t)nbn * n16, os,
-                           -1.0, S.r, vs, (int)n16, mask);
-        if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask);
+                           -1.0, S.r, vs, (int)n16, mask, S.sl);
+        if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask, S.sl);
     }
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
@@ -1861,13 +2075,13 @@ int forced_sweep_f32(Solver& S, double* acc = nullptr, double* ymax2 = nullptr) 
     if (S.dir == FWD) {
         for (int t = 0; t < nbn; ++t) {
             hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, t == 0 ? nullptr : x, vs, S.E + (size_t)t * n16, es,
-                               rhs, vs, (int)n16, mask);
-            int rc = inner_solve(S.B, t, FWD, SP_F32, S.inner, rhs, vs, y, vs, mask);
+                               rhs, vs, (int)n16, mask, S.sl);
+            int rc = inner_solve(S.B, t, FWD, SP_F32, S.inner, rhs, vs, y, vs, mask, false, S.inner_cnt);
             if (rc) return rc;
             if (acc)
                 hipLaunchKernelGGL(k_acc_f32_norm, dim3(S.nb, S.B.nslots), dim3(RED_THREADS), 0, st, y, vs,
                                    acc + (size_t)(t + 1) * n16, (long long)(nbn + 1) * n16, (int)n16, mask, S.nb,
-                                   nbn, t, S.partials);
+                                   nbn, t, S.partials, S.sl);
             std::swap(x, y);
         }
         if (acc && ymax2)
@@ -1876,14 +2090,14 @@ int forced_sweep_f32(Solver& S, double* acc = nullptr, double* ymax2 = nullptr) 
     } else {
         // x = R_{nb-1}; then x = P_t^T x + R_t for t = nb-2..0
         hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.E + (size_t)(nbn - 1) * n16, es, x, vs,
-                           (int)n16, mask);
+                           (int)n16, mask, S.sl);
         for (int t = nbn - 2; t >= 0; --t) {
-            int rc = inner_solve(S.B, t, ADJ, SP_F32, S.inner, x, vs, y, vs, mask);
+            int rc = inner_solve(S.B, t, ADJ, SP_F32, S.inner, x, vs, y, vs, mask, false, S.inner_cnt);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, y, vs, S.E + (size_t)t * n16, es, x, vs, (int)n16, mask);
+            hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, y, vs, S.E + (size_t)t * n16, es, x, vs, (int)n16, mask, S.sl);
         }
     }
-    hipLaunchKernelGGL((k_convert_slots<float, double>), g, dim3(256), 0, st, x, vs, S.r, vs, (int)n16, mask);
+    hipLaunchKernelGGL((k_convert_slots<float, double>), g, dim3(256), 0, st, x, vs, S.r, vs, (int)n16, mask, S.sl);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
@@ -1896,7 +2110,7 @@ int mono_sweeps_f32(Solver& S, float** result) {
     const int nbn = S.nbins;
     for (int k = 0; k < nbn; ++k) {
         const int t = S.dir == FWD ? k : nbn - 1 - k;
-        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.inner, in, vs, out, vs, S.B.active_dev);
+        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.inner, in, vs, out, vs, S.B.active_dev, false, S.inner_cnt);
         if (rc) return rc;
         std::swap(in, out);
     }
@@ -1913,7 +2127,7 @@ int apply_I_minus_M(Solver& S, const double* v, long long vslot) {
     dim3 g(grid1(n16).x, S.B.nslots);
     const int* mask = S.B.active_dev;
     if (S.prec == ORBIT_PREC_MIXED) {
-        hipLaunchKernelGGL((k_convert_slots<double, float>), g, dim3(256), 0, st, v, vslot, S.fa, vs, (int)n16, mask);
+        hipLaunchKernelGGL((k_convert_slots<double, float>), g, dim3(256), 0, st, v, vslot, S.fa, vs, (int)n16, mask, S.sl);
         float* res = nullptr;
         if (S.use_graph) {
             GraphKey key{S.dir, 2, S.inner, S.B.nslots, {}, S.fa, S.fb};
@@ -1942,7 +2156,7 @@ int apply_I_minus_M(Solver& S, const double* v, long long vslot) {
             int rc = mono_sweeps_f32(S, &res);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL((k_sub_from<float>), g, dim3(256), 0, st, v, vslot, res, vs, S.w, vs, (int)n16, mask);
+        hipLaunchKernelGGL((k_sub_from<float>), g, dim3(256), 0, st, v, vslot, res, vs, S.w, vs, (int)n16, mask, S.sl);
     } else {
         const double* in = v;
         long long ins = vslot;
@@ -1952,13 +2166,13 @@ int apply_I_minus_M(Solver& S, const double* v, long long vslot) {
         for (int k = 0; k < nbn; ++k) {
             const int t = S.dir == FWD ? k : nbn - 1 - k;
             double* out = bufs[which];
-            int rc = inner_solve(S.B, t, S.dir, SP_F64, S.acc, in, ins, out, vs, mask);
+            int rc = inner_solve(S.B, t, S.dir, SP_F64, S.acc, in, ins, out, vs, mask, false, S.acc_cnt);
             if (rc) return rc;
             in = out;
             ins = vs;
             which ^= 1;
         }
-        hipLaunchKernelGGL((k_sub_from<double>), g, dim3(256), 0, st, v, vslot, in, vs, S.w, vs, (int)n16, mask);
+        hipLaunchKernelGGL((k_sub_from<double>), g, dim3(256), 0, st, v, vslot, in, vs, S.w, vs, (int)n16, mask, S.sl);
     }
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
@@ -1978,10 +2192,10 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     std::vector<int> act(part);
     { int rc0 = set_ints(st, S.B.active_dev, act.data(), ns); if (rc0) return rc0; }
     // beta = ||r||, V0 = r / beta
-    hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.r, vs, (int)n16, nb, S.partials);
+    hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.r, vs, (int)n16, nb, S.partials, S.sl);
     hipLaunchKernelGGL(k_sum_partials, grid_waves(ns), dim3(256), 0, st, S.partials, nb, S.scal, ns);
     hipLaunchKernelGGL(k_gmres_start, grid1(ns), dim3(256), 0, st, S.gst, S.scal, ns, S.B.active_dev);
-    hipLaunchKernelGGL(k_scale_by, g, dim3(256), 0, st, S.r, vs, S.Vk, kslot, S.gst, 0, (int)n16, S.B.active_dev);
+    hipLaunchKernelGGL(k_scale_by, g, dim3(256), 0, st, S.r, vs, S.Vk, kslot, S.gst, 0, (int)n16, S.B.active_dev, S.sl);
     steps.assign(ns, 0);
     std::vector<int> kc(ns, 0);
     for (int j = 0; j < S.m; ++j) {
@@ -1989,21 +2203,21 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
         if (rc) return rc;
         // CGS2
         hipLaunchKernelGGL(k_mdot, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.w, vs,
-                           (int)n16, nb, S.partials, S.B.active_dev);
+                           (int)n16, nb, S.partials, S.B.active_dev, S.sl);
         hipLaunchKernelGGL(k_finalize, grid_waves((size_t)ns * (j + 1)), dim3(256), 0, st, S.partials, j + 1, nb, S.h1, ns);
         hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.h1, 1.0,
-                           S.w, vs, (int)n16, nb, (double*)nullptr, S.B.active_dev, (const int*)nullptr);
+                           S.w, vs, (int)n16, nb, (double*)nullptr, S.B.active_dev, (const int*)nullptr, S.sl);
         hipLaunchKernelGGL(k_mdot, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.w, vs,
-                           (int)n16, nb, S.partials, S.B.active_dev);
+                           (int)n16, nb, S.partials, S.B.active_dev, S.sl);
         hipLaunchKernelGGL(k_finalize, grid_waves((size_t)ns * (j + 1)), dim3(256), 0, st, S.partials, j + 1, nb, S.h2, ns);
         double* np = S.partials + (size_t)ns * MAXM1 * nb;
         hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.h2, 1.0,
-                           S.w, vs, (int)n16, nb, np, S.B.active_dev, (const int*)nullptr);
+                           S.w, vs, (int)n16, nb, np, S.B.active_dev, (const int*)nullptr, S.sl);
         hipLaunchKernelGGL(k_sum_partials, grid_waves(ns), dim3(256), 0, st, np, nb, S.scal, ns);
         hipLaunchKernelGGL(k_gmres_column, grid1(ns), dim3(256), 0, st, S.gst, S.h1, S.h2, S.scal, j, ns,
                            S.B.active_dev, S.resid);
         hipLaunchKernelGGL(k_scale_by, g, dim3(256), 0, st, S.w, vs, S.Vk + (size_t)(j + 1) * n16, kslot,
-                           S.gst, 1, (int)n16, S.B.active_dev);
+                           S.gst, 1, (int)n16, S.B.active_dev, S.sl);
         ORBIT_HIP_CHECK(hipGetLastError());
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.resid, ns * 8, hipMemcpyDeviceToHost, st));
         ORBIT_HIP_CHECK(hipStreamSynchronize(st));
@@ -2027,7 +2241,7 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     { int rc0 = set_ints(st, S.kcols, kc.data(), ns); if (rc0) return rc0; }
     hipLaunchKernelGGL(k_gmres_solve, grid1(ns), dim3(256), 0, st, S.gst, S.kcols, ns, S.h1);
     hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, S.m, S.h1, -1.0, S.c, vs,
-                       (int)n16, nb, (double*)nullptr, S.B.part_dev, (const int*)S.kcols);
+                       (int)n16, nb, (double*)nullptr, S.B.part_dev, (const int*)S.kcols, S.sl);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
@@ -2057,9 +2271,12 @@ size_t solve_scratch_bytes(size_t nt, int nbn, size_t n16, int m) {
 
 int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, const double* in_dev,
                    const double* warm_dev, const orbit_opts* opts_in, double* out_dev, orbit_info* info,
-                   int dir, PassDelta* pd = nullptr, void* arena = nullptr, size_t arena_bytes = 0) {
+                   int dir, PassDelta* pd = nullptr, void* arena = nullptr, size_t arena_bytes = 0,
+                   const GroupSpec* G = nullptr) {
     if (!ctx || !op || !tracers || nt < 1 || !in_dev || !out_dev || !info)
         return fail(ORBIT_ERR_INVALID, "solve: NULL argument or empty tracer list");
+    // the operator set of slot s (a group solve's slots come from several)
+    auto op_of = [&](int s) { return G ? G->slot_op[s] : op; };
     orbit_opts o;
     orbit_opts_default(&o);
     if (opts_in) o = *opts_in;
@@ -2069,8 +2286,10 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     if (o.precision != ORBIT_PREC_FP64 && o.precision != ORBIT_PREC_MIXED)
         return fail(ORBIT_ERR_INVALID, "solve: unknown precision");
     for (int s = 0; s < nt; ++s)
-        if (tracers[s] < 0 || tracers[s] >= op->ntr)
+        if (tracers[s] < 0 || tracers[s] >= op_of(s)->ntr)
             return fail(ORBIT_ERR_INVALID, "solve: tracer index out of range");
+    if (G && (pd || o.rhs_is_s || warm_dev))
+        return fail(ORBIT_ERR_INVALID, "solve: a group solve starts cold from e (no s, warm start or pass state)");
     const bool delta = pd && pd->e_prev;
     if (pd && (dir != FWD || o.precision != ORBIT_PREC_MIXED || o.rhs_is_s || !pd->s_state))
         return fail(ORBIT_ERR_INVALID, "solve: pass-to-pass state needs a mixed-precision forward solve and s_state");
@@ -2084,13 +2303,24 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.dir = dir;
     S.rho = op->rho;
     S.eps_acc = std::min(1e-12, 0.01 * o.tol);
-    S.inner = o.inner_sweeps > 0 ? o.inner_sweeps : op->def_inner;
-    const double eps_in = std::pow(S.rho, S.inner);  // relative error of one inexact (FP32) P apply
-    S.acc = o.acc_sweeps > 0 ? o.acc_sweeps
-                             : std::min(80, std::max(4, (int)std::ceil(std::log(S.eps_acc) / std::log(S.rho))));
+    // per slot (its set's calibration): sweep contraction, inner and accurate
+    // sweep counts, the relative error of one inexact (FP32) P apply
+    std::vector<double> rho_s(nt), eps_in_s(nt);
+    std::vector<int> inner_s(nt), acc_s(nt);
+    for (int s = 0; s < nt; ++s) {
+        const orbit_opset* os_ = op_of(s);
+        rho_s[s] = os_->rho;
+        inner_s[s] = o.inner_sweeps > 0 ? o.inner_sweeps : os_->def_inner;
+        eps_in_s[s] = std::pow(rho_s[s], inner_s[s]);
+        acc_s[s] = o.acc_sweeps > 0 ? o.acc_sweeps
+                                    : std::min(80, std::max(4, (int)std::ceil(std::log(S.eps_acc) / std::log(rho_s[s]))));
+    }
+    S.inner = *std::max_element(inner_s.begin(), inner_s.end());
+    S.acc = *std::max_element(acc_s.begin(), acc_s.end());
+    const double eps_in = eps_in_s[0];  // (single-set solves: the same for every slot)
     S.m = o.restart;
-    S.use_graph = o.use_graph != 0 && S.prec == ORBIT_PREC_MIXED && !ctx->timing;
-    S.n16 = op->n16;
+    S.use_graph = o.use_graph != 0 && S.prec == ORBIT_PREC_MIXED && !ctx->timing && !G;
+    S.n16 = G ? G->n16max : op->n16;
     S.nbins = op->nbins;
     const size_t n16 = S.n16;
     const int nbn = op->nbins;
@@ -2148,6 +2378,15 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.B.ctx = ctx;
     S.B.op = op;
     S.B.nslots = nt;
+    if (G) {
+        S.B.grp = G;
+        S.sl = SlotLen{G->slen, (int)S.n16};
+        S.inner_cnt = G->inner_cnt;
+        S.acc_cnt = G->acc_cnt;
+        int rc0 = set_ints(ctx->stream, G->inner_cnt, inner_s.data(), nt);
+        if (!rc0) rc0 = set_ints(ctx->stream, G->acc_cnt, acc_s.data(), nt);
+        if (rc0) return rc0;
+    }
     std::vector<double> host(std::max(nt, 1) * 3 + 8);
     S.host = host.data();
 
@@ -2162,9 +2401,12 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     if (o.rhs_is_s) {
         // homogeneous sweeps: no forcing; s itself enters every residual
         hipLaunchKernelGGL(k_zero, dim3(grid1((size_t)nbn * n16).x, nt), dim3(256), 0, st, S.E, (long long)nbn * n16,
-                           (int)(nbn * n16));
+                           (int)(nbn * n16), S.sl);
         hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, in_dev, (long long)op->N, S.sv,
                            (long long)n16, 1, op->nz, op->ncol, 1.0);
+    } else if (G) {
+        hipLaunchKernelGGL(k_ext_to_int_g, grid_permute(G->ncolmax, nt * nbn), dim3(256), 0, st, G->in_ptrs, G->sncol,
+                           G->snz, S.E, (long long)nbn * n16, (long long)n16, nbn, scale);
     } else if (!delta) {  // (a forcing change loads e after propagating the change)
         hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
                            (long long)nbn * op->N, S.E, (long long)nbn * n16, nbn, op->nz, op->ncol, scale);
@@ -2182,7 +2424,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         std::vector<double> dnorm;
         const long long os_ = (long long)(nbn + 1) * n16, es_ = (long long)nbn * n16;
         if (S.sv) {
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.sv, vs, S.sv, vs, 0.0, S.r, vs, (int)n16, nullptr);
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.sv, vs, S.sv, vs, 0.0, S.r, vs, (int)n16, nullptr, S.sl);
         } else if (delta) {
             // The forcing changed by e - e_prev since the previous orbit: propagate
             // dtau (e - e_prev) from zero with the FP32 sweeps, add its states to the
@@ -2195,7 +2437,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             rc = forced_sweep_f32(S, S.orbit, S.h1);  // h1 (free until GMRES): max_t ||y_t||^2
             if (rc) return rc;
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, pd->s_state, vs, 1.0, S.r, vs, (int)n16,
-                               nullptr);
+                               nullptr, S.sl);
             hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
                                (long long)nbn * op->N, S.E, es_, nbn, op->nz, op->ncol, scale);
         } else {
@@ -2207,7 +2449,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         if (delta) {
             // max_t ||y_t|| of the change, ||s~|| and ||c|| with one host round trip
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os_, S.orbit, os_, 0.0, S.c, vs, (int)n16,
-                               nullptr);
+                               nullptr, S.sl);
             norms_dev(S, S.r, vs, (int)n16, S.h1 + nt);
             norms_dev(S, S.c, vs, (int)n16, S.h1 + 2 * nt);
             ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.h1, (size_t)3 * nt * 8, hipMemcpyDeviceToHost, st));
@@ -2241,13 +2483,13 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
                                (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
             const double* c0 = dir == FWD ? S.orbit : S.orbit + (size_t)nbn * n16;
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, c0, os_, c0, os_, 0.0, S.c, vs, (int)n16, nullptr);
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, c0, os_, c0, os_, 0.0, S.c, vs, (int)n16, nullptr, S.sl);
             warm_first = o.warm_sweeps > 0 ? std::min(o.warm_sweeps, S.acc) : S.acc;
         } else if (warm_dev) {
             hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
                                S.c, vs, 1, op->nz, op->ncol, 1.0);
         } else {
-            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.c, vs, (int)n16);  // r = s~ - (I-M)0
+            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.c, vs, (int)n16, S.sl);  // r = s~ - (I-M)0
             need_check = false;
         }
     } else {
@@ -2270,7 +2512,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                                S.c, vs, 1, op->nz, op->ncol, 1.0);
         } else {
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, sptr, sstride, sptr, sstride, 0.0, S.c, vs, (int)n16,
-                               nullptr);
+                               nullptr, S.sl);
         }
     }
     std::vector<int> part(nt, 1), conv(nt, 0), zero(nt, 0), iters(nt, 0), rounds(nt, 0);
@@ -2325,7 +2567,8 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         std::vector<double> target(nt, 0.0);
         for (int s = 0; s < nt; ++s) {
             if (S.prec == ORBIT_PREC_MIXED)
-                target[s] = std::max(0.3 * o.tol * snorm[s], (on_s_tilde ? std::max(eps_in, 1e-6) : 1e-6) * rnorm[s]);
+                target[s] = std::max(0.3 * o.tol * snorm[s],
+                                     (on_s_tilde ? std::max(eps_in_s[s], 1e-6) : 1e-6) * rnorm[s]);
             else
                 target[s] = o.tol * snorm[s];
         }
@@ -2357,10 +2600,10 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                 if (!part[s]) continue;
                 const double cn = std::sqrt(S.host[s]), yn = S.host[nt + s];
                 const double rel = cn > 0.0 ? yn / cn : 1.0;
-                int k = S.acc;
+                int k = acc_s[s];
                 if (rel > 0.0 && rel < 1.0)
-                    k = (int)std::ceil(std::log(S.eps_acc / rel) / std::log(S.rho));
-                nsw[s] = std::min(S.acc, std::max(2, k));
+                    k = (int)std::ceil(std::log(S.eps_acc / rel) / std::log(rho_s[s]));
+                nsw[s] = std::min(acc_s[s], std::max(2, k));
                 next_sweeps = std::max(next_sweeps, nsw[s]);
             }
             { int rc0 = set_ints(st, S.nsw_dev, nsw.data(), nt); if (rc0) return rc0; }
@@ -2381,25 +2624,29 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         { int rc0 = set_ints(st, S.B.part_dev, zmask.data(), nt); if (rc0) return rc0; }
         for (int k = 0; k <= nbn; ++k)
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)k * n16, os, S.orbit + (size_t)k * n16,
-                               os, -1.0, S.orbit + (size_t)k * n16, os, (int)n16, S.B.part_dev);
-        hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, -1.0, S.c, vs, (int)n16, S.B.part_dev);
+                               os, -1.0, S.orbit + (size_t)k * n16, os, (int)n16, S.B.part_dev, S.sl);
+        hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, -1.0, S.c, vs, (int)n16, S.B.part_dev, S.sl);
     }
     // adjoint output convention: state 0 and state nb are the GMRES solution lam0
     norms_dev(S, S.c, vs, (int)n16, S.h1);  // ||c||^2, read back with the negative counts
     if (dir == ADJ)
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, S.orbit + (size_t)nbn * n16,
-                           os, 0.0, S.orbit, os, (int)n16, nullptr);
+                           os, 0.0, S.orbit, os, (int)n16, nullptr, S.sl);
     // negative fraction over the 9 states
     {
         const int n = (int)((nbn + 1) * n16);
         const int nb = (n + RED_CHUNK - 1) / RED_CHUNK;
-        hipLaunchKernelGGL(k_negcount, dim3(nb, nt), dim3(RED_THREADS), 0, st, S.orbit, os, n, nb, S.partials);
+        hipLaunchKernelGGL(k_negcount, dim3(nb, nt), dim3(RED_THREADS), 0, st, S.orbit, os, n, nb, S.partials, S.sl);
         hipLaunchKernelGGL(k_sum_partials, grid_waves(nt), dim3(256), 0, st, S.partials, nb, S.scal, nt);
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.scal, nt * 8, hipMemcpyDeviceToHost, st));
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host + nt, S.h1, nt * 8, hipMemcpyDeviceToHost, st));
     }
-    hipLaunchKernelGGL(k_int_to_ext, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, S.orbit, os, out_dev,
-                       (long long)(nbn + 1) * op->N, nbn + 1, op->nz, op->ncol);
+    if (G)
+        hipLaunchKernelGGL(k_int_to_ext_g, grid_permute(G->ncolmax, nt * (nbn + 1)), dim3(256), 0, st, S.orbit, os,
+                           (long long)n16, G->out_ptrs, G->sncol, G->snz, nbn + 1);
+    else
+        hipLaunchKernelGGL(k_int_to_ext, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, S.orbit, os,
+                           out_dev, (long long)(nbn + 1) * op->N, nbn + 1, op->nz, op->ncol);
     ORBIT_HIP_CHECK(hipGetLastError());
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
     bool all_conv = true;
@@ -2412,7 +2659,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         I.s_norm = snorm[s];
         I.residual = zero[s] ? 0.0 : last_rel[s];
         I.periodicity = zero[s] ? 0.0 : (last_rel[s] * snorm[s]) / (std::sqrt(S.host[nt + s]) + 1e-30);
-        I.neg_frac = S.host[s] / ((double)(nbn + 1) * op->N);
+        I.neg_frac = S.host[s] / ((double)(nbn + 1) * op_of(s)->N);
         if (!conv[s]) all_conv = false;
     }
     return all_conv ? ORBIT_OK : ORBIT_WARN_NOT_CONVERGED;
@@ -2505,6 +2752,164 @@ extern "C" int orbit_solve_adjoint(orbit_ctx* ctx, orbit_opset* op, const int* t
     return host_solve(ctx, op, tracers, nt, R, lam_warm, opts, lam_out, info, ADJ);
 }
 
+// Periodic solves of slots from several operator sets in one batch: every
+// sweep, vector and reduction kernel covers all slots in one launch.  Slot s
+// solves tracer tracers[s] of sets[slot_set[s]] for the device input in_dev[s]
+// ((n_bins, N) of its set: e, or R with adjoint = 1) into out_dev[s].
+extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* const* sets, int nt, const int* slot_set,
+                                     const int* tracers, const double* const* in_dev, int adjoint,
+                                     const orbit_opts* opts, double* const* out_dev, orbit_info* info) {
+    if (!ctx || !sets || nsets < 1 || nt < 1 || !slot_set || !tracers || !in_dev || !out_dev || !info)
+        return fail(ORBIT_ERR_INVALID, "group solve: NULL argument or no slots");
+    if (adjoint != 0 && adjoint != 1) return fail(ORBIT_ERR_INVALID, "group solve: adjoint must be 0 or 1");
+    for (int q = 0; q < nsets; ++q) {
+        const orbit_opset* so = sets[q];
+        if (!so) return fail(ORBIT_ERR_INVALID, "group solve: NULL operator set");
+        if (so->ngroups != 1)
+            return fail(ORBIT_ERR_INVALID, "group solve: every operator set needs one shared transport per bin "
+                                           "(orbit_opset_create_shared or orbit_opset_create_grids)");
+        if (so->ncolours != 2)
+            return fail(ORBIT_ERR_INVALID, "group solve: a periodic longitude with an odd nx (four sweep colours) "
+                                           "is not supported here; solve that set alone");
+        if (so->nbins != sets[0]->nbins || so->dtau != sets[0]->dtau)
+            return fail(ORBIT_ERR_INVALID, "group solve: every operator set needs the same n_bins and dtau");
+    }
+    for (int s = 0; s < nt; ++s) {
+        if (slot_set[s] < 0 || slot_set[s] >= nsets) return fail(ORBIT_ERR_INVALID, "group solve: slot_set out of range");
+        if (tracers[s] < 0 || tracers[s] >= sets[slot_set[s]]->ntr)
+            return fail(ORBIT_ERR_INVALID, "group solve: tracer index out of range");
+        if (!in_dev[s] || !out_dev[s]) return fail(ORBIT_ERR_INVALID, "group solve: NULL input or output pointer");
+    }
+    orbit_opts o;
+    orbit_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.rhs_is_s || o.warm_orbit) return fail(ORBIT_ERR_INVALID, "group solve: starts cold from e (or R)");
+    hipSetDevice(ctx->device);
+    const int nbn = sets[0]->nbins;
+    // slots by set (stable): up to four consecutive slots of one set form a slot group
+    std::vector<int> perm(nt);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return slot_set[a] < slot_set[b]; });
+    GroupSpec G;
+    struct Grp { int set, first, count; };
+    std::vector<Grp> grps;
+    std::vector<int> tmap(nt);
+    G.mf = true;
+    for (int k = 0; k < nt; ++k) {
+        const int s = perm[k], q = slot_set[s];
+        orbit_opset* so = sets[q];
+        G.slot_op.push_back(so);
+        tmap[k] = tracers[s];
+        G.n16max = std::max(G.n16max, (size_t)so->n16);
+        G.ncolmax = std::max(G.ncolmax, so->ncol);
+        G.mf = G.mf && so->mfma;
+        if (grps.empty() || grps.back().set != q || grps.back().count == 4) grps.push_back({q, k, 1});
+        else grps.back().count += 1;
+    }
+    G.ngroups = (int)grps.size();
+    if (G.ngroups >= 32768) return fail(ORBIT_ERR_INVALID, "group solve: too many slot groups");
+    const bool T = adjoint != 0;
+    std::vector<GroupOp> hg((size_t)nbn * N_SWEEP_PRECS * G.ngroups);
+    for (int b = 0; b < nbn; ++b)
+        for (int p = 0; p < N_SWEEP_PRECS; ++p)
+            for (int g = 0; g < G.ngroups; ++g) {
+                const orbit_opset* so = sets[grps[g].set];
+                const BinOps& O = so->bins[b];
+                GroupOp& go = hg[((size_t)b * N_SWEEP_PRECS + p) * G.ngroups + g];
+                if (p == SP_F32) {
+                    go.V = T ? O.VT32 : O.V32; go.Vt = T ? O.V32 : O.VT32; go.H = T ? O.HT32 : O.H32; go.d = O.d32;
+                    go.P = T ? O.PT16 : O.P16;
+                } else {
+                    go.V = T ? O.VT64 : O.V64; go.Vt = T ? O.V64 : O.VT64; go.H = T ? O.HT64 : O.H64; go.d = O.d64;
+                    go.P = p == SP_F64 ? (T ? (const void*)O.PT64 : (const void*)O.P64)
+                                       : (T ? (const void*)O.PT16 : (const void*)O.P16);
+                }
+                go.dstride = so->n16;
+                go.pstride = (long long)so->ncol * 256;
+                for (int c = 0; c < 4; ++c) go.ncols[c] = so->ncols_c[c];
+                go.nx = so->nx; go.ny = so->ny; go.nxc = so->nxc; go.px = so->px;
+                go.first = grps[g].first;
+                go.count = grps[g].count;
+                go.omega = (float)so->omega;
+            }
+    // per colour, the (group, tile) of every block: by set, then tile, then the
+    // set's groups, so a tile's groups are neighbours; XCD-contiguous ranges
+    std::vector<int> tabs[2];
+    for (int c = 0; c < 2; ++c) {
+        std::vector<int> pairs;
+        for (int g0 = 0; g0 < G.ngroups;) {
+            int g1 = g0;
+            while (g1 < G.ngroups && grps[g1].set == grps[g0].set) ++g1;
+            const int ntile = (sets[grps[g0].set]->ncols_c[c] + 3) / 4;
+            if (ntile >= 65536) return fail(ORBIT_ERR_INVALID, "group solve: grid too large for the block table");
+            for (int t = 0; t < ntile; ++t)
+                for (int g = g0; g < g1; ++g) pairs.push_back(g * 65536 + t);
+            g0 = g1;
+        }
+        const size_t P = pairs.size(), per = (P + 7) / 8;
+        tabs[c].assign(8 * per, -1);
+        for (size_t x = 0; x < 8; ++x)
+            for (size_t r = 0; r < per; ++r)
+                if (x * per + r < P) tabs[c][8 * r + x] = pairs[x * per + r];
+        G.nblocks[c] = (int)(8 * per);
+        for (int p = 0; p < N_SWEEP_PRECS; ++p) {
+            const size_t ae = p == SP_F32 ? 4 : 8, pe = p == SP_F64 ? 8 : 2, xe = p == SP_F32 ? 4 : 8;
+            double bytes = 0.0;
+            for (const Grp& gr : grps)
+                bytes += halfsweep_bytes(sets[gr.set]->ncols_c[c], gr.count, ae, pe, xe, false, false);
+            G.bytes[p][c] = bytes;
+        }
+    }
+    // one upload of every table
+    std::vector<int> slen(nt), sncol(nt), snz(nt);
+    std::vector<const double*> inp(nt);
+    std::vector<double*> outp(nt);
+    for (int k = 0; k < nt; ++k) {
+        slen[k] = G.slot_op[k]->n16;
+        sncol[k] = G.slot_op[k]->ncol;
+        snz[k] = G.slot_op[k]->nz;
+        inp[k] = in_dev[perm[k]];
+        outp[k] = out_dev[perm[k]];
+    }
+    size_t off = 0;
+    auto place = [&](size_t bytes) { const size_t o0 = off; off += align256(bytes); return o0; };
+    const size_t o_gops = place(hg.size() * sizeof(GroupOp)), o_t0 = place(tabs[0].size() * 4),
+                 o_t1 = place(tabs[1].size() * 4), o_slen = place((size_t)nt * 4), o_ncol = place((size_t)nt * 4),
+                 o_nz = place((size_t)nt * 4), o_in = place((size_t)nt * 8), o_out = place((size_t)nt * 8),
+                 o_ic = place((size_t)nt * 4), o_ac = place((size_t)nt * 4);
+    std::vector<char> blob(off, 0);
+    auto put = [&](size_t o0, const void* src, size_t bytes) { if (bytes) memcpy(blob.data() + o0, src, bytes); };
+    put(o_gops, hg.data(), hg.size() * sizeof(GroupOp));
+    put(o_t0, tabs[0].data(), tabs[0].size() * 4);
+    put(o_t1, tabs[1].data(), tabs[1].size() * 4);
+    put(o_slen, slen.data(), (size_t)nt * 4);
+    put(o_ncol, sncol.data(), (size_t)nt * 4);
+    put(o_nz, snz.data(), (size_t)nt * 4);
+    put(o_in, inp.data(), (size_t)nt * 8);
+    put(o_out, outp.data(), (size_t)nt * 8);
+    int rc = ctx->ensure(9, off);
+    if (rc) return rc;
+    char* dev = (char*)ctx->scratch[9].p;
+    rc = h2d_sync(ctx->stream, dev, blob.data(), off);
+    if (rc) return rc;
+    G.gops = (const GroupOp*)(dev + o_gops);
+    G.btab[0] = (const int*)(dev + o_t0);
+    G.btab[1] = (const int*)(dev + o_t1);
+    G.slen = (const int*)(dev + o_slen);
+    G.sncol = (const int*)(dev + o_ncol);
+    G.snz = (const int*)(dev + o_nz);
+    G.in_ptrs = (const double* const*)(dev + o_in);
+    G.out_ptrs = (double* const*)(dev + o_out);
+    G.inner_cnt = (int*)(dev + o_ic);
+    G.acc_cnt = (int*)(dev + o_ac);
+    std::vector<orbit_info> inf(nt);
+    rc = solve_periodic(ctx, G.slot_op[0], tmap.data(), nt, inp[0], nullptr, &o, outp[0], inf.data(),
+                        adjoint ? ADJ : FWD, nullptr, nullptr, 0, &G);
+    if (rc < 0) return rc;
+    for (int k = 0; k < nt; ++k) info[perm[k]] = inf[k];
+    return rc;
+}
+
 // LU-protocol shim: accurate single solve, sweeps until ||b - A x|| <= 1e-14 ||b||.
 extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int tracer, int trans,
                              const double* b, double* x) {
@@ -2536,7 +2941,7 @@ extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int trace
     const long long vstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 256;
     const long long hstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 64;
     double hb[2];
-    hipLaunchKernelGGL(k_norm2, dim3(nb, 1), dim3(RED_THREADS), 0, st, rhs, (long long)n16, (int)n16, nb, red);
+    hipLaunchKernelGGL(k_norm2, dim3(nb, 1), dim3(RED_THREADS), 0, st, rhs, (long long)n16, (int)n16, nb, red, SlotLen{});
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, red, nb, red + nb, 1);
     ORBIT_HIP_CHECK(hipMemcpyAsync(&hb[0], red + nb, 8, hipMemcpyDeviceToHost, st));
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
@@ -2555,8 +2960,8 @@ extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int trace
                            trans ? O.HT64 : O.H64, hstride, O.d64, (long long)n16, xs, y, (long long)n16, tmap,
                            op->ncol, op->nx, op->ny, op->px);
         hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, 1), dim3(256), 0, st, rhs, (long long)n16, y, (long long)n16, -1.0,
-                           rr, (long long)n16, (int)n16, nullptr);
-        hipLaunchKernelGGL(k_norm2, dim3(nb, 1), dim3(RED_THREADS), 0, st, rr, (long long)n16, (int)n16, nb, red);
+                           rr, (long long)n16, (int)n16, nullptr, SlotLen{});
+        hipLaunchKernelGGL(k_norm2, dim3(nb, 1), dim3(RED_THREADS), 0, st, rr, (long long)n16, (int)n16, nb, red, SlotLen{});
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, red, nb, red + nb, 1);
         ORBIT_HIP_CHECK(hipMemcpyAsync(&hb[1], red + nb, 8, hipMemcpyDeviceToHost, st));
         ORBIT_HIP_CHECK(hipStreamSynchronize(st));

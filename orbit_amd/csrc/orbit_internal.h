@@ -71,8 +71,9 @@ struct Ctx {
     hipStream_t stream = nullptr;
     std::vector<void*> owned;  // scratch owned by the context
     // reusable scratch, grown on demand: 0-6 the periodic solve, 7 calibration
-    // and benchmarks, 8 the forward FCT driver's emissions, c0 and s~
-    DevBuf scratch[9];
+    // and benchmarks, 8 the forward FCT driver's emissions, c0 and s~, 9 the
+    // tables of a group solve
+    DevBuf scratch[10];
     // this device's copy of the fixed scratch array of the allocation-free
     // solve (its device address, looked up at context creation)
     void* fixed_scratch = nullptr;

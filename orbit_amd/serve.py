@@ -284,6 +284,57 @@ def run_poisson(server: OrbitServer, rate: float, n_queries: int, make, rng) -> 
             "p99_ms": float(np.percentile(lat, 99)), "mean_batch": float(np.mean([q.batch_size for q in queries]))}
 
 
+def solve_group(jobs, precision: str = "mixed", tol: float = 1e-8, adjoint: bool = False, maxiter: int = 200,
+                restart: int = 30, ctx: Optional[Context] = None):
+    """Every orbit of every job in one batched solve (orbit_solve_group_dev):
+    jobs = [(OperatorSet, input (nt, n_bins, N) CUDA float64, tracers)], the
+    input emissions (forward) or receptors (adjoint).  Each launch covers all
+    slots of all sets, whatever their sub-domain sizes.  Returns [(orbit
+    (nt, n_bins+1, N), infos)] per job, each bit-identical to the job solved
+    alone."""
+    import ctypes as C
+    import torch
+    from . import _lib
+    if not jobs:
+        return []
+    ctx = ctx or jobs[0][0].ctx
+    sets, set_idx = [], {}
+    slot_set, tracers, ins, outs, results, keep = [], [], [], [], [], []
+    for op, inp, tr in jobs:
+        tr = list(tr)
+        if id(op) not in set_idx:
+            set_idx[id(op)] = len(sets)
+            sets.append(op)
+        _lib.dev_f64(inp, ctx.device, "input")
+        if tuple(inp.shape) != (len(tr), op.n_bins, op.N):
+            raise ValueError(f"input must have shape {(len(tr), op.n_bins, op.N)}, got {tuple(inp.shape)}")
+        inp = inp.contiguous()
+        out = torch.empty((len(tr), op.n_bins + 1, op.N), dtype=torch.float64, device=inp.device)
+        keep.append(inp)
+        results.append(out)
+        for q, t in enumerate(tr):
+            slot_set.append(set_idx[id(op)])
+            tracers.append(int(t))
+            ins.append(inp[q].data_ptr())
+            outs.append(out[q].data_ptr())
+    nt = len(tracers)
+    opts = OperatorSet._opts(tol, maxiter, restart, precision, 0, 0, False, 0, False)
+    info = (_lib.Info * nt)()
+    arr_i = lambda v: (C.c_int * len(v))(*v)  # noqa: E731
+    arr_p = lambda v: (C.c_void_p * len(v))(*v)  # noqa: E731
+    torch.cuda.current_stream(results[0].device).synchronize()
+    _lib.check(_lib.lib().orbit_solve_group_dev(ctx.handle, len(sets), arr_p([s.handle.value for s in sets]), nt,
+                                                arr_i(slot_set), arr_i(tracers), arr_p(ins), int(bool(adjoint)),
+                                                C.byref(opts), arr_p(outs), info))
+    ctx.sync()
+    infos = [info[k].as_dict() for k in range(nt)]
+    out, k = [], 0
+    for (op, inp, tr), res in zip(jobs, results):
+        out.append((res, infos[k:k + len(tr)]))
+        k += len(tr)
+    return out
+
+
 def solve_many(jobs, n_workers: int = 4, precision: str = "mixed", tol: float = 1e-8,
                tuner: Optional[ShapeTuner] = None):
     """Batched uneven orbits: jobs = [(OperatorSet, e (nt, n_bins, N) CUDA, tracers)],
