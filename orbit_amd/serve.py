@@ -16,10 +16,12 @@ queries.  Every month's operator set stays resident on the device
   unless the caller gives one.
 * Continuous batching: each of n_workers host threads owns a HIP stream
   (Context).  A free worker takes the oldest waiting query and every other
-  waiting query with the same (month, size, mode), up to max_slots tracers and
-  within the budget, and solves them as one batch; queries that arrive while
-  batches are on the device are picked up by the next free worker, so work of
-  different batches overlaps on the GPU.
+  waiting query of the same mode and tolerance -- of any month and sub-domain
+  (a group solve over their operator sets, orbit_solve_group_dev), or of the
+  same month only when a set cannot join a group -- up to max_slots tracers
+  and within the budget, and solves them as one batch; queries that arrive
+  while batches are on the device are picked up by the next free worker, so
+  work of different batches overlaps on the GPU.
 * Results are independent of batching: every kernel and reduction works per
   tracer slot in a fixed order, so a query's answer is bit-identical alone and
   in any mix (tests/test_gpu_serve.py).
@@ -112,7 +114,8 @@ class OrbitServer:
 
     def __init__(self, opsets: Dict[Tuple[int, str], OperatorSet], n_workers: int = 4,
                  max_slots: int = 16, hbm_budget: Optional[float] = None, precision: str = "mixed",
-                 device: int = 0, tuner: Optional[ShapeTuner] = None, reserve_frac: float = 0.1):
+                 device: int = 0, tuner: Optional[ShapeTuner] = None, reserve_frac: float = 0.1,
+                 across_months: bool = True):
         if tuner is not None:  # before any worker launches on these sets
             for op in opsets.values():
                 tuner.apply(op)
@@ -127,6 +130,14 @@ class OrbitServer:
             self.budget_source = "caller"
         self.budget = float(hbm_budget)
         self.in_use = 0.0
+        # queries of different months / sub-domains share a batch (a group
+        # solve) when their sets allow it: one shared transport per bin and
+        # two sweep colours
+        self.across = across_months
+        self.groupable = {}
+        for key, op in opsets.items():
+            st = op.stats()
+            self.groupable[key] = st["n_groups"] == 1 and not (st["periodic_x"] and op.nx >= 3 and op.nx % 2)
         self.precision = precision
         self.queue: deque = deque()
         self.cv = threading.Condition()
@@ -169,26 +180,33 @@ class OrbitServer:
             c.close()
 
     # -- worker side ------------------------------------------------------
+    def _compatible(self, q, head) -> bool:
+        if (q.mode, q.tol) != (head.mode, head.tol):
+            return False
+        if (q.month, q.size) == (head.month, head.size):
+            return True
+        return self.across and self.groupable[(q.month, q.size)] and self.groupable[(head.month, head.size)]
+
     def _take_batch(self):
         """Oldest waiting query plus compatible ones that fit (called with cv held)."""
         while self.queue:
             head = self.queue[0]
-            op = self.opsets[(head.month, head.size)]
-            per = slot_bytes(op)
-            if self.in_use + per * len(head.tracers) > self.budget and self.in_use > 0:
+            per_head = slot_bytes(self.opsets[(head.month, head.size)])
+            if self.in_use + per_head * len(head.tracers) > self.budget and self.in_use > 0:
                 return None, 0.0                      # wait for memory to free up
-            batch, slots = [], 0
+            batch, slots, need = [], 0, 0.0
             for q in list(self.queue):
-                if (q.month, q.size, q.mode, q.tol) != (head.month, head.size, head.mode, head.tol):
+                if not self._compatible(q, head):
                     continue
                 n = len(q.tracers)
-                if batch and (slots + n > self.max_slots or self.in_use + per * (slots + n) > self.budget):
+                b = slot_bytes(self.opsets[(q.month, q.size)]) * n
+                if batch and (slots + n > self.max_slots or self.in_use + need + b > self.budget):
                     continue
                 batch.append(q)
                 slots += n
+                need += b
             for q in batch:
                 self.queue.remove(q)
-            need = per * slots
             self.in_use += need
             return batch, need
         return None, 0.0
@@ -208,20 +226,27 @@ class OrbitServer:
             t0 = time.perf_counter()
             try:
                 head = batch[0]
-                op = self.opsets[(head.month, head.size)]
-                tracers = [t for q in batch for t in q.tracers]
-                inp = torch.cat([q.payload for q in batch], dim=0).contiguous()
-                if head.mode == "forward":
-                    out, info = op.solve_forward(inp, tracers, tol=head.tol, precision=self.precision, ctx=ctx)
+                nslots = sum(len(q.tracers) for q in batch)
+                if len({(q.month, q.size) for q in batch}) > 1:
+                    # several months / sub-domains: one group solve over all their slots
+                    res = solve_group([(self.opsets[(q.month, q.size)], q.payload, q.tracers) for q in batch],
+                                      precision=self.precision, tol=head.tol, adjoint=head.mode == "adjoint",
+                                      ctx=ctx)
                 else:
-                    out, info = op.solve_adjoint(inp, tracers, tol=head.tol, precision=self.precision, ctx=ctx)
+                    op = self.opsets[(head.month, head.size)]
+                    tracers = [t for q in batch for t in q.tracers]
+                    inp = torch.cat([q.payload for q in batch], dim=0).contiguous()
+                    fn = op.solve_forward if head.mode == "forward" else op.solve_adjoint
+                    out, info = fn(inp, tracers, tol=head.tol, precision=self.precision, ctx=ctx)
+                    res, k = [], 0
+                    for q in batch:
+                        n = len(q.tracers)
+                        res.append((out[k:k + n], info[k:k + n]))
+                        k += n
                 t1 = time.perf_counter()
-                k = 0
-                for q in batch:
-                    n = len(q.tracers)
-                    q.t_start, q.t_done, q.batch_size = t0, t1, len(tracers)
-                    q.future.set_result((out[k:k + n], info[k:k + n]))
-                    k += n
+                for q, r in zip(batch, res):
+                    q.t_start, q.t_done, q.batch_size = t0, t1, nslots
+                    q.future.set_result(r)
             except Exception as exc:  # surfaced to every query of the batch
                 for q in batch:
                     q.future.set_exception(exc)
