@@ -55,6 +55,20 @@ int Ctx::ensure(int slot, size_t bytes) {
 // Host-to-device copy of a small host array that completes before returning:
 // a pageable source may be read by the runtime when the stream reaches the copy,
 // so the source must not be freed or rewritten before then.
+int Ctx::ensure_pinned(size_t n) {
+    if (pinned_n >= n) return ORBIT_OK;
+    if (pinned) hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_n = 0;
+    const size_t want = std::max<size_t>(n, 256);
+    if (hipHostMalloc((void**)&pinned, want * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        pinned = nullptr;
+        return fail(ORBIT_ERR_NOMEM, "solve: pinned host allocation failed");
+    }
+    pinned_n = want;
+    return ORBIT_OK;
+}
+
 int h2d_sync(hipStream_t st, void* dst, const void* src, size_t bytes) {
     ORBIT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
     ORBIT_HIP_CHECK(hipStreamSynchronize(st));
@@ -969,6 +983,175 @@ __global__ void k_gmres_solve(GState* st, const int* kcols, int nslots, double* 
     s.ynorm = sqrt(yn);
 }
 
+// ------------------------------------------------------------------------- //
+// fused Arnoldi-step kernels: the same arithmetic and summation orders as the
+// separate kernels above, fewer launches per step
+// ------------------------------------------------------------------------- //
+// w = v - u (the new Krylov direction, stored) and the partials of V_i . w
+// (k_sub_from then k_mdot in one pass over w).
+template <class U>
+__global__ __launch_bounds__(RED_THREADS) void k_mdot_sub(const double* __restrict__ Vb, long long vslot, int nvec,
+                                                          const double* __restrict__ v, long long vs,
+                                                          const U* __restrict__ u, long long us,
+                                                          double* __restrict__ w, long long wslot, int n, int nb,
+                                                          double* __restrict__ partials, const int* active,
+                                                          SlotLen sl) {
+    __shared__ double sh[4];
+    const int slot = blockIdx.y, blk = blockIdx.x;
+    if (active && !active[slot]) return;
+    const int nn = sl.len ? min(n, sl.len[slot]) : n;
+    const double* Vs = Vb + (size_t)slot * vslot;
+    double wv[RED_ELEMS];
+    int idx[RED_ELEMS];
+#pragma unroll
+    for (int e = 0; e < RED_ELEMS; ++e) {
+        idx[e] = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
+        wv[e] = 0.0;
+        if (idx[e] < nn) {
+            wv[e] = v[(size_t)slot * vs + idx[e]] - (double)u[(size_t)slot * us + idx[e]];
+            w[(size_t)slot * wslot + idx[e]] = wv[e];
+        }
+    }
+    for (int i = 0; i < nvec; ++i) {
+        const double* vi = Vs + (size_t)i * n;
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < RED_ELEMS; ++e)
+            if (idx[e] < nn) acc += vi[idx[e]] * wv[e];
+        double s = block_sum(acc, sh);
+        if (threadIdx.x == 0) partials[((size_t)slot * nvec + i) * nb + blk] = s;
+    }
+}
+
+// w -= sum_i c_i V_i with c_i summed here from k_mdot's partials in k_finalize's
+// order (every block the same sums; block 0 also stores them in coef_out), and
+// optionally the partials of ||w||^2 (k_finalize then k_mupdate in one launch).
+__global__ __launch_bounds__(RED_THREADS) void k_mupdate_fin(const double* __restrict__ Vb, long long vslot,
+                                                             int nvec, const double* __restrict__ partials,
+                                                             double* __restrict__ coef_out, double* __restrict__ w,
+                                                             long long wslot, int n, int nb,
+                                                             double* __restrict__ npartials, const int* active,
+                                                             SlotLen sl) {
+    __shared__ double sh[4];
+    __shared__ double c[MAXM1];
+    const int slot = blockIdx.y, blk = blockIdx.x;
+    if (active && !active[slot]) return;
+    for (int i = threadIdx.x >> 6; i < nvec; i += RED_THREADS / 64) {
+        const double s = wave_sum_partials(partials + ((size_t)slot * nvec + i) * nb, nb);
+        if ((threadIdx.x & 63) == 0) {
+            c[i] = s;
+            if (blk == 0) coef_out[(size_t)slot * MAXM1 + i] = s;
+        }
+    }
+    __syncthreads();
+    const int nn = sl.len ? min(n, sl.len[slot]) : n;
+    double* ws = w + (size_t)slot * wslot;
+    const double* Vs = Vb + (size_t)slot * vslot;
+    const double sign = 1.0;
+    double acc2 = 0.0;
+#pragma unroll
+    for (int e = 0; e < RED_ELEMS; ++e) {
+        int ix = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
+        if (ix < nn) {
+            double v = ws[ix];
+            for (int i = 0; i < nvec; ++i) v -= sign * c[i] * Vs[(size_t)i * n + ix];
+            ws[ix] = v;
+            acc2 += v * v;
+        }
+    }
+    if (npartials) {
+        double s = block_sum(acc2, sh);
+        if (threadIdx.x == 0) npartials[(size_t)slot * nb + blk] = s;
+    }
+}
+
+// The end of Arnoldi step j: ||w||^2 summed from its partials (k_sum_partials'
+// order) by every block; block 0 of a slot updates its Hessenberg column and
+// Givens rotation (k_gmres_column, active slots) or reports its residual
+// (inactive ones); every block of an active slot writes its part of
+// V_{j+1} = w / ||w|| (k_scale_by).
+__global__ __launch_bounds__(256) void k_arnoldi_tail(GState* st, const double* __restrict__ h1,
+                                                      const double* __restrict__ h2,
+                                                      const double* __restrict__ npartials, int nb, int j,
+                                                      const int* active, double* resid_out,
+                                                      const double* __restrict__ w, long long ws,
+                                                      double* __restrict__ vnext, long long vs, int n, SlotLen sl) {
+    __shared__ double hn_s;
+    const int slot = blockIdx.y;
+    const bool on = !active || active[slot];
+    if (!on) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) resid_out[slot] = st[slot].resid;
+        return;
+    }
+    if (threadIdx.x < 64) {
+        const double s = wave_sum_partials(npartials + (size_t)slot * nb, nb);
+        if (threadIdx.x == 0) {
+            hn_s = sqrt(s);
+            if (blockIdx.x == 0) {
+                GState& S = st[slot];
+                double* col = S.H + (size_t)j * MAXM1;
+                for (int i = 0; i <= j; ++i) col[i] = h1[(size_t)slot * MAXM1 + i] + h2[(size_t)slot * MAXM1 + i];
+                const double hn = hn_s;
+                col[j + 1] = hn;
+                S.hnext = hn;
+                for (int i = 0; i < j; ++i) {
+                    double a = col[i], b = col[i + 1];
+                    col[i] = S.cs[i] * a + S.sn[i] * b;
+                    col[i + 1] = -S.sn[i] * a + S.cs[i] * b;
+                }
+                double a = col[j], b = col[j + 1];
+                double r = hypot(a, b);
+                double cc = r > 0.0 ? a / r : 1.0, sn = r > 0.0 ? b / r : 0.0;
+                S.cs[j] = cc;
+                S.sn[j] = sn;
+                col[j] = r;
+                col[j + 1] = 0.0;
+                S.g[j + 1] = -sn * S.g[j];
+                S.g[j] = cc * S.g[j];
+                S.resid = fabs(S.g[j + 1]);
+                resid_out[slot] = S.resid;
+            }
+        }
+    }
+    __syncthreads();
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n || !in_slot(sl, slot, g)) return;
+    const double s = hn_s;
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    vnext[(size_t)slot * vs + g] = w[(size_t)slot * ws + g] * inv;
+}
+
+// The start of a cycle: beta = ||r|| summed from k_norm2's partials by every
+// block; block 0 of a slot sets g = beta e1 (k_gmres_start); every block of an
+// active slot writes its part of V_0 = r / beta (k_scale_by).
+__global__ __launch_bounds__(256) void k_gmres_start_scale(GState* st, const double* __restrict__ npartials, int nb,
+                                                           const int* active, const double* __restrict__ r,
+                                                           long long rs, double* __restrict__ v0, long long vs, int n,
+                                                           SlotLen sl) {
+    __shared__ double beta_s;
+    const int slot = blockIdx.y;
+    if (active && !active[slot]) return;
+    if (threadIdx.x < 64) {
+        const double s = wave_sum_partials(npartials + (size_t)slot * nb, nb);
+        if (threadIdx.x == 0) {
+            beta_s = sqrt(s);
+            if (blockIdx.x == 0) {
+                GState& S = st[slot];
+                S.beta = beta_s;
+                for (int i = 0; i < MAXM1; ++i) S.g[i] = 0.0;
+                S.g[0] = beta_s;
+                S.resid = beta_s;
+            }
+        }
+    }
+    __syncthreads();
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n || !in_slot(sl, slot, g)) return;
+    const double s = beta_s;
+    const double inv = s > 0.0 ? 1.0 / s : 0.0;
+    v0[(size_t)slot * vs + g] = r[(size_t)slot * rs + g] * inv;
+}
+
 static inline dim3 grid1(size_t n, int threads = 256) { return dim3((unsigned)((n + threads - 1) / threads)); }
 
 }  // namespace orbit
@@ -1024,6 +1207,7 @@ extern "C" int orbit_ctx_destroy(orbit_ctx* c) {
     for (auto& s : c->scratch)
         if (s.p) hipFree(s.p);
     for (void* p : c->owned) hipFree(p);
+    if (c->pinned) hipHostFree(c->pinned);
     hipStreamDestroy(c->stream);
     delete c;
     return ORBIT_OK;
@@ -1979,7 +2163,7 @@ struct Solver {
     double* resid = nullptr;
     GState* gst = nullptr;
     int* kcols = nullptr;
-    double* host = nullptr;   // pinned-free host staging (plain new[])
+    double* host = nullptr;   // the context's pinned host staging
     int nb = 1;
     SlotLen sl;               // per-slot lengths (group solves), else none
     const int* inner_cnt = nullptr;  // per-slot sweep counts (group solves), else the uniform ones
@@ -2118,9 +2302,10 @@ int mono_sweeps_f32(Solver& S, float** result) {
     return ORBIT_OK;
 }
 
-// w = v - M v with the monodromy used inside GMRES (inexact FP32 for mixed,
-// accurate FP64 for plain).
-int apply_I_minus_M(Solver& S, const double* v, long long vslot) {
+// M v with the monodromy used inside GMRES (inexact FP32 for mixed, accurate
+// FP64 for plain): *res receives the buffer holding it (FP32 when *f32), the
+// caller forms w = v - M v (in the first orthogonalisation pass).
+int apply_M(Solver& S, const double* v, long long vslot, const void** res_out, bool* f32) {
     hipStream_t st = S.B.ctx->stream;
     const size_t n16 = S.n16;
     const long long vs = (long long)n16;
@@ -2156,7 +2341,8 @@ int apply_I_minus_M(Solver& S, const double* v, long long vslot) {
             int rc = mono_sweeps_f32(S, &res);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL((k_sub_from<float>), g, dim3(256), 0, st, v, vslot, res, vs, S.w, vs, (int)n16, mask, S.sl);
+        *res_out = res;
+        *f32 = true;
     } else {
         const double* in = v;
         long long ins = vslot;
@@ -2172,7 +2358,8 @@ int apply_I_minus_M(Solver& S, const double* v, long long vslot) {
             ins = vs;
             which ^= 1;
         }
-        hipLaunchKernelGGL((k_sub_from<double>), g, dim3(256), 0, st, v, vslot, in, vs, S.w, vs, (int)n16, mask, S.sl);
+        *res_out = in;
+        *f32 = false;
     }
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
@@ -2193,31 +2380,35 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     { int rc0 = set_ints(st, S.B.active_dev, act.data(), ns); if (rc0) return rc0; }
     // beta = ||r||, V0 = r / beta
     hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.r, vs, (int)n16, nb, S.partials, S.sl);
-    hipLaunchKernelGGL(k_sum_partials, grid_waves(ns), dim3(256), 0, st, S.partials, nb, S.scal, ns);
-    hipLaunchKernelGGL(k_gmres_start, grid1(ns), dim3(256), 0, st, S.gst, S.scal, ns, S.B.active_dev);
-    hipLaunchKernelGGL(k_scale_by, g, dim3(256), 0, st, S.r, vs, S.Vk, kslot, S.gst, 0, (int)n16, S.B.active_dev, S.sl);
+    hipLaunchKernelGGL(k_gmres_start_scale, g, dim3(256), 0, st, S.gst, S.partials, nb, S.B.active_dev, S.r, vs, S.Vk,
+                       kslot, (int)n16, S.sl);
     steps.assign(ns, 0);
     std::vector<int> kc(ns, 0);
     for (int j = 0; j < S.m; ++j) {
-        int rc = apply_I_minus_M(S, S.Vk + (size_t)j * n16, kslot);
+        const void* res = nullptr;
+        bool f32 = false;
+        int rc = apply_M(S, S.Vk + (size_t)j * n16, kslot, &res, &f32);
         if (rc) return rc;
-        // CGS2
+        // w = v - M v with the first pass of CGS2, then the second pass; the
+        // coefficients are summed inside the updates, ||w|| and the Givens
+        // step inside the scaling of V_{j+1}
+        const double* vj = S.Vk + (size_t)j * n16;
+        if (f32)
+            hipLaunchKernelGGL((k_mdot_sub<float>), dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, vj,
+                               kslot, (const float*)res, vs, S.w, vs, (int)n16, nb, S.partials, S.B.active_dev, S.sl);
+        else
+            hipLaunchKernelGGL((k_mdot_sub<double>), dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, vj,
+                               kslot, (const double*)res, vs, S.w, vs, (int)n16, nb, S.partials, S.B.active_dev,
+                               S.sl);
+        hipLaunchKernelGGL(k_mupdate_fin, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.partials,
+                           S.h1, S.w, vs, (int)n16, nb, (double*)nullptr, S.B.active_dev, S.sl);
         hipLaunchKernelGGL(k_mdot, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.w, vs,
                            (int)n16, nb, S.partials, S.B.active_dev, S.sl);
-        hipLaunchKernelGGL(k_finalize, grid_waves((size_t)ns * (j + 1)), dim3(256), 0, st, S.partials, j + 1, nb, S.h1, ns);
-        hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.h1, 1.0,
-                           S.w, vs, (int)n16, nb, (double*)nullptr, S.B.active_dev, (const int*)nullptr, S.sl);
-        hipLaunchKernelGGL(k_mdot, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.w, vs,
-                           (int)n16, nb, S.partials, S.B.active_dev, S.sl);
-        hipLaunchKernelGGL(k_finalize, grid_waves((size_t)ns * (j + 1)), dim3(256), 0, st, S.partials, j + 1, nb, S.h2, ns);
         double* np = S.partials + (size_t)ns * MAXM1 * nb;
-        hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.h2, 1.0,
-                           S.w, vs, (int)n16, nb, np, S.B.active_dev, (const int*)nullptr, S.sl);
-        hipLaunchKernelGGL(k_sum_partials, grid_waves(ns), dim3(256), 0, st, np, nb, S.scal, ns);
-        hipLaunchKernelGGL(k_gmres_column, grid1(ns), dim3(256), 0, st, S.gst, S.h1, S.h2, S.scal, j, ns,
-                           S.B.active_dev, S.resid);
-        hipLaunchKernelGGL(k_scale_by, g, dim3(256), 0, st, S.w, vs, S.Vk + (size_t)(j + 1) * n16, kslot,
-                           S.gst, 1, (int)n16, S.B.active_dev, S.sl);
+        hipLaunchKernelGGL(k_mupdate_fin, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.partials,
+                           S.h2, S.w, vs, (int)n16, nb, np, S.B.active_dev, S.sl);
+        hipLaunchKernelGGL(k_arnoldi_tail, g, dim3(256), 0, st, S.gst, S.h1, S.h2, np, nb, j, S.B.active_dev,
+                           S.resid, S.w, vs, S.Vk + (size_t)(j + 1) * n16, kslot, (int)n16, S.sl);
         ORBIT_HIP_CHECK(hipGetLastError());
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.resid, ns * 8, hipMemcpyDeviceToHost, st));
         ORBIT_HIP_CHECK(hipStreamSynchronize(st));
@@ -2387,8 +2578,17 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         if (!rc0) rc0 = set_ints(ctx->stream, G->acc_cnt, acc_s.data(), nt);
         if (rc0) return rc0;
     }
-    std::vector<double> host(std::max(nt, 1) * 3 + 8);
-    S.host = host.data();
+    // host staging: the context's pinned buffer; an arena solve (no allocation
+    // of any kind) stages through ordinary host memory
+    std::vector<double> host_pageable;
+    if (arena) {
+        host_pageable.resize((size_t)std::max(nt, 1) * 3 + 8);
+        S.host = host_pageable.data();
+    } else {
+        const int rc0 = ctx->ensure_pinned((size_t)std::max(nt, 1) * 3 + 8);
+        if (rc0) return rc0;
+        S.host = ctx->pinned;
+    }
 
     const long long vs = (long long)n16;
     dim3 g(grid1(n16).x, nt);

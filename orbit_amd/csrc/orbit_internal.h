@@ -78,7 +78,12 @@ struct Ctx {
     // solve (its device address, looked up at context creation)
     void* fixed_scratch = nullptr;
     SweepTiming* timing = nullptr;  // non-null while kernel timing is on
+    // pinned host staging of the solves' small device-to-host reads (residuals,
+    // norms): a direct copy instead of one staged through a bounce buffer
+    double* pinned = nullptr;
+    size_t pinned_n = 0;
     int ensure(int slot, size_t bytes);
+    int ensure_pinned(size_t n);
     template <class T>
     T* buf(int slot) { return reinterpret_cast<T*>(scratch[slot].p); }
 };
