@@ -260,6 +260,15 @@ struct SweepArgs {
     // slot groups' operators and geometry, and per block its (group, tile)
     const struct GroupOp* gops;
     const int* btab;    // group << 16 | tile, -1 for a block with no work
+    // epilogue of a slot's last sweep (FP64 forced sweeps; epi_out null: none):
+    // epi_out = x + epi_alpha * epi_add at the colour's columns, the next bin's
+    // right-hand side or the outer residual, so no separate pass forms it
+    double* epi_out;
+    long long epi_os;
+    const double* epi_add;
+    long long epi_as;
+    double epi_alpha;
+    int nsweeps;        // sweeps of the inner solve (a slot's last is min(count, nsweeps) - 1)
 };
 
 // One slot group of a group launch: up to four consecutive slots of one
@@ -521,7 +530,12 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     CT dx = 0;
 #pragma unroll
     for (int m = 0; m < 16; ++m) dx += (CT)ldv(prow[m]) * sr[(grp << 4) + m];
-    if (on) x[base] = (XT)(xo + (CT)omega * dx);
+    if (on) {
+        const XT xn = (XT)(xo + (CT)omega * dx);
+        x[base] = xn;
+        if (a.epi_out && a.sweep_idx == (cnt < a.nsweeps ? cnt : a.nsweeps) - 1)
+            a.epi_out[(size_t)ls * a.epi_os + base] = (double)xn + a.epi_alpha * a.epi_add[(size_t)ls * a.epi_as + base];
+    }
 }
 
 // y = A x (one operator apply, FP64), used for residual checks of the inner solves.
@@ -1075,7 +1089,8 @@ __global__ __launch_bounds__(256) void k_arnoldi_tail(GState* st, const double* 
                                                       const double* __restrict__ npartials, int nb, int j,
                                                       const int* active, double* resid_out,
                                                       const double* __restrict__ w, long long ws,
-                                                      double* __restrict__ vnext, long long vs, int n, SlotLen sl) {
+                                                      double* __restrict__ vnext, long long vs, int n, SlotLen sl,
+                                                      float* __restrict__ fnext, long long fs) {
     __shared__ double hn_s;
     const int slot = blockIdx.y;
     const bool on = !active || active[slot];
@@ -1118,7 +1133,9 @@ __global__ __launch_bounds__(256) void k_arnoldi_tail(GState* st, const double* 
     if (g >= n || !in_slot(sl, slot, g)) return;
     const double s = hn_s;
     const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    vnext[(size_t)slot * vs + g] = w[(size_t)slot * ws + g] * inv;
+    const double val = w[(size_t)slot * ws + g] * inv;
+    vnext[(size_t)slot * vs + g] = val;
+    if (fnext) fnext[(size_t)slot * fs + g] = (float)val;  // the inexact monodromy's FP32 input
 }
 
 // The start of a cycle: beta = ||r|| summed from k_norm2's partials by every
@@ -1127,7 +1144,7 @@ __global__ __launch_bounds__(256) void k_arnoldi_tail(GState* st, const double* 
 __global__ __launch_bounds__(256) void k_gmres_start_scale(GState* st, const double* __restrict__ npartials, int nb,
                                                            const int* active, const double* __restrict__ r,
                                                            long long rs, double* __restrict__ v0, long long vs, int n,
-                                                           SlotLen sl) {
+                                                           SlotLen sl, float* __restrict__ f0, long long fs) {
     __shared__ double beta_s;
     const int slot = blockIdx.y;
     if (active && !active[slot]) return;
@@ -1149,7 +1166,9 @@ __global__ __launch_bounds__(256) void k_gmres_start_scale(GState* st, const dou
     if (g >= n || !in_slot(sl, slot, g)) return;
     const double s = beta_s;
     const double inv = s > 0.0 ? 1.0 / s : 0.0;
-    v0[(size_t)slot * vs + g] = r[(size_t)slot * rs + g] * inv;
+    const double val = r[(size_t)slot * rs + g] * inv;
+    v0[(size_t)slot * vs + g] = val;
+    if (f0) f0[(size_t)slot * fs + g] = (float)val;
 }
 
 static inline dim3 grid1(size_t n, int threads = 256) { return dim3((unsigned)((n + threads - 1) / threads)); }
@@ -1611,6 +1630,27 @@ extern "C" int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const do
 // solver
 // ------------------------------------------------------------------------- //
 namespace orbit {
+// The start of a forward forced sweep: orbit[0] = c (or 0, every slot) and the
+// first bin's right-hand side ta = orbit[0] + E[0] (k_zero's and k_axpby's
+// expressions; c and ta for the slots the mask keeps).
+__global__ void k_forced_start(const double* __restrict__ c, long long cs, int zero_start,
+                               const double* __restrict__ E, long long es, double* __restrict__ o0, long long os,
+                               double* __restrict__ ta, long long ts, int n, const int* active, SlotLen sl) {
+    const int slot = blockIdx.y;
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n || !in_slot(sl, slot, g)) return;
+    const bool keep = !active || active[slot];
+    double v = 0.0;
+    if (zero_start) {
+        o0[(size_t)slot * os + g] = 0.0;
+    } else if (keep) {
+        const double cv = c[(size_t)slot * cs + g];
+        v = cv + 0.0 * cv;
+        o0[(size_t)slot * os + g] = v;
+    }
+    if (keep) ta[(size_t)slot * ts + g] = v + 1.0 * E[(size_t)slot * es + g];
+}
+
 __global__ void k_zero(double* __restrict__ x, long long xslot, int n, SlotLen sl) {
     const int slot = blockIdx.y;
     int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1847,9 +1887,28 @@ int launch_group_kernel(const Batch& B, int bin, int prec, SweepArgs a, bool own
     return ORBIT_OK;
 }
 
+// Epilogue of an inner solve's last sweep (see SweepArgs::epi_out).
+struct Epi {
+    double* out = nullptr;
+    long long os = 0;
+    const double* add = nullptr;
+    long long as = 0;
+    double alpha = 0.0;
+};
+
+void set_epi(SweepArgs& a, const Epi* epi, int nsweeps) {
+    a.epi_out = epi ? epi->out : nullptr;
+    a.epi_os = epi ? epi->os : 0;
+    a.epi_add = epi ? epi->add : nullptr;
+    a.epi_as = epi ? epi->as : 0;
+    a.epi_alpha = epi ? epi->alpha : 0.0;
+    a.nsweeps = nsweeps;
+}
+
 int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, const void* rhs,
                      long long bstride, void* x, long long xstride, bool own_zero, bool nbr_zero,
-                     const int* mask, const int* counts = nullptr, int sweep_idx = 0) {
+                     const int* mask, const int* counts = nullptr, int sweep_idx = 0, const Epi* epi = nullptr,
+                     int nsweeps = 0) {
     if (B.grp) {  // operators per slot group; the direction is the tables'
         SweepArgs a{};
         a.b = rhs;
@@ -1864,6 +1923,7 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
         a.sweep_idx = sweep_idx;
         a.colour = colour;
         a.nslots = B.nslots;
+        set_epi(a, epi, nsweeps);
         if (prec == SP_F64) return launch_group_kernel<double, double, double, double>(B, bin, prec, a, own_zero, nbr_zero);
         if (prec == SP_ACC_MIXED)
             return launch_group_kernel<double, double, uint16_t, double>(B, bin, prec, a, own_zero, nbr_zero);
@@ -1895,6 +1955,9 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     a.sweep_idx = sweep_idx;
     a.colour = colour;
     a.nslots = B.nslots;
+    a.gops = nullptr;
+    a.btab = nullptr;
+    set_epi(a, epi, nsweeps);
     const bool T = dir == ADJ;
     // launch shape of this precision and slot count (the four-slot mapping reads
     // one shared column block, so it needs a single transport group); the
@@ -1927,7 +1990,8 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
 // yet written in this sweep, read zeros there; slots the mask leaves out are
 // not touched.
 int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long long bstride, void* x,
-               long long xstride, bool first, const int* mask, const int* counts, int sweep_idx) {
+               long long xstride, bool first, const int* mask, const int* counts, int sweep_idx,
+               const Epi* epi = nullptr, int nsweeps = 0) {
     orbit_opset* op = B.op;
     const int ncolours = B.grp ? 2 : op->ncolours;
     if (!B.grp && first && op->ncolours == 4) {
@@ -1941,7 +2005,7 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
     }
     for (int c = 0; c < ncolours; ++c) {
         const int rc = launch_halfsweep(B, bin, dir, prec, c, rhs, bstride, x, xstride, first, first && c == 0, mask,
-                                        counts, sweep_idx);
+                                        counts, sweep_idx, epi, nsweeps);
         if (rc) return rc;
     }
     return ORBIT_OK;
@@ -1949,9 +2013,10 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
 
 int inner_solve(const Batch& B, int bin, int dir, int prec, int sweeps, const void* rhs,
                 long long bstride, void* x, long long xstride, const int* mask, bool warm = false,
-                const int* counts = nullptr) {
+                const int* counts = nullptr, const Epi* epi = nullptr) {
     for (int s = 0; s < sweeps; ++s) {
-        const int rc = sweep_once(B, bin, dir, prec, rhs, bstride, x, xstride, s == 0 && !warm, mask, counts, s);
+        const int rc = sweep_once(B, bin, dir, prec, rhs, bstride, x, xstride, s == 0 && !warm, mask, counts, s, epi,
+                                  sweeps);
         if (rc) return rc;
     }
     return ORBIT_OK;
@@ -2204,19 +2269,31 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
     dim3 g(grid1(n16).x, S.B.nslots);
     const int* mask = S.B.part_dev;
     if (S.dir == FWD) {
-        if (zero_start)
-            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.orbit, os, (int)n16, S.sl);
-        else
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, 0.0, S.orbit, os, (int)n16, mask, S.sl);
+        // orbit[0] and the first right-hand side in one pass; every later
+        // right-hand side ta = orbit[t] + E[t], and at the end r = orbit[nb] -
+        // orbit[0], are written by the previous bin's last sweep (its epilogue)
+        // as each slot's final x comes out (every slot runs at least one sweep)
+        hipLaunchKernelGGL(k_forced_start, g, dim3(256), 0, st, S.c, vs, zero_start ? 1 : 0, S.E, es, S.orbit, os,
+                           S.ta, vs, (int)n16, mask, S.sl);
         for (int t = 0; t < nbn; ++t) {
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)t * n16, os,
-                               S.E + (size_t)t * n16, es, 1.0, S.ta, vs, (int)n16, mask, S.sl);
+            Epi epi;
+            if (t + 1 < nbn) {
+                epi.out = S.ta;
+                epi.os = vs;
+                epi.add = S.E + (size_t)(t + 1) * n16;
+                epi.as = es;
+                epi.alpha = 1.0;
+            } else {
+                epi.out = S.r;
+                epi.os = vs;
+                epi.add = S.orbit;
+                epi.as = os;
+                epi.alpha = -1.0;
+            }
             int rc = inner_solve(S.B, t, FWD, sp, sweeps, S.ta, vs, S.orbit + (size_t)(t + 1) * n16, os, mask, warm,
-                                 warm ? S.nsw_dev : S.acc_cnt);
+                                 warm ? S.nsw_dev : S.acc_cnt, &epi);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)nbn * n16, os, S.orbit, os,
-                           -1.0, S.r, vs, (int)n16, mask, S.sl);
         if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask, S.sl);
     } else {
         if (zero_start)
@@ -2305,14 +2382,16 @@ int mono_sweeps_f32(Solver& S, float** result) {
 // M v with the monodromy used inside GMRES (inexact FP32 for mixed, accurate
 // FP64 for plain): *res receives the buffer holding it (FP32 when *f32), the
 // caller forms w = v - M v (in the first orthogonalisation pass).
-int apply_M(Solver& S, const double* v, long long vslot, const void** res_out, bool* f32) {
+int apply_M(Solver& S, const double* v, long long vslot, const void** res_out, bool* f32, bool v_in_fa) {
     hipStream_t st = S.B.ctx->stream;
     const size_t n16 = S.n16;
     const long long vs = (long long)n16;
     dim3 g(grid1(n16).x, S.B.nslots);
     const int* mask = S.B.active_dev;
     if (S.prec == ORBIT_PREC_MIXED) {
-        hipLaunchKernelGGL((k_convert_slots<double, float>), g, dim3(256), 0, st, v, vslot, S.fa, vs, (int)n16, mask, S.sl);
+        if (!v_in_fa)
+            hipLaunchKernelGGL((k_convert_slots<double, float>), g, dim3(256), 0, st, v, vslot, S.fa, vs, (int)n16, mask,
+                               S.sl);
         float* res = nullptr;
         if (S.use_graph) {
             GraphKey key{S.dir, 2, S.inner, S.B.nslots, {}, S.fa, S.fb};
@@ -2380,14 +2459,17 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     { int rc0 = set_ints(st, S.B.active_dev, act.data(), ns); if (rc0) return rc0; }
     // beta = ||r||, V0 = r / beta
     hipLaunchKernelGGL(k_norm2, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.r, vs, (int)n16, nb, S.partials, S.sl);
+    // the mixed path's monodromy reads its input in FP32 from fa: the kernels
+    // that write a basis vector also write that copy
+    float* fcopy = S.prec == ORBIT_PREC_MIXED ? S.fa : nullptr;
     hipLaunchKernelGGL(k_gmres_start_scale, g, dim3(256), 0, st, S.gst, S.partials, nb, S.B.active_dev, S.r, vs, S.Vk,
-                       kslot, (int)n16, S.sl);
+                       kslot, (int)n16, S.sl, fcopy, vs);
     steps.assign(ns, 0);
     std::vector<int> kc(ns, 0);
     for (int j = 0; j < S.m; ++j) {
         const void* res = nullptr;
         bool f32 = false;
-        int rc = apply_M(S, S.Vk + (size_t)j * n16, kslot, &res, &f32);
+        int rc = apply_M(S, S.Vk + (size_t)j * n16, kslot, &res, &f32, fcopy != nullptr);
         if (rc) return rc;
         // w = v - M v with the first pass of CGS2, then the second pass; the
         // coefficients are summed inside the updates, ||w|| and the Givens
@@ -2408,7 +2490,7 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
         hipLaunchKernelGGL(k_mupdate_fin, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.partials,
                            S.h2, S.w, vs, (int)n16, nb, np, S.B.active_dev, S.sl);
         hipLaunchKernelGGL(k_arnoldi_tail, g, dim3(256), 0, st, S.gst, S.h1, S.h2, np, nb, j, S.B.active_dev,
-                           S.resid, S.w, vs, S.Vk + (size_t)(j + 1) * n16, kslot, (int)n16, S.sl);
+                           S.resid, S.w, vs, S.Vk + (size_t)(j + 1) * n16, kslot, (int)n16, S.sl, fcopy, vs);
         ORBIT_HIP_CHECK(hipGetLastError());
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.resid, ns * 8, hipMemcpyDeviceToHost, st));
         ORBIT_HIP_CHECK(hipStreamSynchronize(st));
@@ -2735,8 +2817,15 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         int rc0 = set_ints(st, S.nsw_dev, nsw.data(), nt);
         if (rc0) return rc0;
     }
+    // the part mask on the device, uploaded only when it changes
+    std::vector<int> part_up;
+    auto upload_part = [&]() {
+        if (part_up == part) return ORBIT_OK;
+        part_up = part;
+        return set_ints(st, S.B.part_dev, part.data(), nt);
+    };
     while (true) {
-        { int rc0 = set_ints(st, S.B.part_dev, part.data(), nt); if (rc0) return rc0; }
+        { int rc0 = upload_part(); if (rc0) return rc0; }
         if (need_check) {
             const bool warm = have_orbit && S.prec == ORBIT_PREC_MIXED;
             rc = forced_sweep(S, false, warm ? next_sweeps : S.acc, warm);
@@ -2773,7 +2862,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                 target[s] = o.tol * snorm[s];
         }
         on_s_tilde = false;
-        { int rc0 = set_ints(st, S.B.part_dev, part.data(), nt); if (rc0) return rc0; }
+        { int rc0 = upload_part(); if (rc0) return rc0; }
         std::vector<int> steps;
         rc = gmres_cycle(S, part, target, steps);
         if (rc) return rc;
