@@ -260,10 +260,11 @@ struct SweepArgs {
     // slot groups' operators and geometry, and per block its (group, tile)
     const struct GroupOp* gops;
     const int* btab;    // group << 16 | tile, -1 for a block with no work
-    // epilogue of a slot's last sweep (FP64 forced sweeps; epi_out null: none):
-    // epi_out = x + epi_alpha * epi_add at the colour's columns, the next bin's
-    // right-hand side or the outer residual, so no separate pass forms it
-    double* epi_out;
+    // epilogue of a slot's last sweep (epi_out null: none): at the colour's
+    // columns epi_out = x + epi_alpha * epi_add (FP64 sweeps) or x + (float)
+    // epi_add (FP32 sweeps), the next bin's right-hand side or the outer
+    // residual, so no separate pass forms it
+    void* epi_out;
     long long epi_os;
     const double* epi_add;
     long long epi_as;
@@ -533,8 +534,13 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     if (on) {
         const XT xn = (XT)(xo + (CT)omega * dx);
         x[base] = xn;
-        if (a.epi_out && a.sweep_idx == (cnt < a.nsweeps ? cnt : a.nsweeps) - 1)
-            a.epi_out[(size_t)ls * a.epi_os + base] = (double)xn + a.epi_alpha * a.epi_add[(size_t)ls * a.epi_as + base];
+        if (a.epi_out && a.sweep_idx == (cnt < a.nsweeps ? cnt : a.nsweeps) - 1) {
+            const double ad = a.epi_add[(size_t)ls * a.epi_as + base];
+            if constexpr (sizeof(XT) == 4)
+                ((float*)a.epi_out)[(size_t)ls * a.epi_os + base] = (float)xn + (float)ad;
+            else
+                ((double*)a.epi_out)[(size_t)ls * a.epi_os + base] = (double)xn + a.epi_alpha * ad;
+        }
     }
 }
 
@@ -1889,7 +1895,7 @@ int launch_group_kernel(const Batch& B, int bin, int prec, SweepArgs a, bool own
 
 // Epilogue of an inner solve's last sweep (see SweepArgs::epi_out).
 struct Epi {
-    double* out = nullptr;
+    void* out = nullptr;
     long long os = 0;
     const double* add = nullptr;
     long long as = 0;
@@ -2334,10 +2340,18 @@ int forced_sweep_f32(Solver& S, double* acc = nullptr, double* ymax2 = nullptr) 
     float* y = S.fb;
     float* rhs = S.fc;
     if (S.dir == FWD) {
+        // the first right-hand side here, every later one (rhs = y + e_t) by the
+        // previous bin's last sweep, in place (a lane reads its rhs before it
+        // writes the next one, and a slot's later sweeps do not store)
+        hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.E, es, rhs, vs, (int)n16, mask, S.sl);
         for (int t = 0; t < nbn; ++t) {
-            hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, t == 0 ? nullptr : x, vs, S.E + (size_t)t * n16, es,
-                               rhs, vs, (int)n16, mask, S.sl);
-            int rc = inner_solve(S.B, t, FWD, SP_F32, S.inner, rhs, vs, y, vs, mask, false, S.inner_cnt);
+            Epi epi;
+            epi.out = rhs;
+            epi.os = vs;
+            epi.add = S.E + (size_t)(t + 1) * n16;
+            epi.as = es;
+            int rc = inner_solve(S.B, t, FWD, SP_F32, S.inner, rhs, vs, y, vs, mask, false, S.inner_cnt,
+                                 t + 1 < nbn ? &epi : nullptr);
             if (rc) return rc;
             if (acc)
                 hipLaunchKernelGGL(k_acc_f32_norm, dim3(S.nb, S.B.nslots), dim3(RED_THREADS), 0, st, y, vs,
@@ -2353,9 +2367,14 @@ int forced_sweep_f32(Solver& S, double* acc = nullptr, double* ymax2 = nullptr) 
         hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.E + (size_t)(nbn - 1) * n16, es, x, vs,
                            (int)n16, mask, S.sl);
         for (int t = nbn - 2; t >= 0; --t) {
-            int rc = inner_solve(S.B, t, ADJ, SP_F32, S.inner, x, vs, y, vs, mask, false, S.inner_cnt);
+            // x = P_t^T x + R_t: the sum written by the solve's last sweep, in place
+            Epi epi;
+            epi.out = x;
+            epi.os = vs;
+            epi.add = S.E + (size_t)t * n16;
+            epi.as = es;
+            int rc = inner_solve(S.B, t, ADJ, SP_F32, S.inner, x, vs, y, vs, mask, false, S.inner_cnt, &epi);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, y, vs, S.E + (size_t)t * n16, es, x, vs, (int)n16, mask, S.sl);
         }
     }
     hipLaunchKernelGGL((k_convert_slots<float, double>), g, dim3(256), 0, st, x, vs, S.r, vs, (int)n16, mask, S.sl);
