@@ -354,12 +354,14 @@ __device__ __forceinline__ void fct_tile(const FctGrid& g, const FctGeo& geo, in
                                          const int* __restrict__ gmask, double dt, int limited,
                                          double* __restrict__ d, long long ds,
                                          const double* __restrict__ base, long long bs) {
+    // per-tracer values in LDS; the faces' terrain weights and validity are the
+    // bin's precomputed geometry, read where used (L2-resident, the same for
+    // every tracer): less LDS per block, more blocks per CU
     __shared__ double sx[FX_H][FX_W];
-    __shared__ double xfA[XF_H][XF_W], xfL[XF_H][XF_W], xfH[XF_H][XF_W];
-    __shared__ double yfA[YF_H][YF_W], yfL[YF_H][YF_W], yfH[YF_H][YF_W];
+    __shared__ double xfA[XF_H][XF_W];
+    __shared__ double yfA[YF_H][YF_W];
     __shared__ double sRp[RR_H][RR_W], sRm[RR_H][RR_W];
     __shared__ double sinvx[XF_H];
-    __shared__ unsigned char xfV[XF_H][XF_W], yfV[YF_H][YF_W];
     const int nx = g.nx, ny = g.ny;
     const int i0 = blockIdx.x * FT_I, j0 = blockIdx.y * FT_J;
     const size_t lvl = (size_t)k * ny * nx;
@@ -392,11 +394,7 @@ __device__ __forceinline__ void fct_tile(const FctGrid& g, const FctGeo& geo, in
             if (j >= 0 && j < ny && i >= 0 && i <= nx - 2) {
                 double beta, A;
                 face_flux_x(g, xat, gat, has_g, k, j, i, beta, A);
-                const double* q = geo.gx + 3 * (((size_t)k * ny + j) * (nx - 1) + i);
                 xfA[fj][fi] = A;
-                xfL[fj][fi] = q[0];
-                xfH[fj][fi] = q[1];
-                xfV[fj][fi] = q[2] != 0.0;
             }
         }
     if (ny > 1)
@@ -406,11 +404,7 @@ __device__ __forceinline__ void fct_tile(const FctGrid& g, const FctGeo& geo, in
             if (i >= 0 && i < nx && j >= 0 && j <= ny - 2) {
                 double beta, A;
                 face_flux_y(g, xat, gat, has_g, k, j, i, beta, A);
-                const double* q = geo.gy + 3 * (((size_t)k * (ny - 1) + j) * nx + i);
                 yfA[fj][fi] = A;
-                yfL[fj][fi] = q[0];
-                yfH[fj][fi] = q[1];
-                yfV[fj][fi] = q[2] != 0.0;
             }
         }
     __syncthreads();
@@ -418,23 +412,25 @@ __device__ __forceinline__ void fct_tile(const FctGrid& g, const FctGeo& geo, in
     // faces with lo cell (j, i), rebuilt from LDS (beta is not needed past A)
     auto fx = [&](int j, int i) {
         const int fj = j - j0 + 1, fi = i - i0 + 2;
+        const double* q = geo.gx + 3 * (((size_t)k * ny + j) * (nx - 1) + i);
         Face f;
         f.beta = 0.0;
         f.A = xfA[fj][fi];
-        f.tr_lo = xfL[fj][fi];
-        f.tr_hi = xfH[fj][fi];
-        f.valid = xfV[fj][fi] != 0;
+        f.tr_lo = q[0];
+        f.tr_hi = q[1];
+        f.valid = q[2] != 0.0;
         f.inv = sinvx[fj];
         return f;
     };
     auto fy = [&](int j, int i) {
         const int fj = j - j0 + 2, fi = i - i0 + 1;
+        const double* q = geo.gy + 3 * (((size_t)k * (ny - 1) + j) * nx + i);
         Face f;
         f.beta = 0.0;
         f.A = yfA[fj][fi];
-        f.tr_lo = yfL[fj][fi];
-        f.tr_hi = yfH[fj][fi];
-        f.valid = yfV[fj][fi] != 0;
+        f.tr_lo = q[0];
+        f.tr_hi = q[1];
+        f.valid = q[2] != 0.0;
         f.inv = invy;
         return f;
     };
