@@ -2311,11 +2311,16 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
             if (warm)  // previous P_t^T output = orbit[t] - R_t
                 hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)t * n16, os, S.E + (size_t)t * n16,
                                    es, -1.0, S.ta, vs, (int)n16, mask, S.sl);
+            // orbit[t] = P_t^T orbit[t+1] + R_t, written by the solve's last sweep
+            Epi epi;
+            epi.out = S.orbit + (size_t)t * n16;
+            epi.os = os;
+            epi.add = S.E + (size_t)t * n16;
+            epi.as = es;
+            epi.alpha = 1.0;
             int rc = inner_solve(S.B, t, ADJ, sp, sweeps, S.orbit + (size_t)(t + 1) * n16, os, S.ta, vs, mask, warm,
-                                 warm ? S.nsw_dev : S.acc_cnt);
+                                 warm ? S.nsw_dev : S.acc_cnt, &epi);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.ta, vs, S.E + (size_t)t * n16, es, 1.0,
-                               S.orbit + (size_t)t * n16, os, (int)n16, mask, S.sl);
         }
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os, S.orbit + (size_t)nbn * n16, os,
                            -1.0, S.r, vs, (int)n16, mask, S.sl);
