@@ -487,7 +487,8 @@ def main():
         xt = {_lib.ORBIT_PREC_FP64: "double,double,double,double", _lib.ORBIT_SWEEP_ACC_MIXED:
               "double,double,unsigned short,double", _lib.ORBIT_PREC_MIXED: "float,float,unsigned short,float"}
         return (f"k_halfsweep<{xt[v.precision]},{1 if v.slots_per_wave == 4 else 0},"
-                f"{'true' if v.own_zero else 'false'},{'true' if v.nbr_zero else 'false'}>")
+                f"{'true' if v.own_zero else 'false'},{'true' if v.nbr_zero else 'false'},"
+                f"{'true' if v.mfma else 'false'},false>")
 
     def roof(v):
         us = 1e3 * v.total_ms / v.launches
