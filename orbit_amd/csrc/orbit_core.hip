@@ -2266,7 +2266,7 @@ void norms_dev(Solver& S, const double* x, long long xslot, int n, double* dst) 
 // Adjoint: orbit[nb] = c (or 0), orbit[t] = P_t^T orbit[t+1] + R_t for t = nb-1..0.
 // Either way the outer residual s - (I - M)c is the last state minus the first.
 // warm: start every inner solve from the previous sweep's state (same slots).
-int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
+int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm, const int* counts_in = nullptr) {
     hipStream_t st = S.B.ctx->stream;
     const int nbn = S.nbins;
     const size_t n16 = S.n16;
@@ -2297,7 +2297,7 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
                 epi.alpha = -1.0;
             }
             int rc = inner_solve(S.B, t, FWD, sp, sweeps, S.ta, vs, S.orbit + (size_t)(t + 1) * n16, os, mask, warm,
-                                 warm ? S.nsw_dev : S.acc_cnt, &epi);
+                                 counts_in ? counts_in : warm ? S.nsw_dev : S.acc_cnt, &epi);
             if (rc) return rc;
         }
         if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask, S.sl);
@@ -2319,7 +2319,7 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm) {
             epi.as = es;
             epi.alpha = 1.0;
             int rc = inner_solve(S.B, t, ADJ, sp, sweeps, S.orbit + (size_t)(t + 1) * n16, os, S.ta, vs, mask, warm,
-                                 warm ? S.nsw_dev : S.acc_cnt, &epi);
+                                 counts_in ? counts_in : warm ? S.nsw_dev : S.acc_cnt, &epi);
             if (rc) return rc;
         }
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os, S.orbit + (size_t)nbn * n16, os,
@@ -2720,6 +2720,22 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     ORBIT_HIP_CHECK(hipGetLastError());
 
     std::vector<double> snorm;
+    // Accuracy of the accurate sweeps.  The FP64 path keeps the fixed target
+    // S.eps_acc.  A mixed check needs its propagation error (at most nbins times
+    // the per-bin error: the propagators contract, and the states are no larger
+    // than ||c|| + ||s||) small against tol * ||s||: eps = 2e-3 tol ||s|| /
+    // (||c|| + ||s||) per slot keeps the reported residual within 1.6 % of tol of
+    // the true one, tighter than a fixed target where ||c|| >> ||s|| (long-lived
+    // tracers), looser where the orbit is short-lived.
+    auto eps_for = [&](int s, double cnorm) {
+        if (S.prec != ORBIT_PREC_MIXED) return S.eps_acc;
+        const double e = 2e-3 * o.tol * snorm[s] / (cnorm + snorm[s] + 1e-300);
+        return std::min(1e-6, std::max(1e-15, e));
+    };
+    auto cold_sweeps = [&](int s, double eps) {  // from zero to eps
+        return o.acc_sweeps > 0 ? o.acc_sweeps
+                                : std::min(80, std::max(4, (int)std::ceil(std::log(eps) / std::log(rho_s[s]))));
+    };
     const int max_cycles = o.max_rounds > 0 ? o.max_rounds : o.maxiter;
     bool need_check = true;
     int warm_first = 0;  // > 0: the first check starts from a caller-given orbit
@@ -2778,10 +2794,12 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             first_nsw.assign(nt, S.acc);
             for (int s = 0; s < nt; ++s) {
                 if (!(cn[s] > 0.0)) continue;
-                const double rel = S.eps_acc + nbn * eps_in * dnorm[s] / cn[s];
-                int k = S.acc;
-                if (rel < 1.0) k = (int)std::ceil(std::log(S.eps_acc / rel) / std::log(S.rho));
-                first_nsw[s] = std::min(S.acc, std::max(2, k));
+                const double eps = eps_for(s, cn[s]);
+                const int cap = cold_sweeps(s, eps);
+                const double rel = eps + nbn * eps_in * dnorm[s] / cn[s];
+                int k = cap;
+                if (rel < 1.0) k = (int)std::ceil(std::log(eps / rel) / std::log(S.rho));
+                first_nsw[s] = std::min(cap, std::max(2, k));
                 warm_first = std::max(warm_first, first_nsw[s]);
             }
         } else if (warm_dev && o.warm_orbit) {
@@ -2852,7 +2870,23 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         { int rc0 = upload_part(); if (rc0) return rc0; }
         if (need_check) {
             const bool warm = have_orbit && S.prec == ORBIT_PREC_MIXED;
-            rc = forced_sweep(S, false, warm ? next_sweeps : S.acc, warm);
+            if (!warm && S.prec == ORBIT_PREC_MIXED) {
+                // a cold mixed check: per-slot sweeps from zero to eps_for(||c||)
+                std::vector<double> cn;
+                rc = norms(S, S.c, vs, (int)n16, cn);
+                if (rc) return rc;
+                std::vector<int> kc(nt, 4);
+                int kmax = 2;
+                for (int s = 0; s < nt; ++s) {
+                    if (!part[s]) continue;
+                    kc[s] = cold_sweeps(s, eps_for(s, cn[s]));
+                    kmax = std::max(kmax, kc[s]);
+                }
+                { int rc0 = set_ints(st, S.nsw_dev, kc.data(), nt); if (rc0) return rc0; }
+                rc = forced_sweep(S, false, kmax, false, S.nsw_dev);
+            } else {
+                rc = forced_sweep(S, false, warm ? next_sweeps : S.acc, warm);
+            }
             if (rc) return rc;
             have_orbit = true;
             rc = norms(S, S.r, vs, (int)n16, rnorm);
@@ -2913,10 +2947,11 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                 if (!part[s]) continue;
                 const double cn = std::sqrt(S.host[s]), yn = S.host[nt + s];
                 const double rel = cn > 0.0 ? yn / cn : 1.0;
-                int k = acc_s[s];
-                if (rel > 0.0 && rel < 1.0)
-                    k = (int)std::ceil(std::log(S.eps_acc / rel) / std::log(rho_s[s]));
-                nsw[s] = std::min(acc_s[s], std::max(2, k));
+                const double eps = eps_for(s, cn);
+                const int cap = cold_sweeps(s, eps);
+                int k = cap;
+                if (rel > 0.0 && rel < 1.0) k = (int)std::ceil(std::log(eps / rel) / std::log(rho_s[s]));
+                nsw[s] = std::min(cap, std::max(2, k));
                 next_sweeps = std::max(next_sweeps, nsw[s]);
             }
             { int rc0 = set_ints(st, S.nsw_dev, nsw.data(), nt); if (rc0) return rc0; }
