@@ -78,3 +78,42 @@ def test_mixed_fct_pass_residual_is_the_true_one(month, n_passes, tol):
         info = hist[-1][s]
         assert info["converged"] and true <= 1.05 * tol, (n_passes, s, true)
         assert abs(info["residual"] - true) <= 0.05 * tol, (n_passes, s, info["residual"], true)
+
+
+@pytest.fixture(scope="module")
+def long_lived():
+    """Tracer 0 with its loss and deposition cut 1000-fold, on a periodic
+    longitude with weak winds (little outflow): a long-lived orbit, ||c||
+    about 5 ||s||, where the mixed checks' accuracy target (2e-3 tol ||s|| /
+    (||c|| + ||s||)) is tighter than for short-lived ones."""
+    wl = synth.make_workload(12, 10, 6, 2, seed=99, convection=True, wind_std=0.5, periodic=True)
+    wl.D[:, 0] *= 1e-3
+    op = OperatorSet.from_workload(wl)
+    lus = [[orb.factor_one_bin(wl.L(t, s)) for t in range(8)] for s in range(wl.n_tracers)]
+    return wl, op, lus
+
+
+@pytest.mark.parametrize("direction", ["forward", "adjoint"])
+def test_long_lived_residual_is_the_true_one(long_lived, direction):
+    wl, op, lus = long_lived
+    tol = 1e-8
+    if direction == "forward":
+        rhs = wl.e
+        out, info = op.solve_forward(rhs, tol=tol, precision="mixed")
+    else:
+        rhs = np.random.default_rng(100).random((wl.n_tracers, 8, wl.N))
+        out, info = op.solve_adjoint(rhs, tol=tol, precision="mixed")
+    ratios = []
+    for s in range(wl.n_tracers):
+        if direction == "forward":
+            sv = orb.compute_s(lus[s], list(rhs[s]))
+            x0 = out[s, 0]
+            true = np.linalg.norm(sv - (x0 - orb.apply_monodromy(x0, lus[s]))) / np.linalg.norm(sv)
+        else:
+            sv = orb.compute_s_adjoint(lus[s], list(rhs[s]))
+            x0 = out[s, 0]
+            true = np.linalg.norm(sv - (x0 - orb.apply_monodromy_adjoint(x0, lus[s]))) / np.linalg.norm(sv)
+        ratios.append(np.linalg.norm(sv) / np.linalg.norm(x0))
+        assert info[s]["converged"] and true <= 1.05 * tol, (direction, s, true)
+        assert abs(info[s]["residual"] - true) <= 0.05 * tol, (direction, s, info[s]["residual"], true)
+    assert ratios[0] < 0.5 * ratios[1], ratios  # tracer 0 is the long-lived one
