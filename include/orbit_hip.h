@@ -328,6 +328,25 @@ typedef struct {
 int orbit_species_diagonals(orbit_ctx* ctx, int nz, int ny, int nx, const double* Dz,
                             const orbit_species_fields* fields, double* d_out, double* k_out);
 
+/* ISORROPIA cross-coupling blocks of the iso-coupled marginal mode (replaces
+ * assemble_iso_cross_blocks, orbit/modes/iso_coupling.py:101-172, with its
+ * _delta_k_nh / _delta_k_no3 at :58-98): for n_bins bins (every array
+ * advances by `stride` doubles from one bin to the next; device pointers on
+ * the context's stream; (nz,ny,nx) k-major, the dry velocities and Dz read
+ * on level 0) the four diagonal K blocks, out[0] TotalNH <- TotalNO3 =
+ * dk_NH * f_nh_dno3, out[1] TotalNH <- pSO4 = dk_NH * f_nh_dso4, out[2]
+ * TotalNO3 <- TotalNH = dk_NO3 * f_no3_dnh, out[3] TotalNO3 <- pSO4 = dk_NO3
+ * * f_no3_dso4, with partials = {f_nh_dno3, f_no3_dnh, f_nh_dso4, f_no3_dso4}
+ * and dk = (particle - gas) wet rate + (particle - gas) dry velocity / Dz at
+ * the surface (gas: NH3 for TotalNH, HNO3 for TotalNO3, 3 cm/s when
+ * HNO3_dry_dep is NULL).  A block whose bit in block_mask is clear is zero
+ * (the reference's ORBIT_ISO_DISABLE_BLOCKS diagnostic). */
+int orbit_iso_cross_blocks_dev(orbit_ctx* ctx, int n_bins, int nz, int ny, int nx, long long stride,
+                               const double* Dz, const double* particle_dry_dep, const double* NH3_dry_dep,
+                               const double* HNO3_dry_dep, const double* particle_wet_dep,
+                               const double* other_gas_wet_dep, const double* const* partials,
+                               unsigned block_mask, double* const* out);
+
 /* Regular-grid multilinear lookup table on the device: the 7-D ISORROPIA
  * partitioning table of the closure (reference IsorropiaLUT.query,
  * orbit/core/isorropia_lut.py:123-147, i.e. scipy RegularGridInterpolator

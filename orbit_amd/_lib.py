@@ -38,7 +38,7 @@ EXPORTS = [
     "orbit_opset_create_grids", "orbit_lut_create", "orbit_lut_query_dev", "orbit_lut_destroy",
     "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation", "orbit_forward_fct_dev",
     "orbit_solve_forward_dev_noalloc", "orbit_timing_start", "orbit_timing_stop",
-    "orbit_forward_fct_passes_dev", "orbit_lut_partitioning_dev", "orbit_opset_set_mfma",
+    "orbit_forward_fct_passes_dev", "orbit_lut_partitioning_dev", "orbit_iso_cross_blocks_dev", "orbit_opset_set_mfma",
     "orbit_solve_group_dev",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
@@ -134,6 +134,8 @@ def signatures():
         "orbit_solve_adjoint": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_solve_forward_dev": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
         "orbit_solve_forward_dev_noalloc": (_I, [_P, _P, _P, _I, _P, _P, C.POINTER(Opts), _P, _P]),
+        "orbit_iso_cross_blocks_dev": (_I, [_P, _I, _I, _I, _I, C.c_longlong, _P, _P, _P, _P, _P, _P, _P, C.c_uint,
+                                           _P]),
         "orbit_lut_partitioning_dev": (_I, [_P, _P, _I, _I, _I, C.c_longlong, _P, C.c_longlong, _P, C.c_longlong,
                                             _I, C.c_double, _P, _P]),
         "orbit_opset_set_mfma": (_I, [_P, _I]),

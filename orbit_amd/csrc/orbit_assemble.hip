@@ -699,6 +699,87 @@ __global__ void k_species_diag(orbit_species_fields f, int nz, int ny, int nx, c
 
 }  // namespace
 
+namespace {
+// ISORROPIA cross-coupling diagonals of the iso-coupled marginal mode
+// (reference orbit/modes/iso_coupling.py:58-172): per cell the particle-minus-
+// gas deposition contrast of TotalNH (wet: particle - other gas; dry at the
+// surface: (particle - NH3) / Dz) and of TotalNO3 (the same wet contrast; dry:
+// (particle - HNO3) / Dz, HNO3 at 3 cm/s when absent), times the cross
+// partials of the partitioning: out[0] = dk_NH f_nh_dno3 (TotalNH <- TotalNO3),
+// out[1] = dk_NH f_nh_dso4 (TotalNH <- pSO4), out[2] = dk_NO3 f_no3_dnh
+// (TotalNO3 <- TotalNH), out[3] = dk_NO3 f_no3_dso4 (TotalNO3 <- pSO4).
+struct IsoCrossArgs {
+    const double* Dz;
+    const double* particle_dry_dep;
+    const double* NH3_dry_dep;
+    const double* HNO3_dry_dep;  // nullable
+    const double* particle_wet_dep;
+    const double* other_gas_wet_dep;
+    const double* f[4];  // f_nh_dno3, f_no3_dnh, f_nh_dso4, f_no3_dso4
+    double* out[4];
+    long long stride;    // between bins, for every array
+    unsigned mask;       // bit b: block b on
+};
+
+__global__ void k_iso_cross(IsoCrossArgs a, int nz, int ny, int nx) {
+    const size_t N = (size_t)nz * ny * nx, plane = (size_t)ny * nx;
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const size_t o = (size_t)blockIdx.y * a.stride;
+    const int k = (int)(n / plane);
+    const size_t c = n - (size_t)k * plane;
+    const double dk_wet = a.particle_wet_dep[o + n] - a.other_gas_wet_dep[o + n];
+    double dk_nh = dk_wet, dk_no3 = dk_wet;
+    if (k == 0) {
+        const double dz = a.Dz[o + c];
+        const double pdd = a.particle_dry_dep[o + c];
+        const double hno3 = a.HNO3_dry_dep ? a.HNO3_dry_dep[o + c] : 0.03;
+        dk_nh = dk_nh + (dz > 0.0 ? (pdd - a.NH3_dry_dep[o + c]) / dz : 0.0);
+        dk_no3 = dk_no3 + (dz > 0.0 ? (pdd - hno3) / dz : 0.0);
+    }
+    const double v0 = dk_nh * a.f[0][o + n], v1 = dk_nh * a.f[2][o + n];
+    const double v2 = dk_no3 * a.f[1][o + n], v3 = dk_no3 * a.f[3][o + n];
+    a.out[0][o + n] = (a.mask & 1u) ? v0 : 0.0;
+    a.out[1][o + n] = (a.mask & 2u) ? v1 : 0.0;
+    a.out[2][o + n] = (a.mask & 4u) ? v2 : 0.0;
+    a.out[3][o + n] = (a.mask & 8u) ? v3 : 0.0;
+}
+}  // namespace
+
+extern "C" int orbit_iso_cross_blocks_dev(orbit_ctx* ctx, int n_bins, int nz, int ny, int nx, long long stride,
+                                          const double* Dz, const double* particle_dry_dep,
+                                          const double* NH3_dry_dep, const double* HNO3_dry_dep,
+                                          const double* particle_wet_dep, const double* other_gas_wet_dep,
+                                          const double* const* partials, unsigned block_mask, double* const* out) {
+    if (!ctx || !Dz || !particle_dry_dep || !NH3_dry_dep || !particle_wet_dep || !other_gas_wet_dep || !partials ||
+        !out)
+        return fail(ORBIT_ERR_INVALID, "iso_cross_blocks: NULL argument");
+    if (n_bins < 1 || nz < 1 || ny < 1 || nx < 1)
+        return fail(ORBIT_ERR_INVALID, "iso_cross_blocks: bad dimensions");
+    const size_t N = (size_t)nz * ny * nx;
+    if (n_bins > 1 && stride < (long long)N)
+        return fail(ORBIT_ERR_INVALID, "iso_cross_blocks: the bin stride is shorter than a field");
+    IsoCrossArgs a{};
+    a.Dz = Dz;
+    a.particle_dry_dep = particle_dry_dep;
+    a.NH3_dry_dep = NH3_dry_dep;
+    a.HNO3_dry_dep = HNO3_dry_dep;
+    a.particle_wet_dep = particle_wet_dep;
+    a.other_gas_wet_dep = other_gas_wet_dep;
+    for (int q = 0; q < 4; ++q) {
+        if (!partials[q] || !out[q]) return fail(ORBIT_ERR_INVALID, "iso_cross_blocks: NULL partial or output");
+        a.f[q] = partials[q];
+        a.out[q] = out[q];
+    }
+    a.stride = stride;
+    a.mask = block_mask;
+    hipSetDevice(ctx->device);
+    hipLaunchKernelGGL(k_iso_cross, dim3((unsigned)((N + 255) / 256), n_bins), dim3(256), 0, ctx->stream, a, nz, ny,
+                       nx);
+    ORBIT_HIP_CHECK(hipGetLastError());
+    return ORBIT_OK;
+}
+
 extern "C" int orbit_species_diagonals(orbit_ctx* ctx, int nz, int ny, int nx, const double* Dz,
                                        const orbit_species_fields* f, double* d_out, double* k_out) {
     if (!ctx || !Dz || !f || !d_out || !k_out) return fail(ORBIT_ERR_INVALID, "species_diagonals: NULL argument");
