@@ -190,13 +190,14 @@ int orbit_solve_adjoint_dev(orbit_ctx* ctx, orbit_opset* op, const int* tracers,
  * forbid hipMalloc).  Its scratch (forcing, orbit, Krylov basis, work vectors,
  * GMRES state) is carved from a fixed device array of
  * ORBIT_FIXED_SCRATCH_BYTES per GPU, held from the library's load until the
- * process ends; a call that needs more (about 32 MB per tracer at 64x70x15
- * with restart 30: 12 tracers fit) is refused with ORBIT_ERR_INVALID.  The
+ * process ends; a call that needs more (at 64x70x15 about 63 MB per tracer in
+ * mixed precision, 32 MB in FP64 with restart 30: 12 tracers fit either way)
+ * is refused with ORBIT_ERR_INVALID.  The
  * M~ sweeps are launched directly, never from a captured graph; results are
  * bit-identical to orbit_solve_forward_dev.  Every call on one GPU shares
  * that GPU's array (whatever the last call left in it; every buffer is
  * written before it is read): calls that overlap on one GPU race. */
-#define ORBIT_FIXED_SCRATCH_BYTES (512ull << 20)
+#define ORBIT_FIXED_SCRATCH_BYTES (1024ull << 20)
 int orbit_solve_forward_dev_noalloc(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt,
                                     const double* e_dev, const double* c_warm_dev,
                                     const orbit_opts* opts, double* orbit_out_dev, orbit_info* info);

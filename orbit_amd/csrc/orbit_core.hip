@@ -611,34 +611,6 @@ __global__ __launch_bounds__(256) void k_ext_to_int(const double* __restrict__ s
     }
 }
 
-// scale * (src - sub) from k-major (N,) to the padded layout, src and sub with
-// the same strides (the change of a forcing); same grid as k_ext_to_int.
-__global__ __launch_bounds__(256) void k_ext_to_int_diff(const double* __restrict__ src,
-                                                         const double* __restrict__ sub, long long sstride,
-                                                         double* __restrict__ dst, long long dstride, int nstates,
-                                                         int nz, int ncol, double scale) {
-    __shared__ double tile[PT_COLS * PT_LD];
-    const int s = blockIdx.y;
-    const int slot = s / nstates, st = s - slot * nstates;
-    const int col0 = blockIdx.x * PT_COLS;
-    const size_t off = (size_t)slot * sstride + (size_t)st * (size_t)nz * ncol;
-    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
-        const int k = e / PT_COLS, c = e - k * PT_COLS;
-        double v = 0.0;
-        if (k < nz && col0 + c < ncol) {
-            const size_t i = off + (size_t)k * ncol + col0 + c;
-            v = scale * (src[i] - sub[i]);
-        }
-        tile[c * PT_LD + k] = v;
-    }
-    __syncthreads();
-    double* out = dst + (size_t)slot * dstride + (size_t)st * (size_t)ncol * 16 + (size_t)col0 * 16;
-    for (int e = threadIdx.x; e < PT_COLS * 16; e += blockDim.x) {
-        const int c = e >> 4, k = e & 15;
-        if (col0 + c < ncol) out[e] = tile[c * PT_LD + k];
-    }
-}
-
 // padded device layout -> k-major (N,) per (slot, state); same grid.
 __global__ __launch_bounds__(256) void k_int_to_ext(const double* __restrict__ src, long long sstride,
                                                     double* __restrict__ dst, long long dstride, int nstates,
@@ -1686,41 +1658,155 @@ __global__ void k_add_f32(const float* __restrict__ in, long long is, const doub
     out[(size_t)slot * os + g] = in ? in[(size_t)slot * is + g] + v : v;
 }
 
-// out += (double)in per slot (an FP32-propagated change added to FP64 states),
-// and the block's partial sum of in^2 into partials[(slot*nvec + vec)*nb + block]
-// (RED_CHUNK elements a block, 0 for inactive slots).  grid (nb, nslots).
-__global__ __launch_bounds__(RED_THREADS) void k_acc_f32_norm(const float* __restrict__ in, long long is,
-                                                              double* __restrict__ out, long long os, int n,
-                                                              const int* active, int nb, int nvec, int vec,
-                                                              double* __restrict__ partials, SlotLen sl) {
-    __shared__ double sh[4];
-    const int slot = blockIdx.y, blk = blockIdx.x;
-    const bool on = !active || active[slot];
-    const int nn = sl.len ? min(n, sl.len[slot]) : n;
-    double acc = 0.0;
-    if (on) {
-#pragma unroll
-        for (int e = 0; e < RED_ELEMS; ++e) {
-            const int ix = blk * RED_CHUNK + e * RED_THREADS + threadIdx.x;
-            if (ix < nn) {
-                const double v = (double)in[(size_t)slot * is + ix];
-                out[(size_t)slot * os + ix] += v;
-                acc += v * v;
-            }
-        }
+// ------------------------------------------------------------------------- //
+// orbit-state refinement (mixed precision)
+// ------------------------------------------------------------------------- //
+// The states x_0 .. x_nb of an orbit satisfy one backward-Euler equation per
+// bin.  Forward: A_t x_{t+1} = x_t + dtau e_t.  Adjoint (lam_t = P_t^T lam_{t+1}
+// + R_t): A_t^T (x_t - R_t) = x_{t+1}.  The periodic closure ties the first and
+// last state.  Its exact residual is one FP64 operator apply per bin:
+//   forward  r_t = x_t + E_t - A_t x_{t+1}
+//   adjoint  r_t = x_{t+1} - A_t^T (x_t - E_t)
+// (E = dtau e, resp. R).  Every bin is independent: one launch covers all bins
+// and slots.  Propagating r through the bins from zero gives the periodic
+// residual s - (I - M) c exactly; no accurate inner solve is needed.
+constexpr int CYC_MAXB = 64;
+struct CycArgs {
+    const double* V[CYC_MAXB];  // per bin: column blocks (transposed for the adjoint)
+    const double* H[CYC_MAXB];  // per bin: stencil (transposed for the adjoint)
+    const double* d[CYC_MAXB];  // per bin: diagonals of every tracer
+    long long vstride, hstride, dstride;  // per tracer
+    int ncol, nx, ny, px, nslots;
+    const GroupOp* gops;  // group solves: GroupOps [bin][precision][group], else null
+    int ngroups, gprec;   // gprec: a precision row whose V/H/d are the FP64 operator
+    const int* tmap;
+    const double* X;      // states [slot][nbins + 1][n16]
+    long long xs;
+    const double* E;      // forcing [slot][nbins][n16]
+    long long es;
+    double* R;            // out: residuals [slot][nbins][n16]
+    long long rs;
+    int n16, adj;
+    int xzero;            // every state is zero (not read)
+    const int* active;    // nullable
+};
+
+// grid (ceil(max ncol / 16), chunks, nbins), 256 threads: 16 columns x 16
+// levels.  A chunk is the slots that share one operator (every slot of a
+// single-set solve, one slot group of a group solve): the block reads a
+// column's block and stencil once (when the transport is shared) and loops
+// over the chunk's slots.  The 16 lanes of a column are one wavefront's, so
+// the block product exchanges through LDS with wave-level ordering only.
+__global__ __launch_bounds__(256) void k_cyc_residual(CycArgs a) {
+    __shared__ double sy[256];
+    const int t = blockIdx.z;
+    const double *V, *H, *d;
+    long long vst, hst, dst;
+    int nx, ny, px, first, count;
+    if (a.gops) {
+        const GroupOp& go = a.gops[((size_t)t * N_SWEEP_PRECS + a.gprec) * a.ngroups + blockIdx.y];
+        V = (const double*)go.V; H = (const double*)go.H; d = (const double*)go.d;
+        vst = 0; hst = 0; dst = go.dstride;
+        nx = go.nx; ny = go.ny; px = go.px;
+        first = go.first; count = go.count;
+    } else {
+        V = a.V[t]; H = a.H[t]; d = a.d[t];
+        vst = a.vstride; hst = a.hstride; dst = a.dstride;
+        nx = a.nx; ny = a.ny; px = a.px;
+        first = 0; count = a.nslots;
     }
-    const double bs = block_sum(acc, sh);
-    if (threadIdx.x == 0) partials[((size_t)slot * nvec + vec) * nb + blk] = bs;
+    const int ncol = nx * ny;
+    if ((int)blockIdx.x * 16 >= ncol) return;  // uniform over the block
+    const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int col0 = blockIdx.x * 16 + grp;
+    const bool valid = col0 < ncol;
+    const int col = valid ? col0 : 0;
+    const int j = col / nx, i = col - j * nx;
+    const int cw = west_col(col, i, nx, px), ce = east_col(col, i, nx, px);
+    const size_t base = (size_t)col * 16 + lane;
+    const size_t n16 = (size_t)a.n16;
+    const bool shared_op = vst == 0 && hst == 0;
+    // the shared operator row of this lane (reloaded per slot for per-tracer transports)
+    double vrow[16], hw = 0.0, he = 0.0, hs = 0.0, hn = 0.0;
+    auto load_op = [&](int tr) {
+        const double* Vr = V + (size_t)tr * vst + (size_t)col * 256 + lane * 16;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) vrow[m] = Vr[m];
+        const double* Hc = H + (size_t)tr * hst + (size_t)col * 64 + lane;
+        hw = Hc[0]; he = Hc[16]; hs = Hc[32]; hn = Hc[48];
+    };
+    if (shared_op) load_op(0);
+    for (int q = 0; q < count; ++q) {
+        const int slot = first + q;
+        if (a.active && !a.active[slot]) continue;  // uniform over the block
+        const int tr = a.tmap[slot];
+        if (!shared_op) load_op(tr);
+        const double* __restrict__ Xs = a.X + (size_t)slot * a.xs;
+        const double* __restrict__ Et = a.E + (size_t)slot * a.es + (size_t)t * n16;
+        // y: the state the operator multiplies; u: the one it is compared with
+        const double* __restrict__ Y = Xs + (size_t)(a.adj ? t : t + 1) * n16;
+        const double* __restrict__ U = Xs + (size_t)(a.adj ? t + 1 : t) * n16;
+        auto yv = [&](size_t ix) {
+            const double x = a.xzero ? 0.0 : Y[ix];
+            return a.adj ? x - Et[ix] : x;
+        };
+        const double y0 = yv(base);
+        double yw = 0.0, ye = 0.0, ys = 0.0, yn = 0.0;
+        if (cw >= 0) yw = yv((size_t)cw * 16 + lane);
+        if (ce >= 0) ye = yv((size_t)ce * 16 + lane);
+        if (j > 0) ys = yv((size_t)(col - nx) * 16 + lane);
+        if (j < ny - 1) yn = yv((size_t)(col + nx) * 16 + lane);
+        double u = a.xzero ? 0.0 : U[base];
+        if (!a.adj) u += Et[base];
+        const double dv = d[(size_t)tr * dst + base];
+        wave_sync();  // the previous slot's reads of sy are done
+        sy[threadIdx.x] = y0;
+        wave_sync();
+        double acc = dv * y0;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) acc += vrow[m] * sy[(grp << 4) + m];
+        acc += hw * yw;
+        acc += he * ye;
+        acc += hs * ys;
+        acc += hn * yn;
+        if (valid) a.R[(size_t)slot * a.rs + (size_t)t * n16 + base] = u - acc;
+    }
 }
 
-// out[slot] = max over vec of the sum over blocks of partials (fixed order).
-__global__ __launch_bounds__(256) void k_max_sum_partials(const double* __restrict__ partials, int nb, int nvec,
-                                                          double* __restrict__ out, int nslots) {
-    const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (slot >= nslots) return;  // uniform over the wave
-    double m = 0.0;
-    for (int v = 0; v < nvec; ++v) m = fmax(m, wave_sum_partials(partials + ((size_t)slot * nvec + v) * nb, nb));
-    if ((threadIdx.x & 63) == 0) out[slot] = m;
+// Correction of every state after a GMRES cycle on (I - M~) delta = z_end:
+// the intermediate states (1 .. nbins-1) get z_t + sum_i y_i W_i[t] (the
+// FP32 states of the residual sweep and of each Krylov vector's monodromy
+// apply, in Krylov order), the free state becomes c (already corrected by
+// the cycle) and the other end c - scl (scl: the s of a solve given s, else
+// zero) plus zend (nullable: the residual sweep's last state, when a
+// converged orbit takes its states from that sweep).  Forward: free state 0,
+// end state nbins; adjoint the reverse.  grid (ceil(n16 / 256), nslots, nbins + 1).
+__global__ void k_cyc_combine(double* __restrict__ X, long long xs, const float* __restrict__ Z, long long zs,
+                              const float* __restrict__ W, long long wj, long long ws, const double* __restrict__ coef,
+                              const int* __restrict__ kcols, const double* __restrict__ c, long long cs,
+                              const double* __restrict__ scl, long long ss, const double* __restrict__ zend, int n16,
+                              int nbins, int adj, const int* active, SlotLen sl) {
+    const int slot = blockIdx.y, t = blockIdx.z;
+    if (active && !active[slot]) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n16 || !in_slot(sl, slot, g)) return;
+    double* x = X + (size_t)slot * xs + (size_t)t * n16 + g;
+    const int tfree = adj ? nbins : 0;
+    const double cv = c[(size_t)slot * cs + g];
+    if (t == tfree) {
+        *x = cv;
+    } else if (t == nbins - tfree) {
+        double v = scl ? cv - scl[(size_t)slot * ss + g] : cv;
+        if (zend) v += zend[(size_t)slot * cs + g];
+        *x = v;
+    } else {
+        const size_t off = (size_t)slot * ws + (size_t)(t - 1) * n16 + g;
+        double v = (double)Z[(size_t)slot * zs + (size_t)(t - 1) * n16 + g];
+        const int k = kcols[slot];
+        const double* cf = coef + (size_t)slot * MAXM1;
+        for (int i = 0; i < k; ++i) v += cf[i] * (double)W[(size_t)i * wj + off];
+        *x += v;
+    }
 }
 }  // namespace orbit
 
@@ -2225,7 +2311,6 @@ struct Solver {
     float* fc = nullptr;
     double rho = 0.1;
     double eps_acc = 1e-12;
-    int* nsw_dev = nullptr;  // per-slot sweep counts of a warm-started forced sweep
     double* sv = nullptr;    // s given directly (rhs_is_s), else null
     double* partials = nullptr;
     double* h1 = nullptr;
@@ -2239,7 +2324,18 @@ struct Solver {
     SlotLen sl;               // per-slot lengths (group solves), else none
     const int* inner_cnt = nullptr;  // per-slot sweep counts (group solves), else the uniform ones
     const int* acc_cnt = nullptr;
+    // orbit-state refinement (the mixed path): the states live in `orbit`
+    bool states = false;
+    double* Rc = nullptr;     // bin residuals [slot][nbins][n16]
+    float* Z = nullptr;       // intermediate states of the residual sweep [slot][nbins-1][n16]
+    float* W = nullptr;       // per Krylov vector j < kp: intermediate states of its M~ apply
+    int kp = 0;               // Krylov vectors whose states are kept: a cycle's step cap
+    const double* scl = nullptr;  // a solve given s: the closure offset (end state = free state - s)
 };
+
+// Steps of a mixed-precision GMRES cycle at most (each keeps the FP32 states
+// of its monodromy apply; a slot that needs more continues in the next round).
+constexpr int MIXED_CYCLE_CAP = 16;
 
 int norms(Solver& S, const double* x, long long xslot, int n, std::vector<double>& out) {
     hipStream_t st = S.B.ctx->stream;
@@ -2265,13 +2361,13 @@ void norms_dev(Solver& S, const double* x, long long xslot, int n, double* dst) 
 // Forced accurate sweep.  Forward: orbit[0] = c (or 0), orbit[t+1] = P_t(orbit[t] + dtau e_t).
 // Adjoint: orbit[nb] = c (or 0), orbit[t] = P_t^T orbit[t+1] + R_t for t = nb-1..0.
 // Either way the outer residual s - (I - M)c is the last state minus the first.
-// warm: start every inner solve from the previous sweep's state (same slots).
-int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm, const int* counts_in = nullptr) {
+// The plain FP64 path's check (FP64 inverse blocks).
+int forced_sweep(Solver& S, bool zero_start, int sweeps) {
     hipStream_t st = S.B.ctx->stream;
     const int nbn = S.nbins;
     const size_t n16 = S.n16;
     const long long os = (long long)(nbn + 1) * n16, es = (long long)nbn * n16, vs = (long long)n16;
-    const int sp = S.prec == ORBIT_PREC_FP64 ? SP_F64 : SP_ACC_MIXED;
+    const int sp = SP_F64;
     dim3 g(grid1(n16).x, S.B.nslots);
     const int* mask = S.B.part_dev;
     if (S.dir == FWD) {
@@ -2296,8 +2392,8 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm, const int* c
                 epi.as = os;
                 epi.alpha = -1.0;
             }
-            int rc = inner_solve(S.B, t, FWD, sp, sweeps, S.ta, vs, S.orbit + (size_t)(t + 1) * n16, os, mask, warm,
-                                 counts_in ? counts_in : warm ? S.nsw_dev : S.acc_cnt, &epi);
+            int rc = inner_solve(S.B, t, FWD, sp, sweeps, S.ta, vs, S.orbit + (size_t)(t + 1) * n16, os, mask, false,
+                                 S.acc_cnt, &epi);
             if (rc) return rc;
         }
         if (S.sv) hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, S.sv, vs, 1.0, S.r, vs, (int)n16, mask, S.sl);
@@ -2308,9 +2404,6 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm, const int* c
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, 0.0,
                                S.orbit + (size_t)nbn * n16, os, (int)n16, mask, S.sl);
         for (int t = nbn - 1; t >= 0; --t) {
-            if (warm)  // previous P_t^T output = orbit[t] - R_t
-                hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)t * n16, os, S.E + (size_t)t * n16,
-                                   es, -1.0, S.ta, vs, (int)n16, mask, S.sl);
             // orbit[t] = P_t^T orbit[t+1] + R_t, written by the solve's last sweep
             Epi epi;
             epi.out = S.orbit + (size_t)t * n16;
@@ -2318,8 +2411,8 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm, const int* c
             epi.add = S.E + (size_t)t * n16;
             epi.as = es;
             epi.alpha = 1.0;
-            int rc = inner_solve(S.B, t, ADJ, sp, sweeps, S.orbit + (size_t)(t + 1) * n16, os, S.ta, vs, mask, warm,
-                                 counts_in ? counts_in : warm ? S.nsw_dev : S.acc_cnt, &epi);
+            int rc = inner_solve(S.B, t, ADJ, sp, sweeps, S.orbit + (size_t)(t + 1) * n16, os, S.ta, vs, mask, false,
+                                 S.acc_cnt, &epi);
             if (rc) return rc;
         }
         hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os, S.orbit + (size_t)nbn * n16, os,
@@ -2330,74 +2423,122 @@ int forced_sweep(Solver& S, bool zero_start, int sweeps, bool warm, const int* c
     return ORBIT_OK;
 }
 
-// Inexact forced sweep from zero in FP32 (the mixed path's s~, used for ||s|| and
-// as the first right-hand side): result (FP64) in S.r.  acc (forward only,
-// nullable): add every end-of-bin state to acc's state t+1 ([slot][nbins+1][n16])
-// and leave the largest squared norm of those states per slot in ymax2[slot].
-int forced_sweep_f32(Solver& S, double* acc = nullptr, double* ymax2 = nullptr) {
-    hipStream_t st = S.B.ctx->stream;
+// Exact residual of every bin's equation (k_cyc_residual) into S.Rc for the
+// slots of the round (part mask); xzero: the states are taken as zero.
+int cyc_residual(Solver& S, bool xzero) {
+    const orbit_opset* op = S.B.op;
     const int nbn = S.nbins;
     const size_t n16 = S.n16;
-    const long long es = (long long)nbn * n16, vs = (long long)n16;
-    dim3 g(grid1(n16).x, S.B.nslots);
-    const int* mask = S.B.part_dev;
-    float* x = S.fa;   // running state
-    float* y = S.fb;
-    float* rhs = S.fc;
-    if (S.dir == FWD) {
-        // the first right-hand side here, every later one (rhs = y + e_t) by the
-        // previous bin's last sweep, in place (a lane reads its rhs before it
-        // writes the next one, and a slot's later sweeps do not store)
-        hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.E, es, rhs, vs, (int)n16, mask, S.sl);
-        for (int t = 0; t < nbn; ++t) {
-            Epi epi;
-            epi.out = rhs;
-            epi.os = vs;
-            epi.add = S.E + (size_t)(t + 1) * n16;
-            epi.as = es;
-            int rc = inner_solve(S.B, t, FWD, SP_F32, S.inner, rhs, vs, y, vs, mask, false, S.inner_cnt,
-                                 t + 1 < nbn ? &epi : nullptr);
-            if (rc) return rc;
-            if (acc)
-                hipLaunchKernelGGL(k_acc_f32_norm, dim3(S.nb, S.B.nslots), dim3(RED_THREADS), 0, st, y, vs,
-                                   acc + (size_t)(t + 1) * n16, (long long)(nbn + 1) * n16, (int)n16, mask, S.nb,
-                                   nbn, t, S.partials, S.sl);
-            std::swap(x, y);
-        }
-        if (acc && ymax2)
-            hipLaunchKernelGGL(k_max_sum_partials, grid_waves(S.B.nslots), dim3(256), 0, st, S.partials, S.nb, nbn,
-                               ymax2, S.B.nslots);
+    const bool T = S.dir == ADJ;
+    CycArgs a{};
+    if (S.B.grp) {
+        a.gops = S.B.grp->gops;
+        a.ngroups = S.B.grp->ngroups;
+        a.gprec = SP_F64;  // (the FP64 column blocks, stencil and diagonals; transposed in the tables)
     } else {
-        // x = R_{nb-1}; then x = P_t^T x + R_t for t = nb-2..0
-        hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.E + (size_t)(nbn - 1) * n16, es, x, vs,
-                           (int)n16, mask, S.sl);
-        for (int t = nbn - 2; t >= 0; --t) {
-            // x = P_t^T x + R_t: the sum written by the solve's last sweep, in place
-            Epi epi;
-            epi.out = x;
-            epi.os = vs;
-            epi.add = S.E + (size_t)t * n16;
-            epi.as = es;
-            int rc = inner_solve(S.B, t, ADJ, SP_F32, S.inner, x, vs, y, vs, mask, false, S.inner_cnt, &epi);
-            if (rc) return rc;
+        if (nbn > CYC_MAXB) return fail(ORBIT_ERR_INVALID, "solve: too many bins for the residual launch");
+        for (int b = 0; b < nbn; ++b) {
+            const BinOps& O = op->bins[b];
+            a.V[b] = T ? O.VT64 : O.V64;
+            a.H[b] = T ? O.HT64 : O.H64;
+            a.d[b] = O.d64;
         }
+        a.vstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 256;
+        a.hstride = op->ngroups == 1 ? 0 : (long long)op->ncol * 64;
+        a.dstride = op->n16;
+        a.ncol = op->ncol;
+        a.nx = op->nx;
+        a.ny = op->ny;
+        a.px = op->px;
     }
-    hipLaunchKernelGGL((k_convert_slots<float, double>), g, dim3(256), 0, st, x, vs, S.r, vs, (int)n16, mask, S.sl);
+    a.nslots = S.B.nslots;
+    a.tmap = S.B.tmap_dev;
+    a.X = S.orbit;
+    a.xs = (long long)(nbn + 1) * n16;
+    a.E = S.E;
+    a.es = (long long)nbn * n16;
+    a.R = S.Rc;
+    a.rs = (long long)nbn * n16;
+    a.n16 = (int)n16;
+    a.adj = T ? 1 : 0;
+    a.xzero = xzero ? 1 : 0;
+    a.active = S.B.part_dev;
+    const int ncolmax = S.B.grp ? S.B.grp->ncolmax : op->ncol;
+    const int nchunks = S.B.grp ? S.B.grp->ngroups : 1;
+    hipLaunchKernelGGL(k_cyc_residual, dim3((unsigned)((ncolmax + 15) / 16), nchunks, nbn), dim3(256), 0,
+                       S.B.ctx->stream, a);
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
 
-// The inexact monodromy's sweeps on fixed FP32 buffers: fa -> ... -> result buffer.
-int mono_sweeps_f32(Solver& S, float** result) {
-    const long long vs = (long long)S.n16;
-    float* in = S.fa;
-    float* out = S.fb;
+// The residual sweep: the bin residuals S.Rc propagated from zero by the FP32
+// sweeps, each bin's right-hand side the previous state plus its residual
+// (forward z_{t+1} = P~_t (z_t + r_t); adjoint z_t = P~_t^T (z_{t+1} + r_t)).
+// The intermediate states go to S.Z, the last one (forward state nbins,
+// adjoint state 0) in FP64 to S.r: the periodic residual s - (I - M) c of the
+// current states, to the FP32 sweeps' accuracy.  Slots of the round only.
+int z_sweep(Solver& S) {
+    hipStream_t st = S.B.ctx->stream;
     const int nbn = S.nbins;
+    const size_t n16 = S.n16;
+    const long long es = (long long)nbn * n16, vs = (long long)n16, zs = (long long)(nbn - 1) * n16;
+    dim3 g(grid1(n16).x, S.B.nslots);
+    const int* mask = S.B.part_dev;
+    const bool fwd = S.dir == FWD;
+    float* rhs = S.fc;
+    // the first right-hand side here; every later one (state + residual) by the
+    // previous bin's last sweep, in place (a lane reads its rhs before it writes
+    // the next one, and a slot's later sweeps do not store)
+    hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.Rc + (size_t)(fwd ? 0 : nbn - 1) * n16, es, rhs,
+                       vs, (int)n16, mask, S.sl);
     for (int k = 0; k < nbn; ++k) {
-        const int t = S.dir == FWD ? k : nbn - 1 - k;
-        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.inner, in, vs, out, vs, S.B.active_dev, false, S.inner_cnt);
+        const int t = fwd ? k : nbn - 1 - k;
+        const bool last = k == nbn - 1;
+        const int sout = fwd ? t + 1 : t;  // the state this bin writes
+        float* out = last ? S.fb : S.Z + (size_t)(sout - 1) * n16;
+        Epi epi;
+        if (!last) {
+            epi.out = rhs;
+            epi.os = vs;
+            epi.add = S.Rc + (size_t)(fwd ? t + 1 : t - 1) * n16;
+            epi.as = es;
+        }
+        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.inner, rhs, vs, out, last ? vs : zs, mask, false, S.inner_cnt,
+                             last ? nullptr : &epi);
         if (rc) return rc;
-        std::swap(in, out);
+    }
+    hipLaunchKernelGGL((k_convert_slots<float, double>), g, dim3(256), 0, st, S.fb, vs, S.r, vs, (int)n16, mask, S.sl);
+    ORBIT_HIP_CHECK(hipGetLastError());
+    return ORBIT_OK;
+}
+
+// The inexact monodromy's sweeps on fixed FP32 buffers, fa -> result.  With
+// the state refinement the intermediate states of Krylov vector j's apply go
+// to S.W (they correct the states after the cycle) and the result to fb;
+// else the bins alternate between fa and fb.
+int mono_sweeps_f32(Solver& S, float** result, int j) {
+    const long long vs = (long long)S.n16, zs = (long long)(S.nbins - 1) * S.n16;
+    const int nbn = S.nbins;
+    const bool fwd = S.dir == FWD;
+    float* in = S.fa;
+    long long ins = vs;
+    float* pp[2] = {S.fa, S.fb};
+    for (int k = 0; k < nbn; ++k) {
+        const int t = fwd ? k : nbn - 1 - k;
+        const bool last = k == nbn - 1;
+        float* out;
+        long long outs = vs;
+        if (S.states) {
+            const int sout = fwd ? t + 1 : t;
+            out = last ? S.fb : S.W + ((size_t)j * S.B.nslots * (nbn - 1) + (size_t)(sout - 1)) * S.n16;
+            if (!last) outs = zs;
+        } else {
+            out = pp[(k + 1) & 1];
+        }
+        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.inner, in, ins, out, outs, S.B.active_dev, false, S.inner_cnt);
+        if (rc) return rc;
+        in = out;
+        ins = outs;
     }
     *result = in;
     return ORBIT_OK;
@@ -2405,8 +2546,9 @@ int mono_sweeps_f32(Solver& S, float** result) {
 
 // M v with the monodromy used inside GMRES (inexact FP32 for mixed, accurate
 // FP64 for plain): *res receives the buffer holding it (FP32 when *f32), the
-// caller forms w = v - M v (in the first orthogonalisation pass).
-int apply_M(Solver& S, const double* v, long long vslot, const void** res_out, bool* f32, bool v_in_fa) {
+// caller forms w = v - M v (in the first orthogonalisation pass).  j: the
+// Krylov vector's index (where the state refinement keeps its states).
+int apply_M(Solver& S, const double* v, long long vslot, const void** res_out, bool* f32, bool v_in_fa, int j) {
     hipStream_t st = S.B.ctx->stream;
     const size_t n16 = S.n16;
     const long long vs = (long long)n16;
@@ -2418,16 +2560,18 @@ int apply_M(Solver& S, const double* v, long long vslot, const void** res_out, b
                                S.sl);
         float* res = nullptr;
         if (S.use_graph) {
-            GraphKey key{S.dir, 2, S.inner, S.B.nslots, {}, S.fa, S.fb};
+            float* wj = S.states ? S.W + (size_t)j * S.B.nslots * (S.nbins - 1) * n16 : nullptr;
+            GraphKey key{S.dir, S.states ? 3 : 2, S.inner, S.B.nslots, {}, S.fa, S.states ? (const void*)wj : S.fb};
             key.tracers = {(int)(intptr_t)S.B.tmap_dev, (int)((intptr_t)S.B.tmap_dev >> 32),
-                           (int)(intptr_t)S.B.active_dev, (int)((intptr_t)S.B.active_dev >> 32)};
+                           (int)(intptr_t)S.B.active_dev, (int)((intptr_t)S.B.active_dev >> 32),
+                           (int)(intptr_t)S.fb, (int)((intptr_t)S.fb >> 32)};
             auto& graphs = S.B.op->graphs;
             std::lock_guard<std::mutex> lock(S.B.op->graph_mu);
             auto it = graphs.find(key);
             if (it == graphs.end()) {
                 hipGraph_t graph;
                 ORBIT_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                int rc = mono_sweeps_f32(S, &res);
+                int rc = mono_sweeps_f32(S, &res, j);
                 hipError_t e = hipStreamEndCapture(st, &graph);
                 if (rc) return rc;
                 if (e != hipSuccess) return fail(ORBIT_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -2437,11 +2581,11 @@ int apply_M(Solver& S, const double* v, long long vslot, const void** res_out, b
                 if (e != hipSuccess) return fail(ORBIT_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
                 it = graphs.emplace(key, exec).first;
             } else {
-                res = (S.nbins % 2 == 0) ? S.fa : S.fb;
+                res = (S.states || S.nbins % 2 == 1) ? S.fb : S.fa;
             }
             ORBIT_HIP_CHECK(hipGraphLaunch(it->second, st));
         } else {
-            int rc = mono_sweeps_f32(S, &res);
+            int rc = mono_sweeps_f32(S, &res, j);
             if (rc) return rc;
         }
         *res_out = res;
@@ -2476,7 +2620,9 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     hipStream_t st = S.B.ctx->stream;
     const int ns = S.B.nslots;
     const size_t n16 = S.n16;
-    const long long vs = (long long)n16, kslot = (long long)(S.m + 1) * n16;
+    // the state refinement keeps every Krylov vector's FP32 states: at most kp steps
+    const int mk = S.states ? std::min(S.m, S.kp) : S.m;
+    const long long vs = (long long)n16, kslot = (long long)(mk + 1) * n16;
     const int nb = S.nb;
     dim3 g(grid1(n16).x, ns);
     std::vector<int> act(part);
@@ -2490,10 +2636,10 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
                        kslot, (int)n16, S.sl, fcopy, vs);
     steps.assign(ns, 0);
     std::vector<int> kc(ns, 0);
-    for (int j = 0; j < S.m; ++j) {
+    for (int j = 0; j < mk; ++j) {
         const void* res = nullptr;
         bool f32 = false;
-        int rc = apply_M(S, S.Vk + (size_t)j * n16, kslot, &res, &f32, fcopy != nullptr);
+        int rc = apply_M(S, S.Vk + (size_t)j * n16, kslot, &res, &f32, fcopy != nullptr, j);
         if (rc) return rc;
         // w = v - M v with the first pass of CGS2, then the second pass; the
         // coefficients are summed inside the updates, ||w|| and the Givens
@@ -2523,7 +2669,7 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
             if (!act[s]) continue;
             steps[s] = j + 1;
             kc[s] = j + 1;
-            if (!(S.host[s] > target[s]) || j + 1 == S.m) {
+            if (!(S.host[s] > target[s]) || j + 1 == mk) {
                 act[s] = 0;
                 changed = true;
             } else {
@@ -2537,8 +2683,17 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
     // c += V y for the participating slots
     { int rc0 = set_ints(st, S.kcols, kc.data(), ns); if (rc0) return rc0; }
     hipLaunchKernelGGL(k_gmres_solve, grid1(ns), dim3(256), 0, st, S.gst, S.kcols, ns, S.h1);
-    hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, S.m, S.h1, -1.0, S.c, vs,
+    hipLaunchKernelGGL(k_mupdate, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, mk, S.h1, -1.0, S.c, vs,
                        (int)n16, nb, (double*)nullptr, S.B.part_dev, (const int*)S.kcols, S.sl);
+    if (S.states) {
+        // every state: the free one is c, the intermediate ones get z_t + sum_i y_i W_i[t]
+        const int nbn = S.nbins;
+        const long long ws = (long long)(nbn - 1) * n16;
+        hipLaunchKernelGGL(k_cyc_combine, dim3(grid1(n16).x, ns, nbn + 1), dim3(256), 0, st, S.orbit,
+                           (long long)(nbn + 1) * n16, S.Z, ws, S.W, (long long)ns * ws, ws, S.h1, S.kcols, S.c, vs,
+                           S.scl, vs, (const double*)nullptr, (int)n16, nbn, S.dir == ADJ ? 1 : 0, S.B.part_dev,
+                           S.sl);
+    }
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
 }
@@ -2546,22 +2701,32 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
 // Device scratch of one periodic solve in N_SOLVE_PARTS parts: 0 the forcing,
 // 1 the orbit, 2 the Krylov basis, 3 six FP64 work vectors, 4 three FP32 sweep
 // buffers, 5 reduction partials, Hessenberg coefficients and per-slot ints,
-// 6 the GMRES states.  Sizes per slot scale with n16 (the padded vector).
-constexpr int N_SOLVE_PARTS = 7;
+// 6 the GMRES states; the state refinement of the mixed path adds 7 the bin
+// residuals, 8 the residual sweep's FP32 states and 9 those of every Krylov
+// vector's monodromy apply.  Sizes per slot scale with n16 (the padded vector).
+constexpr int N_SOLVE_PARTS = 10;
+// the context scratch slot of each part (slots 7-9 hold other work)
+constexpr int SOLVE_PART_SLOT[N_SOLVE_PARTS] = {0, 1, 2, 3, 4, 5, 6, 10, 11, 12};
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-void solve_scratch_parts(size_t nt, int nbn, size_t n16, int m, size_t* b) {
+// Krylov vectors a cycle may take: the restart, capped for the mixed path
+inline int cycle_steps(int m, bool mixed) { return mixed ? std::min(m, MIXED_CYCLE_CAP) : m; }
+void solve_scratch_parts(size_t nt, int nbn, size_t n16, int m, bool mixed, size_t* b) {
     const size_t nb = (n16 + RED_CHUNK - 1) / RED_CHUNK;
+    const size_t mk = (size_t)cycle_steps(m, mixed);
     b[0] = nt * nbn * n16 * 8;
     b[1] = nt * (nbn + 1) * n16 * 8;
-    b[2] = nt * (m + 1) * n16 * 8;
+    b[2] = nt * (mk + 1) * n16 * 8;
     b[3] = nt * 6 * n16 * 8;
     b[4] = nt * 3 * n16 * 4;
     b[5] = nt * MAXM1 * nb * 8 * 2 + nt * MAXM1 * 8 * 2 + nt * 8 * 2 + nt * 4 * 6 + 1024;
     b[6] = nt * sizeof(GState);
+    b[7] = mixed ? nt * nbn * n16 * 8 : 0;
+    b[8] = mixed ? nt * (nbn - 1) * n16 * 4 : 0;
+    b[9] = mixed ? mk * nt * (nbn - 1) * n16 * 4 : 0;
 }
-size_t solve_scratch_bytes(size_t nt, int nbn, size_t n16, int m) {
+size_t solve_scratch_bytes(size_t nt, int nbn, size_t n16, int m, bool mixed) {
     size_t b[N_SOLVE_PARTS], t = 0;
-    solve_scratch_parts(nt, nbn, n16, m, b);
+    solve_scratch_parts(nt, nbn, n16, m, mixed, b);
     for (int q = 0; q < N_SOLVE_PARTS; ++q) t += align256(b[q]);
     return t;
 }
@@ -2587,11 +2752,13 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             return fail(ORBIT_ERR_INVALID, "solve: tracer index out of range");
     if (G && (pd || o.rhs_is_s || warm_dev))
         return fail(ORBIT_ERR_INVALID, "solve: a group solve starts cold from e (no s, warm start or pass state)");
-    const bool delta = pd && pd->e_prev;
     if (pd && (dir != FWD || o.precision != ORBIT_PREC_MIXED || o.rhs_is_s || !pd->s_state))
         return fail(ORBIT_ERR_INVALID, "solve: pass-to-pass state needs a mixed-precision forward solve and s_state");
-    if (delta && !(warm_dev && o.warm_orbit))
-        return fail(ORBIT_ERR_INVALID, "solve: a forcing change needs the previous orbit as the warm whole orbit");
+    // a pass after the first: the previous pass's orbit starts the states and
+    // pd->s_state holds its s~
+    const bool next_pass = pd && warm_dev && o.warm_orbit;
+    if (pd && warm_dev && !o.warm_orbit)
+        return fail(ORBIT_ERR_INVALID, "solve: a later pass starts from the previous pass's whole orbit");
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
 
@@ -2600,6 +2767,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.dir = dir;
     S.rho = op->rho;
     S.eps_acc = std::min(1e-12, 0.01 * o.tol);
+    const bool mixed = S.prec == ORBIT_PREC_MIXED;
     // per slot (its set's calibration): sweep contraction, inner and accurate
     // sweep counts, the relative error of one inexact (FP32) P apply
     std::vector<double> rho_s(nt), eps_in_s(nt);
@@ -2614,9 +2782,8 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     }
     S.inner = *std::max_element(inner_s.begin(), inner_s.end());
     S.acc = *std::max_element(acc_s.begin(), acc_s.end());
-    const double eps_in = eps_in_s[0];  // (single-set solves: the same for every slot)
     S.m = o.restart;
-    S.use_graph = o.use_graph != 0 && S.prec == ORBIT_PREC_MIXED && !ctx->timing && !G;
+    S.use_graph = o.use_graph != 0 && mixed && !ctx->timing && !G;
     S.n16 = G ? G->n16max : op->n16;
     S.nbins = op->nbins;
     const size_t n16 = S.n16;
@@ -2625,11 +2792,11 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
 
     // scratch: the context's growable buffers, or carved from a caller's arena
     size_t part_bytes[N_SOLVE_PARTS];
-    solve_scratch_parts(nt, nbn, n16, S.m, part_bytes);
+    solve_scratch_parts(nt, nbn, n16, S.m, mixed, part_bytes);
     void* part_ptr[N_SOLVE_PARTS];
     int rc = ORBIT_OK;
     if (arena) {
-        const size_t need = solve_scratch_bytes(nt, nbn, n16, S.m);
+        const size_t need = solve_scratch_bytes(nt, nbn, n16, S.m, mixed);
         if (need > arena_bytes)
             return fail(ORBIT_ERR_INVALID, "solve: " + std::to_string(nt) + " tracers of this operator set with restart " +
                                                std::to_string(S.m) + " need " + std::to_string(need) +
@@ -2642,8 +2809,12 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         }
     } else {
         for (int q = 0; q < N_SOLVE_PARTS && !rc; ++q) {
-            rc = ctx->ensure(q, part_bytes[q]);
-            part_ptr[q] = ctx->scratch[q].p;
+            if (!part_bytes[q]) {
+                part_ptr[q] = nullptr;
+                continue;
+            }
+            rc = ctx->ensure(SOLVE_PART_SLOT[q], part_bytes[q]);
+            part_ptr[q] = ctx->scratch[SOLVE_PART_SLOT[q]].p;
         }
         if (rc) return rc;
     }
@@ -2669,9 +2840,16 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.kcols = (int*)sm; sm += (size_t)nt * 4;
     S.B.tmap_dev = (int*)sm; sm += (size_t)nt * 4;
     S.B.active_dev = (int*)sm; sm += (size_t)nt * 4;
-    S.B.part_dev = (int*)sm; sm += (size_t)nt * 4;
-    S.nsw_dev = (int*)sm;
+    S.B.part_dev = (int*)sm;
     S.gst = (GState*)part_ptr[6];
+    if (mixed) {
+        S.states = true;
+        S.Rc = (double*)part_ptr[7];
+        S.Z = (float*)part_ptr[8];
+        S.W = (float*)part_ptr[9];
+        S.kp = cycle_steps(S.m, true);
+        S.scl = S.sv;
+    }
     S.B.ctx = ctx;
     S.B.op = op;
     S.B.nslots = nt;
@@ -2697,6 +2875,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     }
 
     const long long vs = (long long)n16;
+    const long long os = (long long)(nbn + 1) * n16;
     dim3 g(grid1(n16).x, nt);
     std::vector<int> ones(nt, 1), tv(tracers, tracers + nt);
     { int rc0 = set_ints(st, S.B.tmap_dev, tv.data(), nt); if (rc0) return rc0; }
@@ -2713,110 +2892,172 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     } else if (G) {
         hipLaunchKernelGGL(k_ext_to_int_g, grid_permute(G->ncolmax, nt * nbn), dim3(256), 0, st, G->in_ptrs, G->sncol,
                            G->snz, S.E, (long long)nbn * n16, (long long)n16, nbn, scale);
-    } else if (!delta) {  // (a forcing change loads e after propagating the change)
+    } else {
         hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
                            (long long)nbn * op->N, S.E, (long long)nbn * n16, nbn, op->nz, op->ncol, scale);
     }
     ORBIT_HIP_CHECK(hipGetLastError());
 
-    std::vector<double> snorm;
-    // Accuracy of the accurate sweeps.  The FP64 path keeps the fixed target
-    // S.eps_acc.  A mixed check needs its propagation error (at most nbins times
-    // the per-bin error: the propagators contract, and the states are no larger
-    // than ||c|| + ||s||) small against tol * ||s||: eps = 2e-3 tol ||s|| /
-    // (||c|| + ||s||) per slot keeps the reported residual within 1.6 % of tol of
-    // the true one, tighter than a fixed target where ||c|| >> ||s|| (long-lived
-    // tracers), looser where the orbit is short-lived.
-    auto eps_for = [&](int s, double cnorm) {
-        if (S.prec != ORBIT_PREC_MIXED) return S.eps_acc;
-        const double e = 2e-3 * o.tol * snorm[s] / (cnorm + snorm[s] + 1e-300);
-        return std::min(1e-6, std::max(1e-15, e));
-    };
-    auto cold_sweeps = [&](int s, double eps) {  // from zero to eps
-        return o.acc_sweeps > 0 ? o.acc_sweeps
-                                : std::min(80, std::max(4, (int)std::ceil(std::log(eps) / std::log(rho_s[s]))));
-    };
+    std::vector<double> snorm(nt, 0.0);
+    std::vector<int> part(nt, 1), conv(nt, 0), zero(nt, 0), iters(nt, 0), rounds(nt, 0);
+    std::vector<double> rnorm(nt, 0.0), last_rel(nt, 1.0);
     const int max_cycles = o.max_rounds > 0 ? o.max_rounds : o.maxiter;
-    bool need_check = true;
-    int warm_first = 0;  // > 0: the first check starts from a caller-given orbit
-    std::vector<int> first_nsw;  // per-slot sweeps of that first check (else warm_first for all)
-    if (S.prec == ORBIT_PREC_MIXED) {
-        // s~ by an FP32 forced sweep: only its norm and its direction are used; the
-        // residuals that decide convergence are accurate FP64 sweeps.
-        std::vector<double> dnorm;
-        const long long os_ = (long long)(nbn + 1) * n16, es_ = (long long)nbn * n16;
-        if (S.sv) {
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.sv, vs, S.sv, vs, 0.0, S.r, vs, (int)n16, nullptr, S.sl);
-        } else if (delta) {
-            // The forcing changed by e - e_prev since the previous orbit: propagate
-            // dtau (e - e_prev) from zero with the FP32 sweeps, add its states to the
-            // previous orbit's (the start of the first check) and its last state to
-            // the previous s~ (the solve is linear in the forcing).
-            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
-                               (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
-            hipLaunchKernelGGL(k_ext_to_int_diff, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
-                               pd->e_prev, (long long)nbn * op->N, S.E, es_, nbn, op->nz, op->ncol, scale);
-            rc = forced_sweep_f32(S, S.orbit, S.h1);  // h1 (free until GMRES): max_t ||y_t||^2
-            if (rc) return rc;
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.r, vs, pd->s_state, vs, 1.0, S.r, vs, (int)n16,
-                               nullptr, S.sl);
-            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * nbn), dim3(256), 0, st, in_dev,
-                               (long long)nbn * op->N, S.E, es_, nbn, op->nz, op->ncol, scale);
-        } else {
-            rc = forced_sweep_f32(S);
-        }
-        if (rc) return rc;
-        if (pd) ORBIT_HIP_CHECK(hipMemcpyAsync(pd->s_state, S.r, (size_t)nt * n16 * 8, hipMemcpyDeviceToDevice, st));
-        std::vector<double> cn;
-        if (delta) {
-            // max_t ||y_t|| of the change, ||s~|| and ||c|| with one host round trip
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit, os_, S.orbit, os_, 0.0, S.c, vs, (int)n16,
-                               nullptr, S.sl);
-            norms_dev(S, S.r, vs, (int)n16, S.h1 + nt);
-            norms_dev(S, S.c, vs, (int)n16, S.h1 + 2 * nt);
-            ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.h1, (size_t)3 * nt * 8, hipMemcpyDeviceToHost, st));
-            ORBIT_HIP_CHECK(hipStreamSynchronize(st));
-            dnorm.assign(S.host, S.host + nt);
-            snorm.assign(S.host + nt, S.host + 2 * nt);
-            cn.assign(S.host + 2 * nt, S.host + 3 * nt);
-            for (std::vector<double>* v : {&dnorm, &snorm, &cn})
-                for (double& x : *v) x = std::sqrt(x);
-        } else {
-            rc = norms(S, S.r, vs, (int)n16, snorm);
-            if (rc) return rc;
-        }
-        if (delta) {
-            // Error of the start: the previous states are accurate to eps_acc; each of
-            // the change's states y_t is off by at most the errors of the FP32 inner
-            // solves so far, rho^inner relative to their outputs, which are no larger
-            // than max_t ||y_t|| (the propagators contract).  A worst-case sum with
-            // rho's calibration margin: no extra sweep on top.
-            first_nsw.assign(nt, S.acc);
-            for (int s = 0; s < nt; ++s) {
-                if (!(cn[s] > 0.0)) continue;
-                const double eps = eps_for(s, cn[s]);
-                const int cap = cold_sweeps(s, eps);
-                const double rel = eps + nbn * eps_in * dnorm[s] / cn[s];
-                int k = cap;
-                if (rel < 1.0) k = (int)std::ceil(std::log(eps / rel) / std::log(S.rho));
-                first_nsw[s] = std::min(cap, std::max(2, k));
-                warm_first = std::max(warm_first, first_nsw[s]);
+    int cycles = 0;
+    // the part mask on the device, uploaded only when it changes
+    std::vector<int> part_up(ones);
+    auto upload_part = [&]() {
+        if (part_up == part) return ORBIT_OK;
+        part_up = part;
+        return set_ints(st, S.B.part_dev, part.data(), nt);
+    };
+    auto mark_zero_sources = [&]() {
+        for (int s = 0; s < nt; ++s)
+            if (snorm[s] < 1e-30) {
+                zero[s] = 1;
+                part[s] = 0;
+                conv[s] = 1;
             }
-        } else if (warm_dev && o.warm_orbit) {
-            // whole previous orbit: its states start the first accurate sweep
+    };
+
+    if (mixed) {
+        // Orbit-state refinement.  The states x_0 .. x_nb live in S.orbit (FP64).
+        // A round: the exact residual of every bin's equation (one launch), the
+        // residual sweep (FP32: the periodic residual s - (I - M)c, the check and
+        // the right-hand side), one GMRES cycle on (I - M~) delta = that residual
+        // whose monodromy applies keep their FP32 states, and the correction of
+        // every state from them.  No accurate inner solve is run: the residual is
+        // exact, and the residual sweep reports it to the FP32 sweeps' accuracy.
+        const int tfree = dir == FWD ? 0 : nbn;  // the state GMRES solves for
+        const int tend = nbn - tfree;            // tied to it: x_end = x_free - s (s: given s, else 0)
+        bool xzero = false;
+        bool need_s = true;                      // ||s~|| still to be measured
+        if (warm_dev && o.warm_orbit) {
+            // a whole earlier orbit starts every state
             hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt * (nbn + 1)), dim3(256), 0, st, warm_dev,
-                               (long long)(nbn + 1) * op->N, S.orbit, os_, nbn + 1, op->nz, op->ncol, 1.0);
-            const double* c0 = dir == FWD ? S.orbit : S.orbit + (size_t)nbn * n16;
-            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, c0, os_, c0, os_, 0.0, S.c, vs, (int)n16, nullptr, S.sl);
-            warm_first = o.warm_sweeps > 0 ? std::min(o.warm_sweeps, S.acc) : S.acc;
-        } else if (warm_dev) {
-            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
-                               S.c, vs, 1, op->nz, op->ncol, 1.0);
+                               (long long)(nbn + 1) * op->N, S.orbit, os, nbn + 1, op->nz, op->ncol, 1.0);
         } else {
-            hipLaunchKernelGGL(k_zero, g, dim3(256), 0, st, S.c, vs, (int)n16, S.sl);  // r = s~ - (I-M)0
-            need_check = false;
+            hipLaunchKernelGGL(k_zero, dim3(grid1((size_t)(nbn + 1) * n16).x, nt), dim3(256), 0, st, S.orbit, os,
+                               (int)((nbn + 1) * n16), S.sl);
+            if (warm_dev)  // c0 only: the first residual sweep builds the other states from it
+                hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, warm_dev,
+                                   (long long)op->N, S.orbit + (size_t)tfree * n16, os, 1, op->nz, op->ncol, 1.0);
+            else
+                xzero = !S.sv;
+        }
+        // the free state in c and the closure of the other end
+        hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.orbit + (size_t)tfree * n16, os,
+                           S.orbit + (size_t)tfree * n16, os, 0.0, S.c, vs, (int)n16, nullptr, S.sl);
+        if (S.sv)
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.sv, vs, -1.0, S.orbit + (size_t)tend * n16, os,
+                               (int)n16, nullptr, S.sl);
+        else
+            hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, S.c, vs, S.c, vs, 0.0, S.orbit + (size_t)tend * n16, os,
+                               (int)n16, nullptr, S.sl);
+        if (S.sv) {
+            rc = norms(S, S.sv, vs, (int)n16, snorm);
+            if (rc) return rc;
+            need_s = false;
+        } else if (warm_dev && !next_pass) {
+            // a warm start without the previous s~: one residual sweep of the
+            // zero states gives it
+            rc = cyc_residual(S, true);
+            if (!rc) rc = z_sweep(S);
+            if (!rc) rc = norms(S, S.r, vs, (int)n16, snorm);
+            if (rc) return rc;
+            need_s = false;
+        }
+        if (!need_s) mark_zero_sources();
+        bool first = true;
+        while (true) {
+            { int rc0 = upload_part(); if (rc0) return rc0; }
+            rc = cyc_residual(S, first && xzero);
+            if (!rc) rc = z_sweep(S);
+            if (rc) return rc;
+            if (first && pd) {
+                // s~ of this pass: the previous pass's s~ plus the residual of the
+                // previous orbit under the new forcing (off by that orbit's own
+                // residual, at most tol ||s~||); the first pass's is the residual
+                // of the zero states.  ||r|| and ||s~|| with one host round trip.
+                if (next_pass)
+                    hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, pd->s_state, vs, S.r, vs, 1.0, pd->s_state, vs,
+                                       (int)n16, nullptr, S.sl);
+                else
+                    ORBIT_HIP_CHECK(hipMemcpyAsync(pd->s_state, S.r, (size_t)nt * n16 * 8, hipMemcpyDeviceToDevice, st));
+                norms_dev(S, S.r, vs, (int)n16, S.h1);
+                norms_dev(S, pd->s_state, vs, (int)n16, S.h1 + nt);
+                ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.h1, (size_t)2 * nt * 8, hipMemcpyDeviceToHost, st));
+                ORBIT_HIP_CHECK(hipStreamSynchronize(st));
+                for (int s = 0; s < nt; ++s) {
+                    rnorm[s] = std::sqrt(S.host[s]);
+                    snorm[s] = std::sqrt(S.host[nt + s]);
+                }
+            } else {
+                std::vector<double> rn;
+                rc = norms(S, S.r, vs, (int)n16, rn);
+                if (rc) return rc;
+                for (int s = 0; s < nt; ++s)
+                    if (part[s]) rnorm[s] = rn[s];
+                if (first && need_s) snorm = rn;  // the residual of the zero states is s~
+            }
+            if (first && need_s) mark_zero_sources();
+            bool any = false;
+            std::vector<int> done(nt, 0);
+            bool any_done = false;
+            for (int s = 0; s < nt; ++s) {
+                if (!part[s]) continue;
+                last_rel[s] = rnorm[s] / snorm[s];
+                if (!(rnorm[s] > o.tol * snorm[s])) {
+                    conv[s] = 1;
+                    part[s] = 0;
+                    done[s] = 1;
+                    any_done = true;
+                } else {
+                    any = true;
+                }
+            }
+            if (any_done) {
+                // A converged slot's states take the residual sweep's (the
+                // correction with delta = 0, the end state its last one): every
+                // bin's equation then holds to the FP32 sweeps' accuracy, and the
+                // end state differs from the free one by the periodic residual, as
+                // when the states are propagated from c.  The free state and the
+                // periodic residual do not change.
+                std::vector<int> zk(nt, 0);
+                { int rc0 = set_ints(st, S.kcols, zk.data(), nt); if (rc0) return rc0; }
+                { int rc0 = set_ints(st, S.B.part_dev, done.data(), nt); if (rc0) return rc0; }
+                part_up = done;
+                const long long zs = (long long)(nbn - 1) * n16;
+                hipLaunchKernelGGL(k_cyc_combine, dim3(grid1(n16).x, nt, nbn + 1), dim3(256), 0, st, S.orbit, os, S.Z,
+                                   zs, S.W, (long long)nt * zs, zs, S.h1, S.kcols, S.c, vs, S.scl, vs,
+                                   (const double*)S.r, (int)n16, nbn, dir == ADJ ? 1 : 0, S.B.part_dev, S.sl);
+                ORBIT_HIP_CHECK(hipGetLastError());
+            }
+            if (o.verbose) {
+                fprintf(stderr, "[orbit] round %d:", cycles);
+                for (int s = 0; s < nt; ++s) fprintf(stderr, " %.2e", last_rel[s]);
+                fprintf(stderr, "\n");
+            }
+            first = false;
+            if (!any || cycles >= max_cycles) break;
+            // Every right-hand side comes from the FP32 residual sweep, good to about
+            // rho^inner relative: a cycle stops there (or at 1e-6, or well below tol).
+            std::vector<double> target(nt, 0.0);
+            for (int s = 0; s < nt; ++s)
+                target[s] = std::max(0.3 * o.tol * snorm[s], std::max(eps_in_s[s], 1e-6) * rnorm[s]);
+            { int rc0 = upload_part(); if (rc0) return rc0; }
+            std::vector<int> steps;
+            rc = gmres_cycle(S, part, target, steps);
+            if (rc) return rc;
+            for (int s = 0; s < nt; ++s)
+                if (part[s]) {
+                    iters[s] += steps[s];
+                    rounds[s] += 1;
+                }
+            ++cycles;
         }
     } else {
+        // Plain FP64: the reference's restarted GMRES on (I - M)c = s with the
+        // accurate sweeps as M; every check re-propagates the orbit from c.
         // s (forward) / s_adj (adjoint): accurate forced sweep from zero; x0 = s
         const double* sptr;
         long long sstride;
@@ -2824,71 +3065,30 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             sptr = S.sv;
             sstride = vs;
         } else {
-            rc = forced_sweep(S, true, S.acc, false);
+            rc = forced_sweep(S, true, S.acc);
             if (rc) return rc;
             sptr = dir == FWD ? S.orbit + (size_t)nbn * n16 : S.orbit;
-            sstride = (long long)(nbn + 1) * n16;
+            sstride = os;
         }
         rc = norms(S, sptr, sstride, (int)n16, snorm);
         if (rc) return rc;
+        rnorm = snorm;
         if (warm_dev) {
-            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, warm_dev, (long long)op->N,
-                               S.c, vs, 1, op->nz, op->ncol, 1.0);
+            // c0, or the free state of a whole earlier orbit
+            const int tfree = dir == FWD ? 0 : nbn;
+            const long long wst = o.warm_orbit ? (long long)(nbn + 1) * op->N : (long long)op->N;
+            const double* w0 = warm_dev + (o.warm_orbit ? (size_t)tfree * op->N : 0);
+            hipLaunchKernelGGL(k_ext_to_int, grid_permute(op->ncol, nt), dim3(256), 0, st, w0, wst, S.c, vs, 1,
+                               op->nz, op->ncol, 1.0);
         } else {
             hipLaunchKernelGGL(k_axpby, g, dim3(256), 0, st, sptr, sstride, sptr, sstride, 0.0, S.c, vs, (int)n16,
                                nullptr, S.sl);
         }
-    }
-    std::vector<int> part(nt, 1), conv(nt, 0), zero(nt, 0), iters(nt, 0), rounds(nt, 0);
-    std::vector<double> rnorm(snorm), last_rel(nt, 1.0);
-    for (int s = 0; s < nt; ++s)
-        if (snorm[s] < 1e-30) {
-            zero[s] = 1;
-            part[s] = 0;
-            conv[s] = 1;
-        }
-    int cycles = 0;
-    // A cold mixed solve's first cycle solves against s~ itself: its true residual
-    // cannot fall below the FP32 sweep's error in s~ (about rho^inner relative),
-    // so it stops there instead of at 1e-6 (the accurate check then measures).
-    bool on_s_tilde = S.prec == ORBIT_PREC_MIXED && !need_check && !S.sv;
-    bool have_orbit = warm_first > 0;
-    int next_sweeps = warm_first > 0 ? warm_first : S.acc;
-    if (warm_first > 0) {
-        std::vector<int> nsw = first_nsw.empty() ? std::vector<int>(nt, warm_first) : first_nsw;
-        int rc0 = set_ints(st, S.nsw_dev, nsw.data(), nt);
-        if (rc0) return rc0;
-    }
-    // the part mask on the device, uploaded only when it changes
-    std::vector<int> part_up;
-    auto upload_part = [&]() {
-        if (part_up == part) return ORBIT_OK;
-        part_up = part;
-        return set_ints(st, S.B.part_dev, part.data(), nt);
-    };
-    while (true) {
-        { int rc0 = upload_part(); if (rc0) return rc0; }
-        if (need_check) {
-            const bool warm = have_orbit && S.prec == ORBIT_PREC_MIXED;
-            if (!warm && S.prec == ORBIT_PREC_MIXED) {
-                // a cold mixed check: per-slot sweeps from zero to eps_for(||c||)
-                std::vector<double> cn;
-                rc = norms(S, S.c, vs, (int)n16, cn);
-                if (rc) return rc;
-                std::vector<int> kc(nt, 4);
-                int kmax = 2;
-                for (int s = 0; s < nt; ++s) {
-                    if (!part[s]) continue;
-                    kc[s] = cold_sweeps(s, eps_for(s, cn[s]));
-                    kmax = std::max(kmax, kc[s]);
-                }
-                { int rc0 = set_ints(st, S.nsw_dev, kc.data(), nt); if (rc0) return rc0; }
-                rc = forced_sweep(S, false, kmax, false, S.nsw_dev);
-            } else {
-                rc = forced_sweep(S, false, warm ? next_sweeps : S.acc, warm);
-            }
+        mark_zero_sources();
+        while (true) {
+            { int rc0 = upload_part(); if (rc0) return rc0; }
+            rc = forced_sweep(S, false, S.acc);
             if (rc) return rc;
-            have_orbit = true;
             rc = norms(S, S.r, vs, (int)n16, rnorm);
             if (rc) return rc;
             bool any = false;
@@ -2903,63 +3103,26 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                 }
             }
             if (o.verbose) {
-                fprintf(stderr, "[orbit] cycle %d (%d sweeps%s):", cycles, warm ? next_sweeps : S.acc,
-                        warm ? ", warm" : "");
+                fprintf(stderr, "[orbit] cycle %d (%d sweeps):", cycles, S.acc);
                 for (int s = 0; s < nt; ++s) fprintf(stderr, " %.2e", last_rel[s]);
                 fprintf(stderr, "\n");
             }
             if (!any || cycles >= max_cycles) break;
-        }
-        need_check = true;
-        std::vector<double> target(nt, 0.0);
-        for (int s = 0; s < nt; ++s) {
-            if (S.prec == ORBIT_PREC_MIXED)
-                target[s] = std::max(0.3 * o.tol * snorm[s],
-                                     (on_s_tilde ? std::max(eps_in_s[s], 1e-6) : 1e-6) * rnorm[s]);
-            else
-                target[s] = o.tol * snorm[s];
-        }
-        on_s_tilde = false;
-        { int rc0 = upload_part(); if (rc0) return rc0; }
-        std::vector<int> steps;
-        rc = gmres_cycle(S, part, target, steps);
-        if (rc) return rc;
-        for (int s = 0; s < nt; ++s)
-            if (part[s]) {
-                iters[s] += steps[s];
-                rounds[s] += 1;
-            }
-        ++cycles;
-        if (S.prec == ORBIT_PREC_MIXED && have_orbit) {
-            // sweeps for the warm-started residual: the previous states are off by
-            // ||M_t delta|| <= ||delta|| (the propagators contract; a bound that is
-            // tight only at the first bins), and each sweep contracts the error by rho
-            // (with its calibration margin): no extra sweep on top.  ||c|| and
-            // ||delta|| with one host round trip.
-            norms_dev(S, S.c, vs, (int)n16, S.h1);  // h1 is free between GMRES cycles
-            hipLaunchKernelGGL(k_gather_ynorm, grid1(nt), dim3(256), 0, st, S.gst, nt, S.resid);
-            ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.h1, nt * 8, hipMemcpyDeviceToHost, st));
-            ORBIT_HIP_CHECK(hipMemcpyAsync(S.host + nt, S.resid, nt * 8, hipMemcpyDeviceToHost, st));
-            ORBIT_HIP_CHECK(hipStreamSynchronize(st));
-            next_sweeps = 2;
-            std::vector<int> nsw(nt, 0);
-            for (int s = 0; s < nt; ++s) {
-                if (!part[s]) continue;
-                const double cn = std::sqrt(S.host[s]), yn = S.host[nt + s];
-                const double rel = cn > 0.0 ? yn / cn : 1.0;
-                const double eps = eps_for(s, cn);
-                const int cap = cold_sweeps(s, eps);
-                int k = cap;
-                if (rel > 0.0 && rel < 1.0) k = (int)std::ceil(std::log(eps / rel) / std::log(rho_s[s]));
-                nsw[s] = std::min(cap, std::max(2, k));
-                next_sweeps = std::max(next_sweeps, nsw[s]);
-            }
-            { int rc0 = set_ints(st, S.nsw_dev, nsw.data(), nt); if (rc0) return rc0; }
+            std::vector<double> target(nt, 0.0);
+            for (int s = 0; s < nt; ++s) target[s] = o.tol * snorm[s];
+            { int rc0 = upload_part(); if (rc0) return rc0; }
+            std::vector<int> steps;
+            rc = gmres_cycle(S, part, target, steps);
+            if (rc) return rc;
+            for (int s = 0; s < nt; ++s)
+                if (part[s]) {
+                    iters[s] += steps[s];
+                    rounds[s] += 1;
+                }
+            ++cycles;
         }
     }
-    // re-run the last forced sweep for all non-zero slots so every orbit matches its c
-    // (slots that converged earlier kept their orbit; slots still part[] have the latest)
-    // zero-source slots: zero orbit
+    // zero-source slots: zero orbit (a slot that converged earlier kept its orbit)
     std::vector<int> zmask(nt, 0);
     bool anyzero = false;
     for (int s = 0; s < nt; ++s)
@@ -2967,7 +3130,6 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             zmask[s] = 1;
             anyzero = true;
         }
-    const long long os = (long long)(nbn + 1) * n16;
     if (anyzero) {
         { int rc0 = set_ints(st, S.B.part_dev, zmask.data(), nt); if (rc0) return rc0; }
         for (int k = 0; k <= nbn; ++k)

@@ -1138,9 +1138,8 @@ extern "C" int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int
 // d_AD(c[tau+1]) to e and solves again.  FP64: warm-started from the previous
 // c0, the same launches as the host loop over orbit_fct_source_batch_dev and
 // orbit_solve_forward_dev.  Mixed precision: each pass after the first starts
-// from the previous orbit plus the FP32 propagation of the change of the
-// forcing (solve_forward_pass), so its first accurate check needs only the
-// sweeps for the FP32 error of that change instead of a cold check.
+// its states from the previous orbit (solve_forward_pass), so its first
+// residual is only that of the change of the forcing.
 // The forward passes; pass_orbits (nullable) receives the orbits of passes
 // 0 .. n_passes-2 in turn, (n_passes - 1, nt, n_bins + 1, N).
 static int forward_fct_passes(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* const* grids, const int* tracers,
@@ -1168,20 +1167,19 @@ static int forward_fct_passes(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* c
     hipStream_t st = ctx->stream;
     const bool inc = o.precision == ORBIT_PREC_MIXED && n_passes > 1;
     const size_t ne = (size_t)nt * nbn * N, n16 = (size_t)op->n16;
-    // scratch: two forcings (this pass's and, mixed, the previous pass's in
-    // turn), c0 (FP64 passes), s~ (mixed passes) and the per-tracer ghost flags
-    double *ebuf[2] = {nullptr, nullptr}, *c0 = nullptr, *s_state = nullptr;
+    // scratch: the pass's forcing, c0 (FP64 passes), s~ (mixed passes) and the
+    // per-tracer ghost flags
+    double *e_eff = nullptr, *c0 = nullptr, *s_state = nullptr;
     int* gmask = nullptr;
     FctBin* bins = nullptr;  // every bin's grid, for one FCT launch per pass
     const size_t nbin_bytes = ((size_t)nbn * sizeof(FctBin) + 255) / 256 * 256;
     const bool one_launch = (long long)op->nz * nt * nbn <= 65535;
     if (n_passes > 1) {
-        int rc0 = ctx->ensure(8, nbin_bytes + (2 * ne + (size_t)nt * N + (size_t)nt * n16) * 8 + (size_t)nt * 4);
+        int rc0 = ctx->ensure(8, nbin_bytes + (ne + (size_t)nt * N + (size_t)nt * n16) * 8 + (size_t)nt * 4);
         if (rc0) return rc0;
         bins = ctx->buf<FctBin>(8);
-        ebuf[0] = reinterpret_cast<double*>(reinterpret_cast<char*>(bins) + nbin_bytes);
-        ebuf[1] = ebuf[0] + ne;
-        c0 = ebuf[1] + ne;
+        e_eff = reinterpret_cast<double*>(reinterpret_cast<char*>(bins) + nbin_bytes);
+        c0 = e_eff + ne;
         s_state = c0 + (size_t)nt * N;
         if (ghost_dev && ghost_tracers) {
             gmask = reinterpret_cast<int*>(s_state + (size_t)nt * n16);
@@ -1200,12 +1198,10 @@ static int forward_fct_passes(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* c
     int worst = rc;
     if (n_passes == 1) return worst;
     const long long os = (long long)(nbn + 1) * N, es = (long long)nbn * N;
-    const double* e_prev = e_dev;  // the forcing the current orbit was solved with
     for (int p = 1; p < n_passes; ++p) {
         if (pass_orbits)
             ORBIT_HIP_CHECK(hipMemcpyAsync(pass_orbits + (size_t)(p - 1) * nt * os, orbit_dev, (size_t)nt * os * 8,
                                            hipMemcpyDeviceToDevice, st));
-        double* e_eff = inc ? ebuf[(p - 1) & 1] : ebuf[0];
         if (one_launch) {
             // every bin at once: the same per-(bin, tracer, level) tiles as a launch per bin
             const FctGrid& g0 = grids[0]->fg;
@@ -1227,10 +1223,8 @@ static int forward_fct_passes(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* c
         if (inc) {
             orbit_opts oi = o;
             oi.warm_orbit = 1;
-            pd.e_prev = e_prev;
             rc = solve_forward_pass(ctx, op, tracers, nt, e_eff, orbit_dev, &pd, &oi, orbit_dev,
                                     info + (size_t)p * nt);
-            e_prev = e_eff;
         } else {
             ORBIT_HIP_CHECK(
                 hipMemcpy2DAsync(c0, N * 8, orbit_dev, (size_t)os * 8, N * 8, nt, hipMemcpyDeviceToDevice, st));

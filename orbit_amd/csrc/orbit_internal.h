@@ -70,10 +70,10 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> owned;  // scratch owned by the context
-    // reusable scratch, grown on demand: 0-6 the periodic solve, 7 calibration
-    // and benchmarks, 8 the forward FCT driver's emissions, c0 and s~, 9 the
-    // tables of a group solve
-    DevBuf scratch[10];
+    // reusable scratch, grown on demand: 0-6 and 10-12 the periodic solve, 7
+    // calibration and benchmarks, 8 the forward FCT driver's emissions, c0 and
+    // s~, 9 the tables of a group solve
+    DevBuf scratch[13];
     // this device's copy of the fixed scratch array of the allocation-free
     // solve (its device address, looked up at context creation)
     void* fixed_scratch = nullptr;
@@ -187,16 +187,13 @@ int finalize_opset(orbit_ctx* ctx, orbit_opset* op);
 
 // Pass-to-pass state of forward solves whose forcing changes a little from one
 // solve to the next (the FCT deferred-correction passes), mixed precision.  The
-// first solve (e_prev null) leaves s~ (the FP32 forced sweep of its forcing) in
-// s_state.  A later solve, given the previous orbit as its warm whole orbit and
-// the forcing e_prev it was solved with, propagates dtau*(e - e_prev) through
-// the bins with the FP32 sweeps: previous states + those states start its first
-// accurate check, which then needs only the sweeps for the FP32 error of the
-// change (instead of a cold check), and s~ = s~_prev + the change's last state
-// (returned in s_state).
+// first solve (no previous orbit) leaves s~ (the FP32 forced sweep of its
+// forcing) in s_state.  A later solve, given the previous orbit as its warm
+// whole orbit, starts its states there: its first residual is that of the
+// previous orbit under the new forcing, and s~ = s~_prev + that residual's
+// periodic part (returned in s_state).
 struct PassDelta {
-    const double* e_prev = nullptr;  // [nt][n_bins][N] k-major, the previous forcing; null on the first solve
-    double* s_state = nullptr;       // [nt][n16] device layout, FP64: in s~ of e_prev, out s~ of e
+    double* s_state = nullptr;  // [nt][n16] device layout, FP64: in s~ of the previous pass, out s~ of this one
 };
 int solve_forward_pass(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, const double* e_dev,
                        const double* prev_orbit_dev, PassDelta* pd, const orbit_opts* opts, double* out_dev,

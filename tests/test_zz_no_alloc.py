@@ -53,7 +53,7 @@ def test_noalloc_matches_the_allocating_entry_point(sets, precision):
 
 def test_noalloc_refuses_more_than_the_fixed_array(sets):
     wl, op = sets[0]
-    tracers = list(range(6)) * 5  # 30 slots x restart 64 at 48x40x15: about 650 MB
+    tracers = list(range(6)) * 8  # 48 slots at 48x40x15 in mixed precision: about 1.3 GB
     e = torch.from_numpy(np.ascontiguousarray(wl.e[tracers])).cuda()
     rc, out, _ = _call(_lib.lib().orbit_solve_forward_dev_noalloc, op, tracers, e, "mixed", restart=64)
     assert rc == _lib.ORBIT_ERR_INVALID
