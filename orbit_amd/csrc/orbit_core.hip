@@ -245,6 +245,7 @@ struct SweepArgs {
     void* x;
     long long xstride;  // per slot
     const int* tmap;    // slot -> tracer
+    int tmap_identity;  // slot s is tracer s (no tmap load before the per-tracer rows)
     const int* active;  // nullable
     int own_zero;       // x of this colour is known to be zero (not read)
     int nbr_zero;       // x of the other colour is known to be zero (not read)
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     const size_t base = (size_t)col * 16 + lane;
     const XT* __restrict__ b = (const XT*)a.b + (size_t)ls * a.bstride;
     XT* __restrict__ x = (XT*)a.x + (size_t)ls * a.xstride;
-    const int tr = a.tmap[ls];
+    const int tr = a.tmap_identity ? ls : a.tmap[ls];
     int act = 1, cnt = 0x7fffffff;
     if (a.active) act = a.active[ls];
     if (a.counts) cnt = a.counts[ls];
@@ -1692,10 +1693,10 @@ struct CycArgs {
 };
 
 // grid (ceil(max ncol / 16), chunks, nbins), 256 threads: 16 columns x 16
-// levels.  A chunk is the slots that share one operator (every slot of a
-// single-set solve, one slot group of a group solve): the block reads a
-// column's block and stencil once (when the transport is shared) and loops
-// over the chunk's slots.  The 16 lanes of a column are one wavefront's, so
+// levels.  A chunk is up to four slots that share one operator (four
+// consecutive slots of a single-set solve, one slot group of a group solve):
+// the block reads a column's block and stencil once (when the transport is
+// shared) and loops over the chunk's slots.  The 16 lanes of a column are one wavefront's, so
 // the block product exchanges through LDS with wave-level ordering only.
 __global__ __launch_bounds__(256) void k_cyc_residual(CycArgs a) {
     __shared__ double sy[256];
@@ -1713,7 +1714,7 @@ __global__ __launch_bounds__(256) void k_cyc_residual(CycArgs a) {
         V = a.V[t]; H = a.H[t]; d = a.d[t];
         vst = a.vstride; hst = a.hstride; dst = a.dstride;
         nx = a.nx; ny = a.ny; px = a.px;
-        first = 0; count = a.nslots;
+        first = 4 * blockIdx.y; count = min(4, a.nslots - first);
     }
     const int ncol = nx * ny;
     if ((int)blockIdx.x * 16 >= ncol) return;  // uniform over the block
@@ -1847,6 +1848,7 @@ struct Batch {
     int* active_dev;   // slots still iterating inside a GMRES cycle
     int* part_dev;     // slots taking part in the current round
     const GroupSpec* grp = nullptr;  // a group solve's tables, else null
+    bool tmap_identity = false;      // slot s solves tracer s (single-set solves only)
 };
 
 // Algorithmic bytes of one half-sweep launch: per column of the colour the
@@ -2008,6 +2010,7 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
         a.x = x;
         a.xstride = xstride;
         a.tmap = B.tmap_dev;
+        a.tmap_identity = 0;
         a.active = mask;
         a.own_zero = own_zero;
         a.nbr_zero = nbr_zero;
@@ -2040,6 +2043,7 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     a.x = x;
     a.xstride = xstride;
     a.tmap = B.tmap_dev;
+    a.tmap_identity = B.tmap_identity ? 1 : 0;
     a.active = mask;
     a.own_zero = own_zero;
     a.nbr_zero = nbr_zero;
@@ -2464,7 +2468,7 @@ int cyc_residual(Solver& S, bool xzero) {
     a.xzero = xzero ? 1 : 0;
     a.active = S.B.part_dev;
     const int ncolmax = S.B.grp ? S.B.grp->ncolmax : op->ncol;
-    const int nchunks = S.B.grp ? S.B.grp->ngroups : 1;
+    const int nchunks = S.B.grp ? S.B.grp->ngroups : (S.B.nslots + 3) / 4;
     hipLaunchKernelGGL(k_cyc_residual, dim3((unsigned)((ncolmax + 15) / 16), nchunks, nbn), dim3(256), 0,
                        S.B.ctx->stream, a);
     ORBIT_HIP_CHECK(hipGetLastError());
@@ -2561,7 +2565,8 @@ int apply_M(Solver& S, const double* v, long long vslot, const void** res_out, b
         float* res = nullptr;
         if (S.use_graph) {
             float* wj = S.states ? S.W + (size_t)j * S.B.nslots * (S.nbins - 1) * n16 : nullptr;
-            GraphKey key{S.dir, S.states ? 3 : 2, S.inner, S.B.nslots, {}, S.fa, S.states ? (const void*)wj : S.fb};
+            GraphKey key{S.dir, (S.states ? 3 : 2) + (S.B.tmap_identity ? 4 : 0), S.inner, S.B.nslots, {}, S.fa,
+                         S.states ? (const void*)wj : S.fb};
             key.tracers = {(int)(intptr_t)S.B.tmap_dev, (int)((intptr_t)S.B.tmap_dev >> 32),
                            (int)(intptr_t)S.B.active_dev, (int)((intptr_t)S.B.active_dev >> 32),
                            (int)(intptr_t)S.fb, (int)((intptr_t)S.fb >> 32)};
@@ -2878,6 +2883,9 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     const long long os = (long long)(nbn + 1) * n16;
     dim3 g(grid1(n16).x, nt);
     std::vector<int> ones(nt, 1), tv(tracers, tracers + nt);
+    S.B.tmap_identity = !G;
+    for (int s = 0; s < nt; ++s)
+        if (tracers[s] != s) S.B.tmap_identity = false;
     { int rc0 = set_ints(st, S.B.tmap_dev, tv.data(), nt); if (rc0) return rc0; }
     { int rc0 = set_ints(st, S.B.part_dev, ones.data(), nt); if (rc0) return rc0; }
     { int rc0 = set_ints(st, S.B.active_dev, ones.data(), nt); if (rc0) return rc0; }
