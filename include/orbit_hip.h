@@ -31,7 +31,8 @@ extern "C" {
 #define ORBIT_ERR_NOMEM (-4)
 
 #define ORBIT_PREC_FP64 0   /* plain: FP64 inner solves, FP64 GMRES */
-#define ORBIT_PREC_MIXED 1  /* FP32 inner solves, BF16 column blocks, FP64 GMRES + refinement */
+#define ORBIT_PREC_MIXED 1  /* FP32 inner solves, BF16 column blocks, FP64 GMRES + refinement of
+                               every bin state against its exact FP64 residual */
 
 typedef struct orbit_ctx orbit_ctx;
 typedef struct orbit_opset orbit_opset;
@@ -43,26 +44,27 @@ typedef struct {
     int restart;         /* Krylov dimension per cycle (reference 30) */
     int precision;       /* ORBIT_PREC_FP64 or ORBIT_PREC_MIXED */
     int inner_sweeps;    /* red-black column sweeps per inexact P apply; 0 = opset default */
-    int acc_sweeps;      /* sweeps per accurate FP64 P apply; 0 = opset default */
+    int acc_sweeps;      /* sweeps per accurate FP64 P apply (plain FP64 path); 0 = opset default */
     int use_graph;       /* replay the inner sweeps of M from a captured HIP graph */
     int verbose;
-    int max_rounds;      /* mixed: refinement rounds cap (0 = maxiter) */
+    int max_rounds;      /* mixed: refinement rounds cap (0 = maxiter); a mixed round's GMRES
+                            cycle takes at most min(restart, 16) steps */
     int rhs_is_s;        /* 1: the input is s itself ([nt][N]) for (I-M)c0 = s (or
                             (I-M^T)lam0 = s_adj); states 1..n_bins are then the
                             unforced propagation of c0 */
     int warm_orbit;      /* 1: c_warm is a whole orbit [nt][n_bins+1][N] (an output
-                            of an earlier solve); mixed precision starts its first
-                            accurate sweep from those states */
-    int warm_sweeps;     /* sweeps of that first warm-started sweep (0: the cold count);
-                            enough for its states to reach 1e-12 relative */
+                            of an earlier solve); mixed precision starts every bin
+                            state there (FP64 takes its c0) */
+    int warm_sweeps;     /* unused (kept for the layout of this struct) */
 } orbit_opts;
 
 typedef struct {
     int gmres_iters;     /* total Arnoldi steps (reference pr_norm callback count) */
-    int rounds;          /* accurate residual evaluations after the first */
+    int rounds;          /* GMRES cycles (refinement rounds in mixed precision) */
     int converged;       /* 1 when ||r|| <= tol*||s|| */
     int zero_source;     /* 1 when ||s|| < 1e-30 (orbit is identically zero) */
-    double residual;     /* ||(I-M)c0 - s|| / ||s|| with the accurate M */
+    double residual;     /* ||(I-M)c0 - s|| / ||s||: FP64 sweeps to 1e-12 (plain FP64), or the
+                            exact FP64 bin residuals propagated by the FP32 sweeps (mixed) */
     double periodicity;  /* ||c8 - c0|| / (||c0|| + 1e-30)  (orbit.py:199-200) */
     double neg_frac;     /* fraction of negative entries over the 9 states (orbit.py:202-203) */
     double s_norm;
@@ -144,11 +146,11 @@ int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st);
  * reference (its direct factorisation has no iteration).  Not to be called
  * while a solve on the opset runs. */
 int orbit_opset_set_relaxation(orbit_ctx* ctx, orbit_opset* op, double omega);
-/* Block product of the reduced-precision sweeps (FP32 inner, FP64-vector
- * accurate; both with BF16 inverse blocks) of a shared-transport set: on = 1
- * multiplies the shared 16x16 column block by the four slots of a wave on the
- * matrix cores (v_mfma 16x16x4, FP32 or FP64); 0, the default, uses the VALU
- * loop (measured 2-3 % faster per launch at 64x70x15 with 12 slots).  Either way a tracer's result does not depend on its batch; the
+/* Block product of the reduced-precision sweeps (FP32 vectors and operator,
+ * BF16 inverse blocks) of a shared-transport set: on = 1 multiplies the shared
+ * 16x16 column block by the four slots of a wave on the matrix cores (v_mfma
+ * 16x16x4 FP32); 0, the default, uses the VALU loop (measured 1-3 % faster per
+ * headline step at 64x70x15 with 12 slots).  Either way a tracer's result does not depend on its batch; the
  * two settings differ in the last bits.  No counterpart in the reference.  Not
  * to be called while a solve on the opset runs. */
 int orbit_opset_set_mfma(orbit_opset* op, int on);
@@ -431,9 +433,8 @@ int orbit_fct_source_batch_dev(orbit_ctx* ctx, orbit_fct_grid* G, int nvec, cons
  * the passes of forward mode): pass 0 solves the periodic orbits of e; pass p
  * = 1..n_passes-1 adds d_AD of the previous orbit's end-of-bin states (limited,
  * grids[bin] of the operator set's shape and context) to e and solves again:
- * FP64 from the previous c0; mixed precision from the previous orbit plus the
- * FP32 propagation of the forcing's change (the solve is linear), so the first
- * accurate check of each later pass needs fewer sweeps than a cold one.  The
+ * FP64 from the previous c0; mixed precision starts every bin state from the
+ * previous orbit, so its first residual is that of the forcing's change.  The
  * passes converge to the same tolerance either way.  e_dev (nt, n_bins, N) and
  * ghost_dev (nullable, the same layout: the concentration beyond the lateral
  * edge per bin, the lateral boundary values of orbit/core/lateral_bc.py) are
@@ -485,8 +486,9 @@ int orbit_opset_check(orbit_ctx* ctx, orbit_opset* op, long long* counts);
  * one red-black half-sweep launch of `bin` on nslots tracer slots (tracers
  * 0..nslots-1), precision ORBIT_PREC_FP64 (FP64 blocks), ORBIT_PREC_MIXED (the
  * inner sweep of the mixed solve: FP32 vectors, BF16 blocks) or
- * ORBIT_SWEEP_ACC_MIXED (its accurate sweep: FP64 vectors and operator, BF16
- * blocks), measured with HIP events over `reps` back-to-back launches on the
+ * ORBIT_SWEEP_ACC_MIXED (FP64 vectors and operator with BF16 blocks; no solve
+ * runs this variant since the mixed path refines its states against exact FP64
+ * residuals), measured with HIP events over `reps` back-to-back launches on the
  * context's stream.  Also returns the algorithmic bytes one launch moves (each
  * operand read once, the written colour once). */
 #define ORBIT_SWEEP_ACC_MIXED 2

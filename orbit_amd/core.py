@@ -265,7 +265,8 @@ class OperatorSet:
 
         rhs="s": e is s itself, (nt, N), for (I - M)c0 = s.
         c_warm: (nt, N) start for c0, or a whole earlier orbit (nt, n_bins+1, N) whose
-        states also start the first accurate sweep (mixed) with warm_sweeps sweeps.
+        states start every bin state of the mixed refinement (FP64 takes its c0).
+        warm_sweeps: accepted for older callers, unused.
         Returns (orbit (nt, n_bins+1, N), info list).  e may be a numpy array
         (host) or a CUDA torch tensor (device path, no host copies)."""
         tracers = list(range(self.n_tracers)) if tracers is None else list(tracers)

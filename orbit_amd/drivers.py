@@ -45,8 +45,8 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
     native: with warm "c0", run every pass inside the library
     (orbit_forward_fct_dev, no return to Python between passes; a ghost list
     goes in as one tensor with per-tracer flags; FP64: the same launches as
-    the loop here; mixed: each later pass starts from the previous orbit plus
-    the FP32 propagation of the forcing's change, fewer accurate sweeps);
+    the loop here; mixed: each later pass starts every bin state from the
+    previous orbit and carries s~ over, so no sweep measures ||s|| again);
     with native=False the passes are driven from here.
     Returns (orbit (nt, n_bins+1, N), [infos per pass])."""
     ctx = ctx or op.ctx
@@ -59,9 +59,6 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
     kw["ctx"] = ctx
     orbit, info = op.solve_forward(e, tracers, tol=tol, precision=precision, maxiter=maxiter, **kw)
     hist = [info]
-    e_prev = e
-    rho = op.stats()["sweep_contraction"]
-    eps = min(1e-12, 0.01 * tol)
     groups = _ghost_groups(ghost, e)
     for _ in range(1, n_passes):
         e_eff = torch.empty_like(e)
@@ -78,24 +75,12 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
                                             ghost=None if gh is None else gh[:, tau].contiguous(), sync=True)
                 e_eff[:, tau].index_copy_(0, idx, out)
         if precision == "mixed" and warm == "orbit":
-            # The previous orbit starts the first accurate sweep.  Its states are
-            # off by at most dtau * sum_tau ||de_tau|| (the propagators contract),
-            # and each sweep shrinks that error by rho.
-            de = op.dtau * torch.linalg.vector_norm(e_eff - e_prev, dim=2).sum(dim=1)
-            cn = torch.linalg.vector_norm(orbit[:, 0], dim=1).clamp_min(1e-300)
-            rel = float((de / cn).max())
-            if rel == 0.0:
-                k = 2
-            elif rel < 1.0:
-                k = max(2, int(math.ceil(math.log(eps / rel) / math.log(rho))) + 1)
-            else:
-                k = 0  # large change (or nan): the cold sweep count
-            orbit, info = op.solve_forward(e_eff, tracers, c_warm=orbit, warm_sweeps=k,
-                                           tol=tol, precision=precision, maxiter=maxiter, **kw)
+            # the previous orbit starts every bin state of the refinement
+            orbit, info = op.solve_forward(e_eff, tracers, c_warm=orbit, tol=tol, precision=precision,
+                                           maxiter=maxiter, **kw)
         else:
             orbit, info = op.solve_forward(e_eff, tracers, c_warm=orbit[:, 0], tol=tol, precision=precision,
                                            maxiter=maxiter, **kw)
-        e_prev = e_eff
         hist.append(info)
     return orbit, hist
 

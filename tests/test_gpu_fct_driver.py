@@ -136,8 +136,8 @@ def test_per_tracer_ghosts_run_natively(month, precision, tol):
 
 
 def test_python_loop_warm_orbit_mode(month):
-    """warm="orbit" (the Python-driven passes, each started from the whole
-    previous orbit with the sweep count its forcing change bounds): converged
+    """warm="orbit" (the Python-driven passes, each started from every bin
+    state of the whole previous orbit): converged
     passes within the 1e-6 bar of the library's; and a solve warm-started from
     its own converged orbit stays on it."""
     wl, op, grids, e, _ = month
@@ -147,7 +147,7 @@ def test_python_loop_warm_orbit_mode(month):
     for t in range(e.shape[0]):
         assert _rel(a[t, 0], b[t, 0]) <= 1e-6, t
     c, ic = op.solve_forward(e, tol=1e-8, precision="mixed")
-    d, idd = op.solve_forward(e, c_warm=c, warm_sweeps=3, tol=1e-8, precision="mixed")
+    d, idd = op.solve_forward(e, c_warm=c, tol=1e-8, precision="mixed")
     for t in range(e.shape[0]):
         assert idd[t]["converged"] and idd[t]["gmres_iters"] <= ic[t]["gmres_iters"], t
         assert _rel(d[t], c[t]) <= 1e-7, t

@@ -1,10 +1,10 @@
 """The shared column block's product on the matrix cores (v_mfma 16x16x4 in
-FP32 for the inner sweeps, FP64 for the accurate ones, the four slots of a wave
-as B's columns) against the VALU loop it replaces: the same orbits to the
-solve tolerance in both directions, the residual measured by the FP64 accurate
-sweeps still the true one (oracle), and a tracer's result bit-identical alone
-and in any batch with the matrix cores on (every slot count uses the same
-four-slot mapping)."""
+FP32 for the mixed solve's sweeps, the four slots of a wave as B's columns)
+against the VALU loop it replaces: the same orbits to the solve tolerance in
+both directions, the residual the mixed solve reports (its exact FP64 bin
+residuals propagated by these sweeps) still the true one (oracle), and a
+tracer's result bit-identical alone and in any batch with the matrix cores on
+(every slot count uses the same four-slot mapping)."""
 import numpy as np
 import pytest
 import torch
