@@ -328,7 +328,8 @@ def main():
         # the operator set's calibrated sweeps: relaxation omega of the red-black
         # correction, measured contraction (with its 20% margin), sweep counts
         "sweeps": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in op.stats().items()
-                   if k in ("relaxation", "sweep_contraction", "default_acc_sweeps", "default_inner_sweeps", "mfma")},
+                   if k in ("relaxation", "sweep_contraction", "default_acc_sweeps", "default_inner_sweeps",
+                            "default_check_sweeps", "mfma")},
     }
 
     if not args.quick:

@@ -55,7 +55,8 @@ typedef struct {
     int warm_orbit;      /* 1: c_warm is a whole orbit [nt][n_bins+1][N] (an output
                             of an earlier solve); mixed precision starts every bin
                             state there (FP64 takes its c0) */
-    int warm_sweeps;     /* unused (kept for the layout of this struct) */
+    int check_sweeps;    /* mixed: sweeps per inner solve of the residual sweep (each
+                            refinement round's check and right-hand side); 0 = opset default */
 } orbit_opts;
 
 typedef struct {
@@ -79,6 +80,7 @@ typedef struct {
     int periodic_x;  /* 1 when the operators couple i = 0 and i = nx-1 (periodic longitude) */
     double relaxation;  /* omega of the sweeps' correction x += omega P (b - A x) */
     int mfma;           /* 1: BF16-inverse sweeps multiply the shared column block on the matrix cores */
+    int default_check_sweeps;  /* sweeps per inner solve of the mixed refinement's residual sweep */
 } orbit_opset_stats;
 
 /* Air-mass-balance statistics (reference orbit/core/operator.py:598-652). */

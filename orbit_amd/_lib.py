@@ -59,7 +59,7 @@ class Opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxiter", C.c_int), ("restart", C.c_int),
                 ("precision", C.c_int), ("inner_sweeps", C.c_int), ("acc_sweeps", C.c_int),
                 ("use_graph", C.c_int), ("verbose", C.c_int), ("max_rounds", C.c_int),
-                ("rhs_is_s", C.c_int), ("warm_orbit", C.c_int), ("warm_sweeps", C.c_int)]
+                ("rhs_is_s", C.c_int), ("warm_orbit", C.c_int), ("check_sweeps", C.c_int)]
 
 
 class Info(C.Structure):
@@ -82,7 +82,8 @@ class OpsetStats(C.Structure):
                 ("n_tracers", C.c_int), ("n_groups", C.c_int),
                 ("default_inner_sweeps", C.c_int), ("default_acc_sweeps", C.c_int),
                 ("dtau", C.c_double), ("device_bytes", C.c_double), ("sweep_contraction", C.c_double),
-                ("periodic_x", C.c_int), ("relaxation", C.c_double), ("mfma", C.c_int)]
+                ("periodic_x", C.c_int), ("relaxation", C.c_double), ("mfma", C.c_int),
+                ("default_check_sweeps", C.c_int)]
 
 
 class MbStats(C.Structure):

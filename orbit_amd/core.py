@@ -260,40 +260,41 @@ class OperatorSet:
     def solve_forward(self, e, tracers=None, c_warm=None, tol=1e-8, maxiter=200, restart=30,
                       precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
                       max_rounds=0, verbose=False, rhs="e", ctx: Optional[Context] = None,
-                      warm_sweeps=0):
+                      warm_sweeps=0, check_sweeps=0):
         """Periodic orbits of tracers for emissions e (nt, n_bins, N).
 
         rhs="s": e is s itself, (nt, N), for (I - M)c0 = s.
         c_warm: (nt, N) start for c0, or a whole earlier orbit (nt, n_bins+1, N) whose
         states start every bin state of the mixed refinement (FP64 takes its c0).
-        warm_sweeps: accepted for older callers, unused.
+        warm_sweeps: accepted for older callers, unused.  check_sweeps (mixed): sweeps per
+        inner solve of each refinement round's residual sweep (0: the opset's default).
         Returns (orbit (nt, n_bins+1, N), info list).  e may be a numpy array
         (host) or a CUDA torch tensor (device path, no host copies)."""
         tracers = list(range(self.n_tracers)) if tracers is None else list(tracers)
         opts = self._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps,
-                          use_graph, max_rounds, verbose, rhs)
+                          use_graph, max_rounds, verbose, rhs, check_sweeps)
         if c_warm is not None and len(c_warm.shape) == 3:
             opts.warm_orbit = 1
-            opts.warm_sweeps = int(warm_sweeps)
         dev = hasattr(e, "is_cuda") and e.is_cuda
         fn = lib().orbit_solve_forward_dev if dev else lib().orbit_solve_forward
         return self._solve(fn, tracers, e, c_warm, opts, dev, ctx)
 
     def solve_adjoint(self, R, tracers=None, lam_warm=None, tol=1e-8, maxiter=200, restart=30,
                       precision="fp64", inner_sweeps=0, acc_sweeps=0, use_graph=None,
-                      max_rounds=0, verbose=False, rhs="e", ctx: Optional[Context] = None):
+                      max_rounds=0, verbose=False, rhs="e", ctx: Optional[Context] = None,
+                      check_sweeps=0):
         """Adjoint orbits for receptors R (nt, n_bins, N); returns (lam (nt, n_bins+1, N), info).
 
         rhs="s": R is s_adj itself, (nt, N), for (I - M^T) lam0 = s_adj."""
         tracers = list(range(self.n_tracers)) if tracers is None else list(tracers)
         opts = self._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps,
-                          use_graph, max_rounds, verbose, rhs)
+                          use_graph, max_rounds, verbose, rhs, check_sweeps)
         dev = hasattr(R, "is_cuda") and R.is_cuda
         fn = lib().orbit_solve_adjoint_dev if dev else lib().orbit_solve_adjoint
         return self._solve(fn, tracers, R, lam_warm, opts, dev, ctx)
 
     @staticmethod
-    def _opts(tol, maxiter, restart, precision, inner, acc, use_graph, max_rounds, verbose, rhs="e"):
+    def _opts(tol, maxiter, restart, precision, inner, acc, use_graph, max_rounds, verbose, rhs="e", check=0):
         prec = {"fp64": _lib.ORBIT_PREC_FP64, "mixed": _lib.ORBIT_PREC_MIXED}[precision]
         if rhs not in ("e", "s"):
             raise ValueError("rhs must be 'e' or 's'")
@@ -302,7 +303,7 @@ class OperatorSet:
         return _lib.default_opts(tol=float(tol), maxiter=int(maxiter), restart=int(restart),
                                  precision=prec, inner_sweeps=int(inner), acc_sweeps=int(acc),
                                  use_graph=int(bool(use_graph)), max_rounds=int(max_rounds),
-                                 verbose=int(bool(verbose)), rhs_is_s=int(rhs == "s"))
+                                 verbose=int(bool(verbose)), rhs_is_s=int(rhs == "s"), check_sweeps=int(check))
 
     def mass_balance(self, rho: np.ndarray) -> dict:
         """Air-mass-balance row-sum statistics (orbit/core/operator.py:598-652)."""

@@ -1174,7 +1174,7 @@ extern "C" void orbit_opts_default(orbit_opts* o) {
     o->max_rounds = 0;
     o->rhs_is_s = 0;
     o->warm_orbit = 0;
-    o->warm_sweeps = 0;
+    o->check_sweeps = 0;
 }
 
 extern "C" int orbit_ctx_create(int device, orbit_ctx** out) {
@@ -1539,6 +1539,7 @@ extern "C" int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st) {
     st->periodic_x = op->px;
     st->relaxation = op->omega;
     st->mfma = op->mfma ? 1 : 0;
+    st->default_check_sweeps = op->def_check;
     return ORBIT_OK;
 }
 
@@ -1553,7 +1554,7 @@ extern "C" int orbit_opset_set_relaxation(orbit_ctx* ctx, orbit_opset* op, doubl
     hipSetDevice(ctx->device);
     const double old_w = op->omega;
     const double old_rho = op->rho;
-    const int old_acc = op->def_acc, old_inner = op->def_inner;
+    const int old_acc = op->def_acc, old_inner = op->def_inner, old_check = op->def_check;
     drop_graphs(op);  // captured sweeps carry the old relaxation
     int rc = calibrate_opset(ctx, op, omega);
     if (rc) {  // keep the opset usable as it was
@@ -1561,6 +1562,7 @@ extern "C" int orbit_opset_set_relaxation(orbit_ctx* ctx, orbit_opset* op, doubl
         op->rho = old_rho;
         op->def_acc = old_acc;
         op->def_inner = old_inner;
+        op->def_check = old_check;
     }
     return rc;
 }
@@ -1838,6 +1840,7 @@ struct GroupSpec {
     double* const* out_ptrs = nullptr;       // per slot: (n_bins + 1, N) output
     int* inner_cnt = nullptr;           // per slot sweep counts of its set (filled by the solve)
     int* acc_cnt = nullptr;
+    int* chk_cnt = nullptr;
 };
 
 struct Batch {
@@ -2169,17 +2172,21 @@ int measure_contraction(orbit_ctx* ctx, orbit_opset* op, int ns, int* tr, const 
 namespace {
 constexpr int CAL_SLOTS = 8;  // tracers sampled by the sweep calibration
 
-// Sweep counts from a measured contraction: 1e-12 for an accurate apply (20%
-// margin on the rate), 3e-5 for the inexact FP32 one.  The inexact apply sets
-// how much one refinement round gains, not the accuracy of the answer (every
-// round's residual is an accurate FP64 sweep).  Measured on the 64x70x15 FCT
-// passes: an inexact apply good to 1.6e-5 (4 sweeps at rho 0.063) or better
-// keeps the refinement rounds of a more accurate one at fewer sweeps (-6% step
-// time against 5 sweeps), while 9.5e-5 (4 sweeps at rho 0.099) adds rounds.
+// Sweep counts from a measured contraction: 1e-12 for an accurate apply (plain
+// FP64 path; 20% margin on the rate) and, for the mixed refinement, 5e-4 for an
+// FP32 monodromy apply inside GMRES and 1e-3 (with the margin) for the residual
+// sweep.  Neither sets the accuracy of the answer (every round's residual is
+// exact in FP64); the first sets how much one round gains, the second also how
+// honestly the residual is reported: its error is at most about n_bins times
+// the per-bin error of the sweep, 0.4 % of the residual at 1e-3.  Measured on
+// the 64x70x15 FCT passes (rho 0.078): 3 and 3 sweeps take 25.6 ms per step
+// (3 rounds in the first passes, 2 in the later ones) against 30.0 with 4 and 4
+// (2 rounds everywhere), 26.4 with 3 and 4, 25.7 with 2 and 3, 32.0 with 1 and 3.
 void set_sweep_counts(orbit_opset* op, double rmax) {
     op->rho = std::min(0.97, std::max(0.02, 1.2 * rmax));
     op->def_acc = std::min(80, std::max(4, (int)std::ceil(std::log(1e-12) / std::log(op->rho))));
-    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(3e-5) / std::log(std::min(0.97, rmax)))));
+    op->def_inner = std::min(40, std::max(2, (int)std::ceil(std::log(5e-4) / std::log(std::min(0.97, rmax)))));
+    op->def_check = std::min(40, std::max(2, (int)std::ceil(std::log(1e-3) / std::log(op->rho))));
 }
 }  // namespace
 
@@ -2299,6 +2306,7 @@ struct Solver {
     int prec = ORBIT_PREC_FP64;
     int dir = FWD;
     int inner = 6, acc = 14, m = 30;
+    int chk = 6;              // sweeps per inner solve of the mixed residual sweep
     bool use_graph = false;
     size_t n16 = 0;
     int nbins = 8;
@@ -2328,6 +2336,7 @@ struct Solver {
     SlotLen sl;               // per-slot lengths (group solves), else none
     const int* inner_cnt = nullptr;  // per-slot sweep counts (group solves), else the uniform ones
     const int* acc_cnt = nullptr;
+    const int* chk_cnt = nullptr;
     // orbit-state refinement (the mixed path): the states live in `orbit`
     bool states = false;
     double* Rc = nullptr;     // bin residuals [slot][nbins][n16]
@@ -2507,7 +2516,7 @@ int z_sweep(Solver& S) {
             epi.add = S.Rc + (size_t)(fwd ? t + 1 : t - 1) * n16;
             epi.as = es;
         }
-        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.inner, rhs, vs, out, last ? vs : zs, mask, false, S.inner_cnt,
+        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.chk, rhs, vs, out, last ? vs : zs, mask, false, S.chk_cnt,
                              last ? nullptr : &epi);
         if (rc) return rc;
     }
@@ -2776,17 +2785,20 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     // per slot (its set's calibration): sweep contraction, inner and accurate
     // sweep counts, the relative error of one inexact (FP32) P apply
     std::vector<double> rho_s(nt), eps_in_s(nt);
-    std::vector<int> inner_s(nt), acc_s(nt);
+    std::vector<int> inner_s(nt), acc_s(nt), chk_s(nt);
     for (int s = 0; s < nt; ++s) {
         const orbit_opset* os_ = op_of(s);
         rho_s[s] = os_->rho;
         inner_s[s] = o.inner_sweeps > 0 ? o.inner_sweeps : os_->def_inner;
-        eps_in_s[s] = std::pow(rho_s[s], inner_s[s]);
+        chk_s[s] = o.check_sweeps > 0 ? o.check_sweeps : os_->def_check;
+        // what one round can gain: the less accurate of the two inexact applies
+        eps_in_s[s] = std::pow(rho_s[s], std::min(inner_s[s], chk_s[s]));
         acc_s[s] = o.acc_sweeps > 0 ? o.acc_sweeps
                                     : std::min(80, std::max(4, (int)std::ceil(std::log(S.eps_acc) / std::log(rho_s[s]))));
     }
     S.inner = *std::max_element(inner_s.begin(), inner_s.end());
     S.acc = *std::max_element(acc_s.begin(), acc_s.end());
+    S.chk = *std::max_element(chk_s.begin(), chk_s.end());
     S.m = o.restart;
     S.use_graph = o.use_graph != 0 && mixed && !ctx->timing && !G;
     S.n16 = G ? G->n16max : op->n16;
@@ -2863,8 +2875,10 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
         S.sl = SlotLen{G->slen, (int)S.n16};
         S.inner_cnt = G->inner_cnt;
         S.acc_cnt = G->acc_cnt;
+        S.chk_cnt = G->chk_cnt;
         int rc0 = set_ints(ctx->stream, G->inner_cnt, inner_s.data(), nt);
         if (!rc0) rc0 = set_ints(ctx->stream, G->acc_cnt, acc_s.data(), nt);
+        if (!rc0) rc0 = set_ints(ctx->stream, G->chk_cnt, chk_s.data(), nt);
         if (rc0) return rc0;
     }
     // host staging: the context's pinned buffer; an arena solve (no allocation
@@ -3394,7 +3408,7 @@ extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* con
     const size_t o_gops = place(hg.size() * sizeof(GroupOp)), o_t0 = place(tabs[0].size() * 4),
                  o_t1 = place(tabs[1].size() * 4), o_slen = place((size_t)nt * 4), o_ncol = place((size_t)nt * 4),
                  o_nz = place((size_t)nt * 4), o_in = place((size_t)nt * 8), o_out = place((size_t)nt * 8),
-                 o_ic = place((size_t)nt * 4), o_ac = place((size_t)nt * 4);
+                 o_ic = place((size_t)nt * 4), o_ac = place((size_t)nt * 4), o_cc = place((size_t)nt * 4);
     std::vector<char> blob(off, 0);
     auto put = [&](size_t o0, const void* src, size_t bytes) { if (bytes) memcpy(blob.data() + o0, src, bytes); };
     put(o_gops, hg.data(), hg.size() * sizeof(GroupOp));
@@ -3420,6 +3434,7 @@ extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* con
     G.out_ptrs = (double* const*)(dev + o_out);
     G.inner_cnt = (int*)(dev + o_ic);
     G.acc_cnt = (int*)(dev + o_ac);
+    G.chk_cnt = (int*)(dev + o_cc);
     std::vector<orbit_info> inf(nt);
     rc = solve_periodic(ctx, G.slot_op[0], tmap.data(), nt, inp[0], nullptr, &o, outp[0], inf.data(),
                         adjoint ? ADJ : FWD, nullptr, nullptr, 0, &G);

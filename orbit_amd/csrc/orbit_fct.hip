@@ -1162,7 +1162,6 @@ static int forward_fct_passes(orbit_ctx* ctx, orbit_opset* op, orbit_fct_grid* c
     orbit_opts_default(&o);
     if (opts) o = *opts;
     o.warm_orbit = 0;
-    o.warm_sweeps = 0;
     hipSetDevice(ctx->device);
     hipStream_t st = ctx->stream;
     const bool inc = o.precision == ORBIT_PREC_MIXED && n_passes > 1;

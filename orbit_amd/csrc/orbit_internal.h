@@ -149,7 +149,7 @@ struct Opset {
     int px = 0;  // periodic x: the operators couple i = 0 and i = nx - 1
     double dtau = 0.0;
     bool has_tdiag = false;
-    int def_inner = 6, def_acc = 14;
+    int def_inner = 6, def_acc = 14, def_check = 6;
     double rho = 0.1;  // calibrated contraction of one red-black sweep (with margin)
     double omega = 1.0;  // relaxation of the sweeps' correction (chosen by the calibration)
     bool mfma = false;   // block product of the BF16-inverse sweeps on the matrix cores (opt-in)

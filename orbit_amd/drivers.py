@@ -87,7 +87,7 @@ def forward_fct(op: OperatorSet, fct_grids: Sequence[FctGrid], e, tracers=None, 
 
 def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, ghost, maxiter, ctx,
                         restart=30, inner_sweeps=0, acc_sweeps=0, use_graph=None, max_rounds=0, verbose=False,
-                        keep_passes=False):
+                        keep_passes=False, check_sweeps=0):
     import ctypes as C
     import numpy as np
     from . import _lib
@@ -116,7 +116,8 @@ def _forward_fct_native(op, fct_grids, e, tracers, n_passes, tol, precision, gho
         if tuple(ghost.shape) != tuple(e.shape):
             raise ValueError("ghost must have the shape of e")
         ghost = ghost.to(e.device, torch.float64).contiguous()
-    opts = op._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps, use_graph, max_rounds, verbose)
+    opts = op._opts(tol, maxiter, restart, precision, inner_sweeps, acc_sweeps, use_graph, max_rounds, verbose,
+                    check=check_sweeps)
     tr = np.ascontiguousarray(tracers, dtype=np.int32)
     grids = (C.c_void_p * op.n_bins)(*[g.handle.value for g in fct_grids])
     info = (_lib.Info * (n_passes * nt))()
