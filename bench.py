@@ -219,7 +219,7 @@ def main():
     variants = [tab[i] for i in range(min(nvar.value, 32))]
     sweep_ms = sum(v.total_ms for v in variants)
     sweep_bytes = sum(v.total_bytes for v in variants)
-    log(f"instrumented step: {t_instr * 1e3:.1f} ms, {sum(v.launches for v in variants)} half-sweeps, "
+    log(f"instrumented step: {t_instr * 1e3:.1f} ms, {sum(v.launches for v in variants)} sweep launches, "
         f"{sweep_ms:.2f} ms of sweep kernels, {sweep_bytes / 1e9:.1f} GB algorithmic, {untimed.value} untimed")
 
     def timed_variants(fn):
@@ -229,7 +229,8 @@ def main():
         t = (_lib.SweepTiming * 32)()
         n, u = C.c_int(), C.c_longlong()
         _lib.check(_lib.lib().orbit_timing_stop(ctx.handle, t, 32, C.byref(n), C.byref(u)))
-        return {f"prec{t[i].precision}_cs{t[i].slots_per_wave}_mf{t[i].mfma}_oz{t[i].own_zero}_nz{t[i].nbr_zero}":
+        return {f"prec{t[i].precision}_cs{t[i].slots_per_wave}_mf{t[i].mfma}_oz{t[i].own_zero}_nz{t[i].nbr_zero}"
+                + ("_first" if t[i].first_sweep else ""):
                 round(1e3 * t[i].total_ms / t[i].launches, 3) for i in range(min(n.value, 32))}
 
     # the other block product (matrix cores against the VALU loop): the same
@@ -420,7 +421,7 @@ def main():
         ng, ug = C.c_int(), C.c_longlong()
         _lib.check(_lib.lib().orbit_timing_stop(ctx_t.handle, tg, 32, C.byref(ng), C.byref(ug)))
         gl = [tg[i] for i in range(min(ng.value, 32))]
-        grp_sweeps = {"half_sweeps": int(sum(v.launches for v in gl)), "kernel_ms": round(sum(v.total_ms for v in gl), 3),
+        grp_sweeps = {"sweep_launches": int(sum(v.launches for v in gl)), "kernel_ms": round(sum(v.total_ms for v in gl), 3),
                       "us_per_launch": {f"prec{v.precision}_oz{v.own_zero}_nz{v.nbr_zero}":
                                         round(1e3 * v.total_ms / v.launches, 3) for v in gl},
                       "achieved_gbs": round(sum(v.total_bytes for v in gl) / (sum(v.total_ms for v in gl) * 1e-3) / 1e9, 1)}
@@ -482,11 +483,15 @@ def main():
     def vname(v):
         base = {_lib.ORBIT_PREC_FP64: "fp64", _lib.ORBIT_SWEEP_ACC_MIXED: "accurate",
                 _lib.ORBIT_PREC_MIXED: "inner"}[v.precision]
+        if v.first_sweep:
+            return base + "_first"
         return base + ("_ownzero" if v.own_zero else "") + ("_nbrzero" if v.nbr_zero else "")
 
     def kname(v):
         xt = {_lib.ORBIT_PREC_FP64: "double,double,double,double", _lib.ORBIT_SWEEP_ACC_MIXED:
               "double,double,unsigned short,double", _lib.ORBIT_PREC_MIXED: "float,float,unsigned short,float"}
+        if v.first_sweep:
+            return "k_first_sweep (both colours of the first sweep from x = 0)"
         return (f"k_halfsweep<{xt[v.precision]},{1 if v.slots_per_wave == 4 else 0},"
                 f"{'true' if v.own_zero else 'false'},{'true' if v.nbr_zero else 'false'},"
                 f"{'true' if v.mfma else 'false'},false>")
@@ -512,7 +517,7 @@ def main():
 
     out["roofline"] = roof(variants[0])
     out["roofline_other"] = [roof(v) for v in variants[1:4]]
-    out["step_sweeps"] = {"half_sweeps": int(sum(v.launches for v in variants)),
+    out["step_sweeps"] = {"sweep_launches": int(sum(v.launches for v in variants)),
                           "kernel_ms": round(sweep_ms, 3), "algorithmic_gb": round(sweep_bytes / 1e9, 3),
                           "achieved_gbs_over_step": round(sweep_bytes / (ms_per_step * 1e-3) / 1e9, 1),
                           "frac_over_step": round(sweep_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}

@@ -515,6 +515,7 @@ typedef struct {
     long long launches;
     double total_ms;
     double total_bytes;
+    int first_sweep;     /* 1: the first sweep from x = 0, both colours in one launch */
 } orbit_sweep_timing;
 int orbit_timing_start(orbit_ctx* ctx, int max_launches);
 int orbit_timing_stop(orbit_ctx* ctx, orbit_sweep_timing* out, int max_out, int* n_out, long long* untimed);

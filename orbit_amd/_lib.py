@@ -74,7 +74,7 @@ class Info(C.Structure):
 class SweepTiming(C.Structure):
     _fields_ = [("precision", C.c_int), ("slots_per_wave", C.c_int), ("mfma", C.c_int), ("own_zero", C.c_int),
                 ("nbr_zero", C.c_int), ("launches", C.c_longlong), ("total_ms", C.c_double),
-                ("total_bytes", C.c_double)]
+                ("total_bytes", C.c_double), ("first_sweep", C.c_int)]
 
 
 class OpsetStats(C.Structure):

@@ -545,6 +545,108 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     }
 }
 
+// The first sweep of an FP32 inner solve (x = 0, two colours) in one launch.
+// Red: x_r = omega P_r b_r needs no neighbour and no column block, so a black
+// column recomputes the red values of its four neighbours itself (the same
+// arithmetic, hence the same bits as the values the red blocks store) and
+// then x_b = omega P_b (b_b - H x_r): the two half-sweeps of the two-launch
+// form with no kernel boundary between them.  Blocks [0, nred) take red
+// tiles, the rest black ones; a wave is one column x four slots; tiles are
+// XCD-contiguous per colour.  Every load is in bounds (missing neighbours
+// read this column and are masked); inactive slots compute but do not store.
+struct FirstSweepArgs {
+    int ncols[2];         // columns of each colour
+    int nx, ny, px;
+    int tiles_per_xcd[2];
+    int nred;             // blocks of the red part (8 * tiles_per_xcd[0] * slot groups)
+    const float* H;       // stencil (shared; transposed for the adjoint)
+    const uint16_t* P;    // BF16 inverse blocks, + tracer * pstride
+    long long pstride;
+    const float* b;
+    long long bstride;
+    float* x;
+    long long xstride;
+    const int* tmap;
+    int tmap_identity;
+    const int* active;
+    int nslots;
+    float omega;
+};
+
+__global__ __launch_bounds__(256) void k_first_sweep(FirstSweepArgs a) {
+    __shared__ float sr[256];
+    const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int black = (int)blockIdx.x >= a.nred;
+    const int L = black ? (int)blockIdx.x - a.nred : (int)blockIdx.x;
+    const int nsg = (a.nslots + 3) >> 2;
+    int tile, sg;
+    xcd_tile_slot(L, nsg, a.tiles_per_xcd[black], tile, sg);
+    const int ncols = a.ncols[black];
+    if (tile * 4 >= ncols) return;  // uniform over the block
+    const int slot = sg * 4 + (grp & 3);
+    const int idx = tile * 4 + (grp >> 2);
+    const bool slot_ok = slot < a.nslots;
+    const int ls = slot_ok ? slot : a.nslots - 1;
+    const bool valid = idx < ncols && slot_ok;
+    int col, i, j;
+    colour_col(idx < ncols ? idx : 0, black, a.nx, a.nx, col, i, j);
+    const float* __restrict__ b = a.b + (size_t)ls * a.bstride;
+    const int tr = a.tmap_identity ? ls : a.tmap[ls];
+    const uint16_t* __restrict__ P = a.P + (size_t)tr * a.pstride;
+    const bool on = valid & (!a.active || a.active[ls] != 0);
+    float* sg_r = sr + (grp << 4);
+    // omega P_c v for the 16 levels of column c from this lane's row of P_c and
+    // one value of v a lane (level k): the red half-sweep's arithmetic from x = 0
+    auto load_row = [&](int c, uint16_t* row) {
+        const uint16_t* Pr = P + (size_t)c * 256 + lane * 16;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) row[m] = Pr[m];
+    };
+    auto apply_row = [&](const uint16_t* row, float v) {
+        wave_sync();  // earlier reads of sr are done
+        sg_r[lane] = v;
+        wave_sync();
+        float dx = 0;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) dx += (float)ldv(row[m]) * sg_r[m];
+        return (float)(0.0f + a.omega * dx);
+    };
+    uint16_t pc[16];
+    load_row(col, pc);
+    const float bv = b[(size_t)col * 16 + lane];
+    if (!black) {
+        const float xr = apply_row(pc, bv);
+        if (on) a.x[(size_t)ls * a.xstride + (size_t)col * 16 + lane] = xr;
+        return;
+    }
+    // every load of the black column issued before the first product
+    const int cw = west_col(col, i, a.nx, a.px), ce = east_col(col, i, a.nx, a.px);
+    const int cs = j > 0 ? col - a.nx : col, cn = j < a.ny - 1 ? col + a.nx : col;
+    const int ncw = cw >= 0 ? cw : col, nce = ce >= 0 ? ce : col;
+    uint16_t pw[16], pe[16], ps[16], pn[16];
+    load_row(ncw, pw);
+    load_row(nce, pe);
+    load_row(cs, ps);
+    load_row(cn, pn);
+    const float* Hc = a.H + (size_t)col * 64 + lane;
+    const float hw = Hc[0], he = Hc[16], hs = Hc[32], hn = Hc[48];
+    const float bw = b[(size_t)ncw * 16 + lane], be = b[(size_t)nce * 16 + lane];
+    const float bs = b[(size_t)cs * 16 + lane], bn = b[(size_t)cn * 16 + lane];
+    const float xw = apply_row(pw, bw);
+    const float xe = apply_row(pe, be);
+    const float xs = apply_row(ps, bs);
+    const float xn = apply_row(pn, bn);
+    float r = bv;
+    float hx = 0;
+    hx += hw * (cw >= 0 ? xw : 0.0f);
+    hx += he * (ce >= 0 ? xe : 0.0f);
+    hx += hs * (j > 0 ? xs : 0.0f);
+    hx += hn * (j < a.ny - 1 ? xn : 0.0f);
+    r -= hx;
+    const float xb = apply_row(pc, r);
+    if (on) a.x[(size_t)ls * a.xstride + (size_t)col * 16 + lane] = xb;
+}
+
 // y = A x (one operator apply, FP64), used for residual checks of the inner solves.
 __global__ __launch_bounds__(256) void k_apply_A(const double* __restrict__ V, long long vstride,
                                                  const double* __restrict__ H, long long hstride,
@@ -2102,6 +2204,57 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
     return launch_sweep_kernel<float, float, uint16_t, float>(B.ctx, prec, a, sh, own_zero, nbr_zero, B.ctx->stream, mf);
 }
 
+// The first FP32 sweep from x = 0 of a two-colour, shared-transport set in one
+// launch (k_first_sweep): the same bits as its two half-sweeps.
+int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long long bstride, void* x,
+                       long long xstride, const int* mask) {
+    orbit_opset* op = B.op;
+    const BinOps& O = op->bins[bin];
+    const bool T = dir == ADJ;
+    FirstSweepArgs a;
+    a.ncols[0] = op->ncols_c[0];
+    a.ncols[1] = op->ncols_c[1];
+    a.nx = op->nx;
+    a.ny = op->ny;
+    a.px = op->px;
+    const int nsg = (B.nslots + 3) / 4;
+    for (int c = 0; c < 2; ++c) a.tiles_per_xcd[c] = ((a.ncols[c] + 3) / 4 + 7) / 8;
+    a.nred = 8 * a.tiles_per_xcd[0] * nsg;
+    a.H = T ? O.HT32 : O.H32;
+    a.P = T ? O.PT16 : O.P16;
+    a.pstride = (long long)op->ncol * 256;
+    a.b = (const float*)rhs;
+    a.bstride = bstride;
+    a.x = (float*)x;
+    a.xstride = xstride;
+    a.tmap = B.tmap_dev;
+    a.tmap_identity = B.tmap_identity ? 1 : 0;
+    a.active = mask;
+    a.nslots = B.nslots;
+    a.omega = (float)op->omega;
+    const dim3 grid((unsigned)(a.nred + 8 * a.tiles_per_xcd[1] * nsg)), block(256);
+    hipStream_t st = B.ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (SweepTiming* tm = B.ctx->timing) {
+        if (tm->next + 2 <= tm->ev.size()) {
+            e0 = tm->ev[tm->next];
+            e1 = tm->ev[tm->next + 1];
+            tm->next += 2;
+            // each operand once: both colours' P, b and x written, the black columns' stencil
+            const double bytes = halfsweep_bytes(a.ncols[0], B.nslots, 4, 2, 4, true, true) +
+                                 halfsweep_bytes(a.ncols[1], B.nslots, 4, 2, 4, true, true) + 256.0 * a.ncols[1];
+            tm->recs.push_back({FUSED_FIRST_VARIANT + SP_F32 * 16 + 4 + 3, bytes});
+        } else {
+            ++tm->dropped;
+        }
+    }
+    if (e0)
+        hipExtLaunchKernelGGL(k_first_sweep, grid, block, 0, st, e0, e1, 0, a);
+    else
+        hipLaunchKernelGGL(k_first_sweep, grid, block, 0, st, a);
+    return ORBIT_OK;
+}
+
 // x = P_bin(rhs) approximately: `sweeps` red-black sweeps starting from x = 0.
 // One full sweep: every colour in turn.  first: the sweep starts from x = 0
 // (x is not read: own-zero for every colour, neighbour-zero for the first).
@@ -2114,6 +2267,9 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
                const Epi* epi = nullptr, int nsweeps = 0) {
     orbit_opset* op = B.op;
     const int ncolours = B.grp ? 2 : op->ncolours;
+    if (first && !B.grp && op->ncolours == 2 && prec == SP_F32 && !counts && op->ngroups == 1 &&
+        (!epi || nsweeps > 1))
+        return launch_first_sweep(B, bin, dir, rhs, bstride, x, xstride, mask);
     if (!B.grp && first && op->ncolours == 4) {
         const dim3 g((unsigned)((op->ny * 16 + 255) / 256), B.nslots);
         if (prec == SP_F32)
@@ -3631,13 +3787,15 @@ extern "C" int orbit_timing_stop(orbit_ctx* ctx, orbit_sweep_timing* out, int ma
         const int v = tm->recs[i].variant;
         auto it = agg.find(v);
         if (it == agg.end()) {
-            const int p = v / 16;  // internal sweep precision -> ABI constant
+            const int vf = v >= FUSED_FIRST_VARIANT ? v - FUSED_FIRST_VARIANT : v;
+            const int p = vf / 16;  // internal sweep precision -> ABI constant
             orbit_sweep_timing t{};
             t.precision = p == SP_F64 ? ORBIT_PREC_FP64 : p == SP_ACC_MIXED ? ORBIT_SWEEP_ACC_MIXED : ORBIT_PREC_MIXED;
-            t.mfma = (v & 8) ? 1 : 0;
-            t.slots_per_wave = (v & 4) ? 4 : 1;
-            t.own_zero = (v & 2) ? 1 : 0;
-            t.nbr_zero = (v & 1) ? 1 : 0;
+            t.mfma = (vf & 8) ? 1 : 0;
+            t.slots_per_wave = (vf & 4) ? 4 : 1;
+            t.own_zero = (vf & 2) ? 1 : 0;
+            t.nbr_zero = (vf & 1) ? 1 : 0;
+            t.first_sweep = v >= FUSED_FIRST_VARIANT ? 1 : 0;
             it = agg.emplace(v, t).first;
         }
         it->second.launches += 1;

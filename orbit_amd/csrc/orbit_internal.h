@@ -55,9 +55,11 @@ struct DevBuf {
 // every half-sweep launch on the context's stream records its own start and
 // stop events (hipExtLaunchKernelGGL) from a pool made beforehand, with its
 // variant and algorithmic bytes; solves do not replay captured graphs then.
+// Timing variant offset of the first sweep from x = 0 with both colours in one launch.
+constexpr int FUSED_FIRST_VARIANT = 64;
 struct SweepTiming {
     struct Rec {
-        int variant;   // prec * 16 + mfma * 8 + cs * 4 + own_zero * 2 + nbr_zero
+        int variant;   // prec * 16 + mfma * 8 + cs * 4 + own_zero * 2 + nbr_zero (+ FUSED_FIRST_VARIANT)
         double bytes;  // algorithmic bytes of the launch
     };
     std::vector<hipEvent_t> ev;  // 2 per launch
