@@ -574,7 +574,7 @@ struct FirstSweepArgs {
 };
 
 __global__ __launch_bounds__(256) void k_first_sweep(FirstSweepArgs a) {
-    __shared__ float sr[256];
+    __shared__ float sr[5][256];  // the four neighbours' right-hand sides, then this column's residual
     const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
     const int black = (int)blockIdx.x >= a.nred;
     const int L = black ? (int)blockIdx.x - a.nred : (int)blockIdx.x;
@@ -594,28 +594,28 @@ __global__ __launch_bounds__(256) void k_first_sweep(FirstSweepArgs a) {
     const int tr = a.tmap_identity ? ls : a.tmap[ls];
     const uint16_t* __restrict__ P = a.P + (size_t)tr * a.pstride;
     const bool on = valid & (!a.active || a.active[ls] != 0);
-    float* sg_r = sr + (grp << 4);
-    // omega P_c v for the 16 levels of column c from this lane's row of P_c and
-    // one value of v a lane (level k): the red half-sweep's arithmetic from x = 0
+    const int g16 = grp << 4;
+    // this lane's row (level k) of an inverse block
     auto load_row = [&](int c, uint16_t* row) {
         const uint16_t* Pr = P + (size_t)c * 256 + lane * 16;
 #pragma unroll
         for (int m = 0; m < 16; ++m) row[m] = Pr[m];
     };
-    auto apply_row = [&](const uint16_t* row, float v) {
-        wave_sync();  // earlier reads of sr are done
-        sg_r[lane] = v;
-        wave_sync();
+    // omega P v (v one value a lane, exchanged through LDS region q): the red
+    // half-sweep's arithmetic from x = 0
+    auto dot_row = [&](const uint16_t* row, int q) {
         float dx = 0;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) dx += (float)ldv(row[m]) * sg_r[m];
+        for (int m = 0; m < 16; ++m) dx += (float)ldv(row[m]) * sr[q][g16 + m];
         return (float)(0.0f + a.omega * dx);
     };
     uint16_t pc[16];
     load_row(col, pc);
     const float bv = b[(size_t)col * 16 + lane];
     if (!black) {
-        const float xr = apply_row(pc, bv);
+        sr[4][threadIdx.x] = bv;
+        wave_sync();
+        const float xr = dot_row(pc, 4);
         if (on) a.x[(size_t)ls * a.xstride + (size_t)col * 16 + lane] = xr;
         return;
     }
@@ -630,12 +630,16 @@ __global__ __launch_bounds__(256) void k_first_sweep(FirstSweepArgs a) {
     load_row(cn, pn);
     const float* Hc = a.H + (size_t)col * 64 + lane;
     const float hw = Hc[0], he = Hc[16], hs = Hc[32], hn = Hc[48];
-    const float bw = b[(size_t)ncw * 16 + lane], be = b[(size_t)nce * 16 + lane];
-    const float bs = b[(size_t)cs * 16 + lane], bn = b[(size_t)cn * 16 + lane];
-    const float xw = apply_row(pw, bw);
-    const float xe = apply_row(pe, be);
-    const float xs = apply_row(ps, bs);
-    const float xn = apply_row(pn, bn);
+    sr[0][threadIdx.x] = b[(size_t)ncw * 16 + lane];
+    sr[1][threadIdx.x] = b[(size_t)nce * 16 + lane];
+    sr[2][threadIdx.x] = b[(size_t)cs * 16 + lane];
+    sr[3][threadIdx.x] = b[(size_t)cn * 16 + lane];
+    wave_sync();
+    // the four red neighbours' values, as the red blocks compute them
+    const float xw = dot_row(pw, 0);
+    const float xe = dot_row(pe, 1);
+    const float xs = dot_row(ps, 2);
+    const float xn = dot_row(pn, 3);
     float r = bv;
     float hx = 0;
     hx += hw * (cw >= 0 ? xw : 0.0f);
@@ -643,7 +647,9 @@ __global__ __launch_bounds__(256) void k_first_sweep(FirstSweepArgs a) {
     hx += hs * (j > 0 ? xs : 0.0f);
     hx += hn * (j < a.ny - 1 ? xn : 0.0f);
     r -= hx;
-    const float xb = apply_row(pc, r);
+    sr[4][threadIdx.x] = r;
+    wave_sync();
+    const float xb = dot_row(pc, 4);
     if (on) a.x[(size_t)ls * a.xstride + (size_t)col * 16 + lane] = xb;
 }
 
@@ -1186,7 +1192,8 @@ __global__ __launch_bounds__(256) void k_arnoldi_tail(GState* st, const double* 
             if (blockIdx.x == 0) {
                 GState& S = st[slot];
                 double* col = S.H + (size_t)j * MAXM1;
-                for (int i = 0; i <= j; ++i) col[i] = h1[(size_t)slot * MAXM1 + i] + h2[(size_t)slot * MAXM1 + i];
+                for (int i = 0; i <= j; ++i)
+                    col[i] = h1[(size_t)slot * MAXM1 + i] + (h2 ? h2[(size_t)slot * MAXM1 + i] : 0.0);
                 const double hn = hn_s;
                 col[j + 1] = hn;
                 S.hnext = hn;
@@ -2849,15 +2856,21 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
             hipLaunchKernelGGL((k_mdot_sub<double>), dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, vj,
                                kslot, (const double*)res, vs, S.w, vs, (int)n16, nb, S.partials, S.B.active_dev,
                                S.sl);
-        hipLaunchKernelGGL(k_mupdate_fin, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.partials,
-                           S.h1, S.w, vs, (int)n16, nb, (double*)nullptr, S.B.active_dev, S.sl);
-        hipLaunchKernelGGL(k_mdot, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.w, vs,
-                           (int)n16, nb, S.partials, S.B.active_dev, S.sl);
+        // CGS2 (the plain FP64 path); one pass for the mixed refinement, whose
+        // short cycles stay orthogonal enough and whose every round's residual
+        // is recomputed exactly
         double* np = S.partials + (size_t)ns * MAXM1 * nb;
         hipLaunchKernelGGL(k_mupdate_fin, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.partials,
-                           S.h2, S.w, vs, (int)n16, nb, np, S.B.active_dev, S.sl);
-        hipLaunchKernelGGL(k_arnoldi_tail, g, dim3(256), 0, st, S.gst, S.h1, S.h2, np, nb, j, S.B.active_dev,
-                           S.resid, S.w, vs, S.Vk + (size_t)(j + 1) * n16, kslot, (int)n16, S.sl, fcopy, vs);
+                           S.h1, S.w, vs, (int)n16, nb, S.states ? np : (double*)nullptr, S.B.active_dev, S.sl);
+        if (!S.states) {
+            hipLaunchKernelGGL(k_mdot, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1, S.w, vs,
+                               (int)n16, nb, S.partials, S.B.active_dev, S.sl);
+            hipLaunchKernelGGL(k_mupdate_fin, dim3(nb, ns), dim3(RED_THREADS), 0, st, S.Vk, kslot, j + 1,
+                               S.partials, S.h2, S.w, vs, (int)n16, nb, np, S.B.active_dev, S.sl);
+        }
+        hipLaunchKernelGGL(k_arnoldi_tail, g, dim3(256), 0, st, S.gst, S.h1, S.states ? (const double*)nullptr : S.h2,
+                           np, nb, j, S.B.active_dev, S.resid, S.w, vs, S.Vk + (size_t)(j + 1) * n16, kslot,
+                           (int)n16, S.sl, fcopy, vs);
         ORBIT_HIP_CHECK(hipGetLastError());
         ORBIT_HIP_CHECK(hipMemcpyAsync(S.host, S.resid, ns * 8, hipMemcpyDeviceToHost, st));
         ORBIT_HIP_CHECK(hipStreamSynchronize(st));

@@ -38,15 +38,15 @@ __device__ __forceinline__ double rho_at(const FctGrid& g, size_t n) {
     return (p > 0.0 && z > 0.0) ? p / (kGrav * z) : 1.0;
 }
 
+// van Leer limiter phi(r) = (r + |r|) / (1 + |r|) of the ratio r = num / den of
+// successive differences (with the reference's guard: r = 1 or 1e6 when
+// |den| < 1e-30).  With one division: phi = 0 for r <= 0 and 2r / (1 + r) =
+// 2 num / (num + den) for r > 0 (num and den of one sign, so the sum does not
+// cancel); the same value to round-off.
 __device__ __forceinline__ double vl_phi(double up, double far, double down) {
     const double num = up - far, den = down - up;
-    double r;
-    if (fabs(den) < kEps)
-        r = fabs(num) < kEps ? 1.0 : 1.0e6;
-    else
-        r = num / den;
-    const double a = fabs(r);
-    return (r + a) / (1.0 + a);
+    if (fabs(den) < kEps) return fabs(num) < kEps ? 1.0 : 2.0e6 / (1.0 + 1.0e6);
+    return ((num > 0.0 && den > 0.0) || (num < 0.0 && den < 0.0)) ? 2.0 * num / (num + den) : 0.0;
 }
 
 struct Face {
@@ -322,9 +322,9 @@ __global__ __launch_bounds__(256) void k_fct_R(FctGrid g, const double* __restri
             add(f.valid ? -f.tr_lo * f.A * f.inv : 0.0);
         }
     }
-    const double qp = (xmax - xc) / dt, qm = (xc - xmin) / dt;
-    Rp[n] = pp > 0.0 ? fmin(1.0, qp / pp) : 0.0;
-    Rm[n] = pm > 0.0 ? fmin(1.0, qm / pm) : 0.0;
+    // R+- = min(1, Q+- / P+-) with Q+- = (x_max - x) / dt, (x - x_min) / dt: one division
+    Rp[n] = pp > 0.0 ? fmin(1.0, (xmax - xc) / (dt * pp)) : 0.0;
+    Rm[n] = pm > 0.0 ? fmin(1.0, (xc - xmin) / (dt * pm)) : 0.0;
 }
 
 __device__ __forceinline__ double face_C(const Face& f, double Rp_hi, double Rm_hi, double Rp_lo,
@@ -474,9 +474,8 @@ __device__ __forceinline__ void fct_tile(const FctGrid& g, const FctGeo& geo, in
                     add(f.valid ? -f.tr_lo * f.A * f.inv : 0.0);
                 }
             }
-            const double qp = (xmax - xc) / dt, qm = (xc - xmin) / dt;
-            sRp[rj][ri] = pp > 0.0 ? fmin(1.0, qp / pp) : 0.0;
-            sRm[rj][ri] = pm > 0.0 ? fmin(1.0, qm / pm) : 0.0;
+            sRp[rj][ri] = pp > 0.0 ? fmin(1.0, (xmax - xc) / (dt * pp)) : 0.0;
+            sRm[rj][ri] = pm > 0.0 ? fmin(1.0, (xc - xmin) / (dt * pm)) : 0.0;
         }
         __syncthreads();
     }
