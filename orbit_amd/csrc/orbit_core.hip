@@ -263,11 +263,13 @@ struct SweepArgs {
     const int* btab;    // group << 16 | tile, -1 for a block with no work
     // epilogue of a slot's last sweep (epi_out null: none): at the colour's
     // columns epi_out = x + epi_alpha * epi_add (FP64 sweeps) or x + (float)
-    // epi_add (FP32 sweeps), the next bin's right-hand side or the outer
+    // epi_add (FP32 sweeps; epi_add FP32 when epi_add_f32), the next bin's
+    // right-hand side or the outer
     // residual, so no separate pass forms it
     void* epi_out;
     long long epi_os;
-    const double* epi_add;
+    const void* epi_add;
+    int epi_add_f32;
     long long epi_as;
     double epi_alpha;
     int nsweeps;        // sweeps of the inner solve (a slot's last is min(count, nsweeps) - 1)
@@ -536,7 +538,8 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
         const XT xn = (XT)(xo + (CT)omega * dx);
         x[base] = xn;
         if (a.epi_out && a.sweep_idx == (cnt < a.nsweeps ? cnt : a.nsweeps) - 1) {
-            const double ad = a.epi_add[(size_t)ls * a.epi_as + base];
+            const size_t ia = (size_t)ls * a.epi_as + base;
+            const double ad = a.epi_add_f32 ? (double)((const float*)a.epi_add)[ia] : ((const double*)a.epi_add)[ia];
             if constexpr (sizeof(XT) == 4)
                 ((float*)a.epi_out)[(size_t)ls * a.epi_os + base] = (float)xn + (float)ad;
             else
@@ -1759,16 +1762,6 @@ __global__ void k_zero_last_col(T* __restrict__ x, long long xslot, int nx, int 
     x[(size_t)slot * xslot + ((size_t)j * nx + nx - 1) * 16 + k] = T(0);
 }
 
-// out = (in ? in : 0) + (float)e   per slot (FP32 forced sweeps)
-__global__ void k_add_f32(const float* __restrict__ in, long long is, const double* __restrict__ e, long long es,
-                          float* __restrict__ out, long long os, int n, const int* active, SlotLen sl) {
-    const int slot = blockIdx.y;
-    if (active && !active[slot]) return;
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n || !in_slot(sl, slot, g)) return;
-    const float v = (float)e[(size_t)slot * es + g];
-    out[(size_t)slot * os + g] = in ? in[(size_t)slot * is + g] + v : v;
-}
 
 // ------------------------------------------------------------------------- //
 // orbit-state refinement (mixed precision)
@@ -1797,7 +1790,7 @@ struct CycArgs {
     long long xs;
     const double* E;      // forcing [slot][nbins][n16]
     long long es;
-    double* R;            // out: residuals [slot][nbins][n16]
+    float* R;             // out: residuals [slot][nbins][n16] (they feed the FP32 residual sweep)
     long long rs;
     int n16, adj;
     int xzero;            // every state is zero (not read)
@@ -1902,7 +1895,7 @@ __global__ __launch_bounds__(256) void k_cyc_residual(CycArgs a) {
         acc += he * ye[q];
         acc += hs * ys[q];
         acc += hn * yn[q];
-        if (valid) a.R[(size_t)(first + q) * a.rs + (size_t)t * n16 + base] = uu[q] - acc;
+        if (valid) a.R[(size_t)(first + q) * a.rs + (size_t)t * n16 + base] = (float)(uu[q] - acc);
     }
 }
 
@@ -2118,7 +2111,8 @@ int launch_group_kernel(const Batch& B, int bin, int prec, SweepArgs a, bool own
 struct Epi {
     void* out = nullptr;
     long long os = 0;
-    const double* add = nullptr;
+    const void* add = nullptr;
+    bool add_f32 = false;  // add is FP32 (else FP64)
     long long as = 0;
     double alpha = 0.0;
 };
@@ -2127,6 +2121,7 @@ void set_epi(SweepArgs& a, const Epi* epi, int nsweeps) {
     a.epi_out = epi ? epi->out : nullptr;
     a.epi_os = epi ? epi->os : 0;
     a.epi_add = epi ? epi->add : nullptr;
+    a.epi_add_f32 = epi && epi->add_f32 ? 1 : 0;
     a.epi_as = epi ? epi->as : 0;
     a.epi_alpha = epi ? epi->alpha : 0.0;
     a.nsweeps = nsweeps;
@@ -2523,7 +2518,7 @@ struct Solver {
     const int* chk_cnt = nullptr;
     // orbit-state refinement (the mixed path): the states live in `orbit`
     bool states = false;
-    double* Rc = nullptr;     // bin residuals [slot][nbins][n16]
+    float* Rc = nullptr;      // bin residuals [slot][nbins][n16] (FP32: the residual sweep's forcing)
     float* Z = nullptr;       // intermediate states of the residual sweep [slot][nbins-1][n16]
     float* W = nullptr;       // per Krylov vector j < kp: intermediate states of its M~ apply
     int kp = 0;               // Krylov vectors whose states are kept: a cycle's step cap
@@ -2688,25 +2683,26 @@ int z_sweep(Solver& S) {
     dim3 g(grid1(n16).x, S.B.nslots);
     const int* mask = S.B.part_dev;
     const bool fwd = S.dir == FWD;
-    float* rhs = S.fc;
-    // the first right-hand side here; every later one (state + residual) by the
-    // previous bin's last sweep, in place (a lane reads its rhs before it writes
-    // the next one, and a slot's later sweeps do not store)
-    hipLaunchKernelGGL(k_add_f32, g, dim3(256), 0, st, nullptr, vs, S.Rc + (size_t)(fwd ? 0 : nbn - 1) * n16, es, rhs,
-                       vs, (int)n16, mask, S.sl);
+    // the first bin's right-hand side is its residual itself; every later one
+    // (state + residual) is written into fc by the previous bin's last sweep,
+    // in place after the first (a lane reads its rhs before it writes the next
+    // one, and a slot's later sweeps do not store)
     for (int k = 0; k < nbn; ++k) {
         const int t = fwd ? k : nbn - 1 - k;
         const bool last = k == nbn - 1;
         const int sout = fwd ? t + 1 : t;  // the state this bin writes
         float* out = last ? S.fb : S.Z + (size_t)(sout - 1) * n16;
+        const float* rhs = k == 0 ? S.Rc + (size_t)t * n16 : S.fc;
+        const long long rs = k == 0 ? es : vs;
         Epi epi;
         if (!last) {
-            epi.out = rhs;
+            epi.out = S.fc;
             epi.os = vs;
             epi.add = S.Rc + (size_t)(fwd ? t + 1 : t - 1) * n16;
+            epi.add_f32 = true;
             epi.as = es;
         }
-        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.chk, rhs, vs, out, last ? vs : zs, mask, false, S.chk_cnt,
+        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.chk, rhs, rs, out, last ? vs : zs, mask, false, S.chk_cnt,
                              last ? nullptr : &epi);
         if (rc) return rc;
     }
@@ -2930,7 +2926,7 @@ void solve_scratch_parts(size_t nt, int nbn, size_t n16, int m, bool mixed, size
     b[4] = nt * 3 * n16 * 4;
     b[5] = nt * MAXM1 * nb * 8 * 2 + nt * MAXM1 * 8 * 2 + nt * 8 * 2 + nt * 4 * 6 + 1024;
     b[6] = nt * sizeof(GState);
-    b[7] = mixed ? nt * nbn * n16 * 8 : 0;
+    b[7] = mixed ? nt * nbn * n16 * 4 : 0;
     b[8] = mixed ? nt * (nbn - 1) * n16 * 4 : 0;
     b[9] = mixed ? mk * nt * (nbn - 1) * n16 * 4 : 0;
 }
@@ -3057,7 +3053,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.gst = (GState*)part_ptr[6];
     if (mixed) {
         S.states = true;
-        S.Rc = (double*)part_ptr[7];
+        S.Rc = (float*)part_ptr[7];
         S.Z = (float*)part_ptr[8];
         S.W = (float*)part_ptr[9];
         S.kp = cycle_steps(S.m, true);
