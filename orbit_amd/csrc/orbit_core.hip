@@ -1906,7 +1906,9 @@ __global__ __launch_bounds__(256) void k_cyc_residual(CycArgs a) {
 // the cycle) and the other end c - scl (scl: the s of a solve given s, else
 // zero) plus zend (nullable: the residual sweep's last state, when a
 // converged orbit takes its states from that sweep).  Forward: free state 0,
-// end state nbins; adjoint the reverse.  grid (ceil(n16 / 256), nslots, nbins + 1).
+// end state nbins; adjoint the reverse.
+// Four consecutive elements a thread (n16 and every slot's length are
+// multiples of 16): grid (ceil(n16 / 1024), nslots, nbins + 1).
 __global__ void k_cyc_combine(double* __restrict__ X, long long xs, const float* __restrict__ Z, long long zs,
                               const float* __restrict__ W, long long wj, long long ws, const double* __restrict__ coef,
                               const int* __restrict__ kcols, const double* __restrict__ c, long long cs,
@@ -1914,25 +1916,46 @@ __global__ void k_cyc_combine(double* __restrict__ X, long long xs, const float*
                               int nbins, int adj, const int* active, SlotLen sl) {
     const int slot = blockIdx.y, t = blockIdx.z;
     if (active && !active[slot]) return;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
     if (g >= n16 || !in_slot(sl, slot, g)) return;
     double* x = X + (size_t)slot * xs + (size_t)t * n16 + g;
     const int tfree = adj ? nbins : 0;
-    const double cv = c[(size_t)slot * cs + g];
-    if (t == tfree) {
-        *x = cv;
-    } else if (t == nbins - tfree) {
-        double v = scl ? cv - scl[(size_t)slot * ss + g] : cv;
-        if (zend) v += zend[(size_t)slot * cs + g];
-        *x = v;
+    double v[4];
+    if (t == tfree || t == nbins - tfree) {
+        const double2* cp = (const double2*)(c + (size_t)slot * cs + g);
+        const double2 c0 = cp[0], c1 = cp[1];
+        v[0] = c0.x; v[1] = c0.y; v[2] = c1.x; v[3] = c1.y;
+        if (t != tfree) {
+            if (scl) {
+                const double2* sp = (const double2*)(scl + (size_t)slot * ss + g);
+                const double2 s0 = sp[0], s1 = sp[1];
+                v[0] -= s0.x; v[1] -= s0.y; v[2] -= s1.x; v[3] -= s1.y;
+            }
+            if (zend) {
+                const double2* zp = (const double2*)(zend + (size_t)slot * cs + g);
+                const double2 z0 = zp[0], z1 = zp[1];
+                v[0] += z0.x; v[1] += z0.y; v[2] += z1.x; v[3] += z1.y;
+            }
+        }
     } else {
         const size_t off = (size_t)slot * ws + (size_t)(t - 1) * n16 + g;
-        double v = (double)Z[(size_t)slot * zs + (size_t)(t - 1) * n16 + g];
+        const float4 z = *(const float4*)(Z + (size_t)slot * zs + (size_t)(t - 1) * n16 + g);
+        double u[4] = {(double)z.x, (double)z.y, (double)z.z, (double)z.w};
         const int k = kcols[slot];
         const double* cf = coef + (size_t)slot * MAXM1;
-        for (int i = 0; i < k; ++i) v += cf[i] * (double)W[(size_t)i * wj + off];
-        *x += v;
+        for (int i = 0; i < k; ++i) {
+            const float4 w = *(const float4*)(W + (size_t)i * wj + off);
+            const double ci = cf[i];
+            u[0] += ci * (double)w.x;
+            u[1] += ci * (double)w.y;
+            u[2] += ci * (double)w.z;
+            u[3] += ci * (double)w.w;
+        }
+        const double2 x0 = ((const double2*)x)[0], x1 = ((const double2*)x)[1];
+        v[0] = x0.x + u[0]; v[1] = x0.y + u[1]; v[2] = x1.x + u[2]; v[3] = x1.y + u[3];
     }
+    ((double2*)x)[0] = make_double2(v[0], v[1]);
+    ((double2*)x)[1] = make_double2(v[2], v[3]);
 }
 }  // namespace orbit
 
@@ -2895,10 +2918,10 @@ int gmres_cycle(Solver& S, const std::vector<int>& part, const std::vector<doubl
         // every state: the free one is c, the intermediate ones get z_t + sum_i y_i W_i[t]
         const int nbn = S.nbins;
         const long long ws = (long long)(nbn - 1) * n16;
-        hipLaunchKernelGGL(k_cyc_combine, dim3(grid1(n16).x, ns, nbn + 1), dim3(256), 0, st, S.orbit,
-                           (long long)(nbn + 1) * n16, S.Z, ws, S.W, (long long)ns * ws, ws, S.h1, S.kcols, S.c, vs,
-                           S.scl, vs, (const double*)nullptr, (int)n16, nbn, S.dir == ADJ ? 1 : 0, S.B.part_dev,
-                           S.sl);
+        hipLaunchKernelGGL(k_cyc_combine, dim3((unsigned)((n16 + 1023) / 1024), ns, nbn + 1), dim3(256), 0, st,
+                           S.orbit, (long long)(nbn + 1) * n16, S.Z, ws, S.W, (long long)ns * ws, ws, S.h1, S.kcols,
+                           S.c, vs, S.scl, vs, (const double*)nullptr, (int)n16, nbn, S.dir == ADJ ? 1 : 0,
+                           S.B.part_dev, S.sl);
     }
     ORBIT_HIP_CHECK(hipGetLastError());
     return ORBIT_OK;
@@ -3241,7 +3264,8 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
                 { int rc0 = set_ints(st, S.B.part_dev, done.data(), nt); if (rc0) return rc0; }
                 part_up = done;
                 const long long zs = (long long)(nbn - 1) * n16;
-                hipLaunchKernelGGL(k_cyc_combine, dim3(grid1(n16).x, nt, nbn + 1), dim3(256), 0, st, S.orbit, os, S.Z,
+                hipLaunchKernelGGL(k_cyc_combine, dim3((unsigned)((n16 + 1023) / 1024), nt, nbn + 1), dim3(256), 0, st,
+                                   S.orbit, os, S.Z,
                                    zs, S.W, (long long)nt * zs, zs, S.h1, S.kcols, S.c, vs, S.scl, vs,
                                    (const double*)S.r, (int)n16, nbn, dir == ADJ ? 1 : 0, S.B.part_dev, S.sl);
                 ORBIT_HIP_CHECK(hipGetLastError());
