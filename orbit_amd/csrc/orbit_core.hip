@@ -3243,7 +3243,9 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             for (int s = 0; s < nt; ++s) {
                 if (!part[s]) continue;
                 last_rel[s] = rnorm[s] / snorm[s];
-                if (!(rnorm[s] > o.tol * snorm[s])) {
+                if (!std::isfinite(last_rel[s]) || !std::isfinite(snorm[s])) {
+                    part[s] = 0;  // a non-finite input or state: stops, not converged
+                } else if (rnorm[s] <= o.tol * snorm[s]) {
                     conv[s] = 1;
                     part[s] = 0;
                     done[s] = 1;
@@ -3336,7 +3338,9 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
             for (int s = 0; s < nt; ++s) {
                 if (!part[s]) continue;
                 last_rel[s] = rnorm[s] / snorm[s];
-                if (!(rnorm[s] > o.tol * snorm[s])) {
+                if (!std::isfinite(last_rel[s]) || !std::isfinite(snorm[s])) {
+                    part[s] = 0;  // a non-finite input or state: stops, not converged
+                } else if (rnorm[s] <= o.tol * snorm[s]) {
                     conv[s] = 1;
                     part[s] = 0;
                 } else {

@@ -126,6 +126,25 @@ def test_not_converged_is_a_warning_not_an_error(small):
     assert np.all(np.isfinite(orbit))
 
 
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_non_finite_input_stops_its_slot_unconverged(small, prec):
+    """A NaN in one tracer's emissions makes its residual non-finite: that slot
+    stops at once and reports converged = 0 (the call returns the warning
+    status, which the wrapper accepts, not an error), and the other slots of
+    the batch are the same bits as without it (every decision is per slot)."""
+    import torch
+    wl, op, _ = small
+    e = torch.from_numpy(wl.e).cuda()
+    ref, rinfo = op.solve_forward(e, tol=1e-8, precision=prec)
+    bad = e.clone()
+    bad[1, 3, 17] = float("nan")
+    out, info = op.solve_forward(bad, tol=1e-8, precision=prec)
+    assert not info[1]["converged"] and not np.isfinite(info[1]["residual"])
+    for s in (0, 2):
+        assert info[s]["converged"] and info[s] == rinfo[s]
+        assert torch.equal(out[s], ref[s])
+
+
 def test_adjoint_matches_oracle_and_dot_identity(small):
     wl, op, lus = small
     rng = np.random.default_rng(3)
