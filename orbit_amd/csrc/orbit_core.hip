@@ -2400,7 +2400,7 @@ void set_sweep_counts(orbit_opset* op, double rmax) {
 // steeply after it (64x70x15: 0.099 at 1, 0.063 at 1.02, 0.088 at 1.03, 0.15 at
 // 1.08).  The search measures 1, 1.02, 1.04, 1.06, 1.08, then +-0.01, +-0.005
 // and +-0.0025 around the running best (48x40x15: 0.060 at 1.015 against 0.071
-// at 1.02, one accurate sweep fewer), and keeps omega != 1 only when it
+// at 1.02, one sweep fewer per inner solve), and keeps omega != 1 only when it
 // contracts at least 5% better.
 // The tracers the calibration measures (scratch slot 7): all of them when there are at most ns;
 // else the ns/2 with the smallest diagonal on bin 0 (the weakest loss

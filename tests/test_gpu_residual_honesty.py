@@ -2,9 +2,10 @@
 the CPU oracle recomputes ||s - (I - M) c0|| / ||s|| with exact SuperLU
 propagators (reference orbit/core/orbit.py:99-120, orbit_adjoint.py:36-67):
 it is within the tolerance, and the reported value agrees with it to 5 % of
-the tolerance, in both precisions and directions -- the accurate sweeps that
-measure it stop at the sweep counts their error bounds give, and a cold mixed
-solve's first cycle stops at the FP32 floor of s~, so this is what keeps those
+the tolerance, in both precisions and directions -- the mixed solve measures
+it from one FP64 residual launch over its resident bin states plus an FP32
+residual sweep whose sweep count an error bound sets, and the plain FP64 solve
+from accurate sweeps stopped at their own bound, so this is what keeps those
 shortcuts honest."""
 import numpy as np
 import pytest
@@ -52,8 +53,8 @@ def test_adjoint_residual_is_the_true_one(month, precision, tol):
 @pytest.mark.parametrize("n_passes,tol", [(2, 1e-8), (3, 1e-8), (3, 1e-6)])
 def test_mixed_fct_pass_residual_is_the_true_one(month, n_passes, tol):
     """The library's mixed forward FCT passes start each later pass from the
-    previous orbit plus the FP32 propagation of the forcing's change, with the
-    sweeps of its first accurate check set by an error bound.  For the last
+    previous orbit plus the FP32 propagation of the forcing's change, which
+    the refinement's first residual then measures exactly.  For the last
     pass the oracle rebuilds the forcing e + d_AD(previous pass's end-of-bin
     states) (reference orbit/cli.py:1212-1241, fct.py:459-492) and recomputes
     the residual of the returned c0 with SuperLU propagators: within the
