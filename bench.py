@@ -491,7 +491,8 @@ def main():
         xt = {_lib.ORBIT_PREC_FP64: "double,double,double,double", _lib.ORBIT_SWEEP_ACC_MIXED:
               "double,double,unsigned short,double", _lib.ORBIT_PREC_MIXED: "float,float,unsigned short,float"}
         if v.first_sweep:
-            return "k_first_sweep (both colours of the first sweep from x = 0)"
+            return (f"k_first_sweep (both colours of the first sweep from x = 0 in one launch over column patches, "
+                    f"shape {op.stats()['patch_shape']})")
         return (f"k_halfsweep<{xt[v.precision]},{1 if v.slots_per_wave == 4 else 0},"
                 f"{'true' if v.own_zero else 'false'},{'true' if v.nbr_zero else 'false'},"
                 f"{'true' if v.mfma else 'false'},false>")

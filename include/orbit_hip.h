@@ -81,6 +81,7 @@ typedef struct {
     double relaxation;  /* omega of the sweeps' correction x += omega P (b - A x) */
     int mfma;           /* 1: BF16-inverse sweeps multiply the shared column block on the matrix cores */
     int default_check_sweeps;  /* sweeps per inner solve of the mixed refinement's residual sweep */
+    int patch_shape;    /* orbit_opset_set_patch's shape of the first sweep (0: two half-sweep launches) */
 } orbit_opset_stats;
 
 /* Air-mass-balance statistics (reference orbit/core/operator.py:598-652). */
@@ -156,6 +157,15 @@ int orbit_opset_set_relaxation(orbit_ctx* ctx, orbit_opset* op, double omega);
  * two settings differ in the last bits.  No counterpart in the reference.  Not
  * to be called while a solve on the opset runs. */
 int orbit_opset_set_mfma(orbit_opset* op, int on);
+/* The first sweep (from x = 0) of every FP32 inner solve of a set with one
+ * shared transport and two colours: one launch over column patches of the
+ * given shape (1: 4 x 4 columns, 8 waves a workgroup; 2: 4 x 2, 4 waves, the
+ * default; 3: 8 x 2, 8 waves; 4: 8 x 4, 16 waves), each workgroup computing
+ * the red columns of its patch and of the ring just outside it that its black
+ * columns touch, then its black columns; or 0, two half-sweep launches.  The
+ * same bits either way.  No counterpart in the reference.  Not to be called
+ * while a solve on the opset runs. */
+int orbit_opset_set_patch(orbit_opset* op, int shape);
 /* Replace the per-tracer diagonals (D laid out [bin][t][N]) and refactor the
  * column blocks in place: the outer chemistry loop changes only loss rates. */
 int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const double* D);

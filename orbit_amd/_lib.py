@@ -39,7 +39,7 @@ EXPORTS = [
     "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation", "orbit_forward_fct_dev",
     "orbit_solve_forward_dev_noalloc", "orbit_timing_start", "orbit_timing_stop",
     "orbit_forward_fct_passes_dev", "orbit_lut_partitioning_dev", "orbit_iso_cross_blocks_dev", "orbit_opset_set_mfma",
-    "orbit_solve_group_dev",
+    "orbit_solve_group_dev", "orbit_opset_set_patch",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
 N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
@@ -83,7 +83,7 @@ class OpsetStats(C.Structure):
                 ("default_inner_sweeps", C.c_int), ("default_acc_sweeps", C.c_int),
                 ("dtau", C.c_double), ("device_bytes", C.c_double), ("sweep_contraction", C.c_double),
                 ("periodic_x", C.c_int), ("relaxation", C.c_double), ("mfma", C.c_int),
-                ("default_check_sweeps", C.c_int)]
+                ("default_check_sweeps", C.c_int), ("patch_shape", C.c_int)]
 
 
 class MbStats(C.Structure):
@@ -140,6 +140,7 @@ def signatures():
         "orbit_lut_partitioning_dev": (_I, [_P, _P, _I, _I, _I, C.c_longlong, _P, C.c_longlong, _P, C.c_longlong,
                                             _I, C.c_double, _P, _P]),
         "orbit_opset_set_mfma": (_I, [_P, _I]),
+        "orbit_opset_set_patch": (_I, [_P, _I]),
         "orbit_solve_group_dev": (_I, [_P, _I, _P, _I, _P, _P, _P, _I, C.POINTER(Opts), _P, _P]),
         "orbit_timing_start": (_I, [_P, _I]),
         "orbit_timing_stop": (_I, [_P, _P, _I, _P, _P]),

@@ -548,112 +548,159 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
     }
 }
 
-// The first sweep of an FP32 inner solve (x = 0, two colours) in one launch.
-// Red: x_r = omega P_r b_r needs no neighbour and no column block, so a black
-// column recomputes the red values of its four neighbours itself (the same
-// arithmetic, hence the same bits as the values the red blocks store) and
-// then x_b = omega P_b (b_b - H x_r): the two half-sweeps of the two-launch
-// form with no kernel boundary between them.  Blocks [0, nred) take red
-// tiles, the rest black ones; a wave is one column x four slots; tiles are
-// XCD-contiguous per colour.  Every load is in bounds (missing neighbours
-// read this column and are masked); inactive slots compute but do not store.
+// The first sweep of an FP32 inner solve (x = 0; BF16 inverse blocks, one
+// shared transport, two colours) in one launch over column patches (the
+// geometry is PatchCol's).  From x = 0 a red column's new value is
+// x_r = omega P_r b_r (no neighbour, no column block), so a workgroup computes
+// the reds of its patch and the ring reds its black columns touch, passes them
+// to the blacks through LDS, and the blacks finish with
+// x_b = omega P_b (b_b - H x_r): the arithmetic of the two half-sweeps, hence
+// their bits, with no kernel boundary between them.  A ring red is computed
+// by every patch it borders.  A workgroup of W waves takes one patch and four
+// slots; a wave is one column x four slots (lane = level + 16 * slot) and
+// takes the red items w, w + W, ... of the patch, then its black items the
+// same way.  Every load is in bounds and issued before the first product;
+// missing neighbours and columns past the grid's edge read a column of the
+// patch and are masked or not stored; inactive slots compute but do not store.
+// With one sweep per inner solve this is also the last sweep, so it carries
+// the half-sweeps' epilogue.
 struct FirstSweepArgs {
-    int ncols[2];         // columns of each colour
     int nx, ny, px;
-    int tiles_per_xcd[2];
-    int nred;             // blocks of the red part (8 * tiles_per_xcd[0] * slot groups)
-    const float* H;       // stencil (shared; transposed for the adjoint)
-    const uint16_t* P;    // BF16 inverse blocks, + tracer * pstride
-    long long pstride;
+    int npx, npatches, tiles_per_xcd;
+    const float* H;       // shared stencil (transposed for the adjoint)
+    const uint16_t* P;    // BF16 inverse blocks
+    long long pstride;    // per tracer
     const float* b;
-    long long bstride;
+    long long bstride;    // per slot
     float* x;
     long long xstride;
     const int* tmap;
     int tmap_identity;
-    const int* active;
+    const int* active;    // nullable
     int nslots;
     float omega;
+    // epilogue (epi_out null: none): epi_out = x + epi_add at every column
+    float* epi_out;
+    long long epi_os;
+    const void* epi_add;  // FP32 when epi_add_f32, else FP64
+    int epi_add_f32;
+    long long epi_as;
 };
 
-__global__ __launch_bounds__(256) void k_first_sweep(FirstSweepArgs a) {
-    __shared__ float sr[5][256];  // the four neighbours' right-hand sides, then this column's residual
-    const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    const int black = (int)blockIdx.x >= a.nred;
-    const int L = black ? (int)blockIdx.x - a.nred : (int)blockIdx.x;
+template <int PA, int PB, int W>
+__global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
+    constexpr int NRO = PA * PB / 2, NR = NRO + PA + PB, NB = PA * PB / 2, NPOS = (PA + 2) * (PB + 2);
+    constexpr int RW = (NR + W - 1) / W, BW = (NB + W - 1) / W, IW = RW > BW ? RW : BW;
+    __shared__ float sgrid[NPOS][64];  // new red values by ext position
+    __shared__ float sr[W][IW][64];    // a column's right-hand side / residual, per item of the wave
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, k = l & 15, q = l >> 4, g16 = q << 4;
     const int nsg = (a.nslots + 3) >> 2;
-    int tile, sg;
-    xcd_tile_slot(L, nsg, a.tiles_per_xcd[black], tile, sg);
-    const int ncols = a.ncols[black];
-    if (tile * 4 >= ncols) return;  // uniform over the block
-    const int slot = sg * 4 + (grp & 3);
-    const int idx = tile * 4 + (grp >> 2);
+    int patch, sg;
+    xcd_tile_slot(blockIdx.x, nsg, a.tiles_per_xcd, patch, sg);
+    if (patch >= a.npatches) return;  // uniform over the block
+    const int pj = patch / a.npx, i0 = (patch - pj * a.npx) * PA, j0 = pj * PB;
+    const int nx = a.nx, ny = a.ny, px = a.px;
+    const int slot = sg * 4 + q;
     const bool slot_ok = slot < a.nslots;
     const int ls = slot_ok ? slot : a.nslots - 1;
-    const bool valid = idx < ncols && slot_ok;
-    int col, i, j;
-    colour_col(idx < ncols ? idx : 0, black, a.nx, a.nx, col, i, j);
-    const float* __restrict__ b = a.b + (size_t)ls * a.bstride;
     const int tr = a.tmap_identity ? ls : a.tmap[ls];
+    const bool on = slot_ok && (!a.active || a.active[ls] != 0);
+    const float* __restrict__ b = a.b + (size_t)ls * a.bstride;
+    float* __restrict__ x = a.x + (size_t)ls * a.xstride;
     const uint16_t* __restrict__ P = a.P + (size_t)tr * a.pstride;
-    const bool on = valid & (!a.active || a.active[ls] != 0);
-    const int g16 = grp << 4;
-    // this lane's row (level k) of an inverse block
-    auto load_row = [&](int c, uint16_t* row) {
-        const uint16_t* Pr = P + (size_t)c * 256 + lane * 16;
+
+    // ---- every load of the wave's items, red and black
+    int rcol[RW], rpos[RW];
+    bool rown[RW], rgo[RW];
+    float rb[RW];
+    uint16_t rp[RW][16];
 #pragma unroll
-        for (int m = 0; m < 16; ++m) row[m] = Pr[m];
-    };
-    // omega P v (v one value a lane, exchanged through LDS region q): the red
-    // half-sweep's arithmetic from x = 0
-    auto dot_row = [&](const uint16_t* row, int q) {
+    for (int u = 0; u < RW; ++u) {
+        const int t = wv + W * u;
+        const bool has = t < NR;
+        const PatchCol c = patch_red<PA, PB>(has ? t : 0, i0, j0, nx, ny, px);
+        rcol[u] = c.col;
+        rpos[u] = c.pos;
+        rown[u] = has && t < NRO && c.valid;
+        rgo[u] = has;
+        rb[u] = b[(size_t)c.col * 16 + k];
+        const uint16_t* Pr = P + (size_t)c.col * 256 + k * 16;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) rp[u][m] = Pr[m];
+    }
+    int bcol[BW], bq[BW][4];
+    bool bgo[BW], bst[BW], bw_[BW], be_[BW], bs_[BW], bn_[BW];
+    float bb[BW], bh[BW][4];
+    uint16_t bp[BW][16];
+#pragma unroll
+    for (int u = 0; u < BW; ++u) {
+        const int t = wv + W * u;
+        const bool has = t < NB;
+        const PatchCol c = patch_black<PA, PB>(has ? t : 0, i0, j0, nx, ny);
+        bcol[u] = c.col;
+        bgo[u] = has;
+        bst[u] = has && c.valid;
+        patch_black_nbrs<PA>(c.pos, c.i, i0, nx, bq[u]);
+        bw_[u] = west_col(c.col, c.i, nx, px) >= 0;
+        be_[u] = east_col(c.col, c.i, nx, px) >= 0;
+        bs_[u] = c.j > 0;
+        bn_[u] = c.j < ny - 1;
+        bb[u] = b[(size_t)c.col * 16 + k];
+        const uint16_t* Pr = P + (size_t)c.col * 256 + k * 16;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) bp[u][m] = Pr[m];
+        const float* Hc = a.H + (size_t)c.col * 64 + k;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bh[u][e] = Hc[16 * e];
+    }
+    // omega P v with v the wave's item u in sr: the half-sweep's product from x = 0
+    auto apply_P = [&](const uint16_t* row, int u) {
         float dx = 0;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) dx += (float)ldv(row[m]) * sr[q][g16 + m];
+        for (int m = 0; m < 16; ++m) dx += (float)ldv(row[m]) * sr[wv][u][g16 + m];
         return (float)(0.0f + a.omega * dx);
     };
-    uint16_t pc[16];
-    load_row(col, pc);
-    const float bv = b[(size_t)col * 16 + lane];
-    if (!black) {
-        sr[4][threadIdx.x] = bv;
-        wave_sync();
-        const float xr = dot_row(pc, 4);
-        if (on) a.x[(size_t)ls * a.xstride + (size_t)col * 16 + lane] = xr;
-        return;
+    auto store = [&](int col, float x1) {
+        const size_t base = (size_t)col * 16 + k;
+        x[base] = x1;
+        if (a.epi_out) {
+            const size_t ia = (size_t)ls * a.epi_as + base;
+            const double ad = a.epi_add_f32 ? (double)((const float*)a.epi_add)[ia] : ((const double*)a.epi_add)[ia];
+            a.epi_out[(size_t)ls * a.epi_os + base] = x1 + (float)ad;
+        }
+    };
+
+    // ---- red: x_r = omega P_r b_r for the patch's reds and its ring reds
+#pragma unroll
+    for (int u = 0; u < RW; ++u) sr[wv][u][l] = rb[u];
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < RW; ++u) {
+        if (!rgo[u]) continue;  // uniform over the wave
+        const float x1 = apply_P(rp[u], u);
+        sgrid[rpos[u]][l] = x1;
+        if (rown[u] && on) store(rcol[u], x1);
     }
-    // every load of the black column issued before the first product
-    const int cw = west_col(col, i, a.nx, a.px), ce = east_col(col, i, a.nx, a.px);
-    const int cs = j > 0 ? col - a.nx : col, cn = j < a.ny - 1 ? col + a.nx : col;
-    const int ncw = cw >= 0 ? cw : col, nce = ce >= 0 ? ce : col;
-    uint16_t pw[16], pe[16], ps[16], pn[16];
-    load_row(ncw, pw);
-    load_row(nce, pe);
-    load_row(cs, ps);
-    load_row(cn, pn);
-    const float* Hc = a.H + (size_t)col * 64 + lane;
-    const float hw = Hc[0], he = Hc[16], hs = Hc[32], hn = Hc[48];
-    sr[0][threadIdx.x] = b[(size_t)ncw * 16 + lane];
-    sr[1][threadIdx.x] = b[(size_t)nce * 16 + lane];
-    sr[2][threadIdx.x] = b[(size_t)cs * 16 + lane];
-    sr[3][threadIdx.x] = b[(size_t)cn * 16 + lane];
+    __syncthreads();
+    // ---- black: x_b = omega P_b (b_b - H x_r) from the new reds
+#pragma unroll
+    for (int u = 0; u < BW; ++u) {
+        float r = bb[u];
+        float hx = 0;
+        hx += bh[u][0] * (bw_[u] ? sgrid[bq[u][0]][l] : 0.0f);
+        hx += bh[u][1] * (be_[u] ? sgrid[bq[u][1]][l] : 0.0f);
+        hx += bh[u][2] * (bs_[u] ? sgrid[bq[u][2]][l] : 0.0f);
+        hx += bh[u][3] * (bn_[u] ? sgrid[bq[u][3]][l] : 0.0f);
+        r -= hx;
+        sr[wv][u][l] = r;
+    }
     wave_sync();
-    // the four red neighbours' values, as the red blocks compute them
-    const float xw = dot_row(pw, 0);
-    const float xe = dot_row(pe, 1);
-    const float xs = dot_row(ps, 2);
-    const float xn = dot_row(pn, 3);
-    float r = bv;
-    float hx = 0;
-    hx += hw * (cw >= 0 ? xw : 0.0f);
-    hx += he * (ce >= 0 ? xe : 0.0f);
-    hx += hs * (j > 0 ? xs : 0.0f);
-    hx += hn * (j < a.ny - 1 ? xn : 0.0f);
-    r -= hx;
-    sr[4][threadIdx.x] = r;
-    wave_sync();
-    const float xb = dot_row(pc, 4);
-    if (on) a.x[(size_t)ls * a.xstride + (size_t)col * 16 + lane] = xb;
+#pragma unroll
+    for (int u = 0; u < BW; ++u) {
+        if (!bgo[u]) continue;  // uniform over the wave
+        const float x1 = apply_P(bp[u], u);
+        if (bst[u] && on) store(bcol[u], x1);
+    }
 }
 
 // y = A x (one operator apply, FP64), used for residual checks of the inner solves.
@@ -1652,6 +1699,7 @@ extern "C" int orbit_opset_stats_get(orbit_opset* op, orbit_opset_stats* st) {
     st->relaxation = op->omega;
     st->mfma = op->mfma ? 1 : 0;
     st->default_check_sweeps = op->def_check;
+    st->patch_shape = op->patch;
     return ORBIT_OK;
 }
 
@@ -2230,21 +2278,32 @@ int launch_halfsweep(const Batch& B, int bin, int dir, int prec, int colour, con
 }
 
 // The first FP32 sweep from x = 0 of a two-colour, shared-transport set in one
-// launch (k_first_sweep): the same bits as its two half-sweeps.
+// launch over column patches (k_first_sweep): the same bits as its two
+// half-sweeps.  The patch shapes (orbit_opset_set_patch; 0: two half-sweep
+// launches): columns per patch in x and y, waves per workgroup.
+struct PatchShape { int pa, pb, waves; };
+constexpr PatchShape PATCH_SHAPES[] = {{0, 0, 0}, {4, 4, 8}, {4, 2, 4}, {8, 2, 8}, {8, 4, 16}};
+constexpr int N_PATCH_SHAPES = (int)(sizeof(PATCH_SHAPES) / sizeof(PATCH_SHAPES[0]));
+
+template <int PA, int PB, int W>
+void first_sweep_launch(hipStream_t st, const FirstSweepArgs& a, dim3 grid, hipEvent_t e0, hipEvent_t e1) {
+    if (e0) hipExtLaunchKernelGGL((k_first_sweep<PA, PB, W>), grid, dim3(W * 64), 0, st, e0, e1, 0, a);
+    else hipLaunchKernelGGL((k_first_sweep<PA, PB, W>), grid, dim3(W * 64), 0, st, a);
+}
+
 int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long long bstride, void* x,
-                       long long xstride, const int* mask) {
+                       long long xstride, const int* mask, const Epi* epi) {
     orbit_opset* op = B.op;
     const BinOps& O = op->bins[bin];
     const bool T = dir == ADJ;
+    const PatchShape sh = PATCH_SHAPES[op->patch];
     FirstSweepArgs a;
-    a.ncols[0] = op->ncols_c[0];
-    a.ncols[1] = op->ncols_c[1];
     a.nx = op->nx;
     a.ny = op->ny;
     a.px = op->px;
-    const int nsg = (B.nslots + 3) / 4;
-    for (int c = 0; c < 2; ++c) a.tiles_per_xcd[c] = ((a.ncols[c] + 3) / 4 + 7) / 8;
-    a.nred = 8 * a.tiles_per_xcd[0] * nsg;
+    a.npx = (op->nx + sh.pa - 1) / sh.pa;
+    a.npatches = a.npx * ((op->ny + sh.pb - 1) / sh.pb);
+    a.tiles_per_xcd = (a.npatches + 7) / 8;
     a.H = T ? O.HT32 : O.H32;
     a.P = T ? O.PT16 : O.P16;
     a.pstride = (long long)op->ncol * 256;
@@ -2257,7 +2316,13 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
     a.active = mask;
     a.nslots = B.nslots;
     a.omega = (float)op->omega;
-    const dim3 grid((unsigned)(a.nred + 8 * a.tiles_per_xcd[1] * nsg)), block(256);
+    a.epi_out = epi ? (float*)epi->out : nullptr;
+    a.epi_os = epi ? epi->os : 0;
+    a.epi_add = epi ? epi->add : nullptr;
+    a.epi_add_f32 = epi && epi->add_f32 ? 1 : 0;
+    a.epi_as = epi ? epi->as : 0;
+    const int nsg = (B.nslots + 3) / 4;
+    const dim3 grid((unsigned)(8 * a.tiles_per_xcd * nsg));
     hipStream_t st = B.ctx->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (SweepTiming* tm = B.ctx->timing) {
@@ -2266,17 +2331,21 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
             e1 = tm->ev[tm->next + 1];
             tm->next += 2;
             // each operand once: both colours' P, b and x written, the black columns' stencil
-            const double bytes = halfsweep_bytes(a.ncols[0], B.nslots, 4, 2, 4, true, true) +
-                                 halfsweep_bytes(a.ncols[1], B.nslots, 4, 2, 4, true, true) + 256.0 * a.ncols[1];
+            const double bytes = halfsweep_bytes(op->ncols_c[0], B.nslots, 4, 2, 4, true, true) +
+                                 halfsweep_bytes(op->ncols_c[1], B.nslots, 4, 2, 4, true, true) +
+                                 256.0 * op->ncols_c[1];
             tm->recs.push_back({FUSED_FIRST_VARIANT + SP_F32 * 16 + 4 + 3, bytes});
         } else {
             ++tm->dropped;
         }
     }
-    if (e0)
-        hipExtLaunchKernelGGL(k_first_sweep, grid, block, 0, st, e0, e1, 0, a);
-    else
-        hipLaunchKernelGGL(k_first_sweep, grid, block, 0, st, a);
+    switch (op->patch) {
+        case 1: first_sweep_launch<4, 4, 8>(st, a, grid, e0, e1); break;
+        case 2: first_sweep_launch<4, 2, 4>(st, a, grid, e0, e1); break;
+        case 3: first_sweep_launch<8, 2, 8>(st, a, grid, e0, e1); break;
+        case 4: first_sweep_launch<8, 4, 16>(st, a, grid, e0, e1); break;
+        default: return fail(ORBIT_ERR_INVALID, "first sweep: unknown patch shape");
+    }
     return ORBIT_OK;
 }
 
@@ -2292,9 +2361,9 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
                const Epi* epi = nullptr, int nsweeps = 0) {
     orbit_opset* op = B.op;
     const int ncolours = B.grp ? 2 : op->ncolours;
-    if (first && !B.grp && op->ncolours == 2 && prec == SP_F32 && !counts && op->ngroups == 1 &&
-        (!epi || nsweeps > 1))
-        return launch_first_sweep(B, bin, dir, rhs, bstride, x, xstride, mask);
+    if (first && !B.grp && op->ncolours == 2 && prec == SP_F32 && !counts && op->ngroups == 1 && op->patch > 0 &&
+        !op->mfma)
+        return launch_first_sweep(B, bin, dir, rhs, bstride, x, xstride, mask, nsweeps == 1 ? epi : nullptr);
     if (!B.grp && first && op->ncolours == 4) {
         const dim3 g((unsigned)((op->ny * 16 + 255) / 256), B.nslots);
         if (prec == SP_F32)
@@ -2523,6 +2592,7 @@ struct Solver {
     float* fa = nullptr;
     float* fb = nullptr;
     float* fc = nullptr;
+    float* fc2 = nullptr;     // the residual sweep's right-hand sides alternate between fc and fc2
     double rho = 0.1;
     double eps_acc = 1e-12;
     double* sv = nullptr;    // s given directly (rhs_is_s), else null
@@ -2707,19 +2777,20 @@ int z_sweep(Solver& S) {
     const int* mask = S.B.part_dev;
     const bool fwd = S.dir == FWD;
     // the first bin's right-hand side is its residual itself; every later one
-    // (state + residual) is written into fc by the previous bin's last sweep,
-    // in place after the first (a lane reads its rhs before it writes the next
-    // one, and a slot's later sweeps do not store)
+    // (state + residual) is written by the previous bin's last sweep, into fc
+    // and fc2 in turn (a one-launch sweep reads the right-hand sides of the
+    // columns around its patch, so none is overwritten while the bin runs)
+    float* fcb[2] = {S.fc, S.fc2};
     for (int k = 0; k < nbn; ++k) {
         const int t = fwd ? k : nbn - 1 - k;
         const bool last = k == nbn - 1;
         const int sout = fwd ? t + 1 : t;  // the state this bin writes
         float* out = last ? S.fb : S.Z + (size_t)(sout - 1) * n16;
-        const float* rhs = k == 0 ? S.Rc + (size_t)t * n16 : S.fc;
+        const float* rhs = k == 0 ? S.Rc + (size_t)t * n16 : fcb[(k - 1) & 1];
         const long long rs = k == 0 ? es : vs;
         Epi epi;
         if (!last) {
-            epi.out = S.fc;
+            epi.out = fcb[k & 1];
             epi.os = vs;
             epi.add = S.Rc + (size_t)(fwd ? t + 1 : t - 1) * n16;
             epi.add_f32 = true;
@@ -2946,7 +3017,7 @@ void solve_scratch_parts(size_t nt, int nbn, size_t n16, int m, bool mixed, size
     b[1] = nt * (nbn + 1) * n16 * 8;
     b[2] = nt * (mk + 1) * n16 * 8;
     b[3] = nt * 6 * n16 * 8;
-    b[4] = nt * 3 * n16 * 4;
+    b[4] = nt * 4 * n16 * 4;
     b[5] = nt * MAXM1 * nb * 8 * 2 + nt * MAXM1 * 8 * 2 + nt * 8 * 2 + nt * 4 * 6 + 1024;
     b[6] = nt * sizeof(GState);
     b[7] = mixed ? nt * nbn * n16 * 4 : 0;
@@ -3063,6 +3134,7 @@ int solve_periodic(orbit_ctx* ctx, orbit_opset* op, const int* tracers, int nt, 
     S.fa = (float*)part_ptr[4];
     S.fb = S.fa + (size_t)nt * n16;
     S.fc = S.fa + (size_t)2 * nt * n16;
+    S.fc2 = S.fa + (size_t)3 * nt * n16;
     char* sm = (char*)part_ptr[5];
     S.partials = (double*)sm; sm += (size_t)nt * MAXM1 * S.nb * 8 * 2;
     S.h1 = (double*)sm; sm += (size_t)nt * MAXM1 * 8;
@@ -3868,6 +3940,14 @@ void drop_graphs(orbit_opset* op) {
 }
 
 }  // namespace
+
+extern "C" int orbit_opset_set_patch(orbit_opset* op, int shape) {
+    if (!op) return fail(ORBIT_ERR_INVALID, "set_patch: NULL operator set");
+    if (shape < 0 || shape >= N_PATCH_SHAPES) return fail(ORBIT_ERR_INVALID, "set_patch: unknown patch shape");
+    op->patch = shape;
+    drop_graphs(op);  // captured sweeps carry the old kernels
+    return ORBIT_OK;
+}
 
 extern "C" int orbit_opset_set_mfma(orbit_opset* op, int on) {
     if (!op) return fail(ORBIT_ERR_INVALID, "set_mfma: NULL operator set");

@@ -118,6 +118,102 @@ __host__ __device__ inline int east_col(int col, int i, int nx, int px) {
     return i < nx - 1 ? col + 1 : (px ? col - nx + 1 : -1);
 }
 
+// The first red-black sweep from x = 0 in one launch over column patches
+// (k_first_sweep).  A workgroup owns the PA x PB columns i0 .. i0+PA-1,
+// j0 .. j0+PB-1 of its patch (PA, PB even, so i0 + j0 is even; columns past
+// the grid's edge belong to no patch) and first computes the new values of
+// its red columns and of the red columns just outside it that its black
+// columns touch (the ring reds), each exactly as the red half-sweep does, then
+// the new values of its black columns from those.  Positions in the
+// (PA+2) x (PB+2) ext grid: own column (a, c) at (a+1, c+1); the ring west /
+// east of row c at (0, c+1) / (PA+1, c+1), south / north of column a at
+// (a+1, 0) / (a+1, PB+1).  The east ring is the column after the patch's last
+// one in the grid (i = min(i0+PA, nx), across the wrap face when x is
+// periodic), so a patch cut by the grid's edge still finds its east
+// neighbours.  Red items: PA*PB/2 own reds, then PB/2 west, PB/2 east, PA/2
+// south and PA/2 north ring reds; black items: PA*PB/2 own blacks.
+struct PatchCol {
+    int col;     // column id j * nx + i (a valid column of the patch's own if !valid)
+    int i, j;
+    int pos;     // position in the ext grid
+    bool valid;  // a column of the grid
+};
+template <int PA, int PB>
+__host__ __device__ inline PatchCol patch_red(int t, int i0, int j0, int nx, int ny, int px) {
+    constexpr int HA = PA / 2, HB = PB / 2, EW = PA + 2;
+    PatchCol r;
+    int i, j, a, c;
+    bool ok = true;
+    if (t < HA * PB) {  // own red: row c holds the reds a = (c & 1), (c & 1) + 2, ...
+        c = t / HA;
+        a = (c & 1) + 2 * (t - c * HA);
+        i = i0 + a;
+        j = j0 + c;
+        ok = i < nx && j < ny;
+        r.pos = (a + 1) + (c + 1) * EW;
+    } else if ((t -= HA * PB) < HB) {  // west ring, odd rows
+        c = 2 * t + 1;
+        i = i0 - 1;
+        j = j0 + c;
+        if (i < 0) {
+            if (px) i += nx; else ok = false;
+        }
+        ok = ok && j < ny;
+        r.pos = (c + 1) * EW;
+    } else if ((t -= HB) < HB) {  // east ring, even rows
+        c = 2 * t;
+        i = i0 + PA < nx ? i0 + PA : nx;
+        j = j0 + c;
+        if (i >= nx) {
+            if (px) i -= nx; else ok = false;
+        }
+        ok = ok && j < ny;
+        r.pos = (PA + 1) + (c + 1) * EW;
+    } else if ((t -= HB) < HA) {  // south ring, odd columns
+        a = 2 * t + 1;
+        i = i0 + a;
+        j = j0 - 1;
+        ok = i < nx && j >= 0;
+        r.pos = a + 1;
+    } else {  // north ring, even columns
+        t -= HA;
+        a = 2 * t;
+        i = i0 + a;
+        j = j0 + PB;
+        ok = i < nx && j < ny;
+        r.pos = (a + 1) + (PB + 1) * EW;
+    }
+    r.valid = ok;
+    r.i = ok ? i : i0;
+    r.j = ok ? j : j0;
+    r.col = r.j * nx + r.i;
+    return r;
+}
+template <int PA, int PB>
+__host__ __device__ inline PatchCol patch_black(int t, int i0, int j0, int nx, int ny) {
+    constexpr int HA = PA / 2, EW = PA + 2;
+    const int c = t / HA, a = 1 - (c & 1) + 2 * (t - c * HA);
+    PatchCol r;
+    r.valid = i0 + a < nx && j0 + c < ny;
+    r.i = r.valid ? i0 + a : i0;
+    r.j = r.valid ? j0 + c : j0;
+    r.col = r.j * nx + r.i;
+    r.pos = (a + 1) + (c + 1) * EW;
+    return r;
+}
+// Ext-grid positions of the four neighbours of an own black column at pos
+// (grid column i): west, east (the east ring when i + 1 is past the patch's
+// last grid column), south, north.
+template <int PA>
+__host__ __device__ inline void patch_black_nbrs(int pos, int i, int i0, int nx, int* q) {
+    constexpr int EW = PA + 2;
+    const int iend = i0 + PA < nx ? i0 + PA : nx;
+    q[0] = pos - 1;
+    q[1] = i + 1 == iend ? (PA + 1) + (pos / EW) * EW : pos + 1;
+    q[2] = pos - EW;
+    q[3] = pos + EW;
+}
+
 // Launch shape of the half-sweep kernel: slots per wave (1 or 4), threads per
 // workgroup (64, 128, 256) and block order (XCD-contiguous or linear).  Chosen
 // per sweep precision and slot-count bucket; every shape computes the same
@@ -155,6 +251,7 @@ struct Opset {
     double rho = 0.1;  // calibrated contraction of one red-black sweep (with margin)
     double omega = 1.0;  // relaxation of the sweeps' correction (chosen by the calibration)
     bool mfma = false;   // block product of the BF16-inverse sweeps on the matrix cores (opt-in)
+    int patch = 2;       // first FP32 sweep of an inner solve in one launch over column patches of this shape (0: two half-sweeps)
     std::vector<BinOps> bins;
     // Column colours of the sweeps: red / black ((i + j) even / odd) over the
     // first nxc columns of every row; with a periodic x and an odd nx the wrap

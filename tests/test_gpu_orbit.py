@@ -251,16 +251,20 @@ def test_consecutive_and_permuted_tracer_lists_replay_the_right_graphs(small):
     """A batch of consecutive tracers derives each slot's tracer arithmetically
     and its captured sweeps carry that base; a permuted list of the same size on
     the same context reads the slot map instead.  Each tracer's orbit must be
-    the same bits either way, in either order of the calls."""
+    the same bits either way, in either order of the calls, and the same as
+    direct launches without graphs."""
     wl, op, _ = small
-    a, _ = op.solve_forward(wl.e, tracers=[0, 1, 2], tol=1e-8, precision="mixed")
+    g = dict(tol=1e-8, precision="mixed", use_graph=True)
+    a, _ = op.solve_forward(wl.e, tracers=[0, 1, 2], **g)
     perm = [2, 0, 1]
-    b, _ = op.solve_forward(wl.e[perm], tracers=perm, tol=1e-8, precision="mixed")
-    c, _ = op.solve_forward(wl.e, tracers=[0, 1, 2], tol=1e-8, precision="mixed")
+    b, _ = op.solve_forward(wl.e[perm], tracers=perm, **g)
+    c, _ = op.solve_forward(wl.e, tracers=[0, 1, 2], **g)
     for k, t in enumerate(perm):
         assert np.array_equal(b[k], a[t]), t
     assert np.array_equal(c, a)
-    la, _ = op.solve_adjoint(wl.e, tracers=[0, 1, 2], tol=1e-8, precision="mixed")
-    lb, _ = op.solve_adjoint(wl.e[perm], tracers=perm, tol=1e-8, precision="mixed")
+    d, _ = op.solve_forward(wl.e, tracers=[0, 1, 2], tol=1e-8, precision="mixed", use_graph=False)
+    assert np.array_equal(d, a)  # direct launches: the same bits
+    la, _ = op.solve_adjoint(wl.e, tracers=[0, 1, 2], **g)
+    lb, _ = op.solve_adjoint(wl.e[perm], tracers=perm, **g)
     for k, t in enumerate(perm):
         assert np.array_equal(lb[k], la[t]), t

@@ -97,10 +97,11 @@ def test_changing_relaxation_recaptures_the_sweep_graphs():
     a = OperatorSet.from_workload(wl)
     if a.stats()["relaxation"] == 1.0:
         pytest.skip("this operator keeps omega = 1")
-    a.solve_forward(wl.e, tol=1e-8, precision="mixed")  # captures with the chosen omega
+    g = dict(tol=1e-8, precision="mixed", use_graph=True)
+    a.solve_forward(wl.e, **g)  # captures with the chosen omega
     a.set_relaxation(1.0)
-    ra, _ = a.solve_forward(wl.e, tol=1e-8, precision="mixed")
+    ra, _ = a.solve_forward(wl.e, **g)
     b = OperatorSet.from_workload(wl)
     b.set_relaxation(1.0)
-    rb, _ = b.solve_forward(wl.e, tol=1e-8, precision="mixed")
+    rb, _ = b.solve_forward(wl.e, **g)
     assert np.array_equal(ra, rb)
