@@ -2361,8 +2361,8 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
                const Epi* epi = nullptr, int nsweeps = 0) {
     orbit_opset* op = B.op;
     const int ncolours = B.grp ? 2 : op->ncolours;
-    if (first && !B.grp && op->ncolours == 2 && prec == SP_F32 && !counts && op->ngroups == 1 && op->patch > 0 &&
-        !op->mfma)
+    // (the first sweep has no column-block product, so the matrix-core setting does not enter)
+    if (first && !B.grp && op->ncolours == 2 && prec == SP_F32 && !counts && op->ngroups == 1 && op->patch > 0)
         return launch_first_sweep(B, bin, dir, rhs, bstride, x, xstride, mask, nsweeps == 1 ? epi : nullptr);
     if (!B.grp && first && op->ncolours == 4) {
         const dim3 g((unsigned)((op->ny * 16 + 255) / 256), B.nslots);
