@@ -579,35 +579,66 @@ struct FirstSweepArgs {
     const int* active;    // nullable
     int nslots;
     float omega;
-    // epilogue (epi_out null: none): epi_out = x + epi_add at every column
+    // epilogue (epi_out null: none): epi_out = x + epi_add at every column of
+    // a slot whose last sweep this is (min(counts[slot], nsweeps) == 1)
     float* epi_out;
     long long epi_os;
     const void* epi_add;  // FP32 when epi_add_f32, else FP64
     int epi_add_f32;
     long long epi_as;
+    const int* counts;    // nullable: per-slot sweep counts
+    int nsweeps;
+    // group launches (slots of several operator sets): per block its
+    // (slot group, patch); geometry and operators are the slot group's
+    const GroupOp* gops;
+    const int* btab;      // group << 16 | patch, -1 for a block with no work
 };
 
-template <int PA, int PB, int W>
+template <int PA, int PB, int W, bool GRP = false>
 __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
     constexpr int NRO = PA * PB / 2, NR = NRO + PA + PB, NB = PA * PB / 2, NPOS = (PA + 2) * (PB + 2);
     constexpr int RW = (NR + W - 1) / W, BW = (NB + W - 1) / W, IW = RW > BW ? RW : BW;
     __shared__ float sgrid[NPOS][64];  // new red values by ext position
     __shared__ float sr[W][IW][64];    // a column's right-hand side / residual, per item of the wave
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, k = l & 15, q = l >> 4, g16 = q << 4;
-    const int nsg = (a.nslots + 3) >> 2;
-    int patch, sg;
-    xcd_tile_slot(blockIdx.x, nsg, a.tiles_per_xcd, patch, sg);
-    if (patch >= a.npatches) return;  // uniform over the block
-    const int pj = patch / a.npx, i0 = (patch - pj * a.npx) * PA, j0 = pj * PB;
-    const int nx = a.nx, ny = a.ny, px = a.px;
-    const int slot = sg * 4 + q;
-    const bool slot_ok = slot < a.nslots;
-    const int ls = slot_ok ? slot : a.nslots - 1;
+    // this block's patch and slots, and its operator and grid: the launch's,
+    // or (group launch) its slot group's
+    int patch, npx, nx, ny, px, ls;
+    bool slot_ok;
+    const uint16_t* Pb;
+    const float* Hg;
+    long long pstride;
+    float omega;
+    if constexpr (GRP) {
+        const int e = a.btab[blockIdx.x];
+        if (e < 0) return;  // uniform over the block
+        const GroupOp& go = a.gops[e >> 16];
+        patch = e & 0xffff;
+        nx = go.nx; ny = go.ny; px = go.px;
+        npx = (nx + PA - 1) / PA;
+        Pb = (const uint16_t*)go.P; Hg = (const float*)go.H; pstride = go.pstride; omega = go.omega;
+        slot_ok = q < go.count;
+        ls = go.first + (slot_ok ? q : go.count - 1);
+    } else {
+        const int nsg = (a.nslots + 3) >> 2;
+        int sg;
+        xcd_tile_slot(blockIdx.x, nsg, a.tiles_per_xcd, patch, sg);
+        if (patch >= a.npatches) return;  // uniform over the block
+        npx = a.npx; nx = a.nx; ny = a.ny; px = a.px;
+        Pb = a.P; Hg = a.H; pstride = a.pstride; omega = a.omega;
+        const int slot = sg * 4 + q;
+        slot_ok = slot < a.nslots;
+        ls = slot_ok ? slot : a.nslots - 1;
+    }
+    const int pj = patch / npx, i0 = (patch - pj * npx) * PA, j0 = pj * PB;
     const int tr = a.tmap_identity ? ls : a.tmap[ls];
     const bool on = slot_ok && (!a.active || a.active[ls] != 0);
+    // the epilogue at this slot's last sweep only
+    const int cnt = a.counts ? a.counts[ls] : 0x7fffffff;
+    const bool epi = a.epi_out && (cnt < a.nsweeps ? cnt : a.nsweeps) == 1;
     const float* __restrict__ b = a.b + (size_t)ls * a.bstride;
     float* __restrict__ x = a.x + (size_t)ls * a.xstride;
-    const uint16_t* __restrict__ P = a.P + (size_t)tr * a.pstride;
+    const uint16_t* __restrict__ P = Pb + (size_t)tr * pstride;
 
     // ---- every load of the wave's items, red and black
     int rcol[RW], rpos[RW];
@@ -649,7 +680,7 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
         const uint16_t* Pr = P + (size_t)c.col * 256 + k * 16;
 #pragma unroll
         for (int m = 0; m < 16; ++m) bp[u][m] = Pr[m];
-        const float* Hc = a.H + (size_t)c.col * 64 + k;
+        const float* Hc = Hg + (size_t)c.col * 64 + k;
 #pragma unroll
         for (int e = 0; e < 4; ++e) bh[u][e] = Hc[16 * e];
     }
@@ -658,12 +689,12 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
         float dx = 0;
 #pragma unroll
         for (int m = 0; m < 16; ++m) dx += (float)ldv(row[m]) * sr[wv][u][g16 + m];
-        return (float)(0.0f + a.omega * dx);
+        return (float)(0.0f + omega * dx);
     };
     auto store = [&](int col, float x1) {
         const size_t base = (size_t)col * 16 + k;
         x[base] = x1;
-        if (a.epi_out) {
+        if (epi) {
             const size_t ia = (size_t)ls * a.epi_as + base;
             const double ad = a.epi_add_f32 ? (double)((const float*)a.epi_add)[ia] : ((const double*)a.epi_add)[ia];
             a.epi_out[(size_t)ls * a.epi_os + base] = x1 + (float)ad;
@@ -2026,6 +2057,9 @@ struct GroupSpec {
     const GroupOp* gops = nullptr;      // [bin][prec][group]
     const int* btab[2] = {nullptr, nullptr};
     int nblocks[2] = {0, 0};
+    const int* ptab = nullptr;          // the first sweep's (group, 4 x 2 patch) per block; null: half-sweeps
+    int npblocks = 0;
+    double first_bytes = 0.0;           // algorithmic bytes of a first-sweep launch
     double bytes[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // algorithmic bytes of a full launch, per precision and colour
     const int* slen = nullptr;          // n16 of each slot's set
     const int* sncol = nullptr;
@@ -2285,19 +2319,61 @@ struct PatchShape { int pa, pb, waves; };
 constexpr PatchShape PATCH_SHAPES[] = {{0, 0, 0}, {4, 4, 8}, {4, 2, 4}, {8, 2, 8}, {8, 4, 16}};
 constexpr int N_PATCH_SHAPES = (int)(sizeof(PATCH_SHAPES) / sizeof(PATCH_SHAPES[0]));
 
-template <int PA, int PB, int W>
+template <int PA, int PB, int W, bool GRP = false>
 void first_sweep_launch(hipStream_t st, const FirstSweepArgs& a, dim3 grid, hipEvent_t e0, hipEvent_t e1) {
-    if (e0) hipExtLaunchKernelGGL((k_first_sweep<PA, PB, W>), grid, dim3(W * 64), 0, st, e0, e1, 0, a);
-    else hipLaunchKernelGGL((k_first_sweep<PA, PB, W>), grid, dim3(W * 64), 0, st, a);
+    if (e0) hipExtLaunchKernelGGL((k_first_sweep<PA, PB, W, GRP>), grid, dim3(W * 64), 0, st, e0, e1, 0, a);
+    else hipLaunchKernelGGL((k_first_sweep<PA, PB, W, GRP>), grid, dim3(W * 64), 0, st, a);
 }
 
+// Group launches use one patch shape (4 x 2 columns, 4 waves), whose block
+// table the group solve builds.
+constexpr int GROUP_PATCH_SHAPE = 2;
+
 int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long long bstride, void* x,
-                       long long xstride, const int* mask, const Epi* epi) {
+                       long long xstride, const int* mask, const Epi* epi, const int* counts, int nsweeps) {
+    if (B.grp) {
+        const GroupSpec& G = *B.grp;
+        FirstSweepArgs a{};
+        a.gops = G.gops + ((size_t)bin * N_SWEEP_PRECS + SP_F32) * G.ngroups;
+        a.btab = G.ptab;
+        a.b = (const float*)rhs;
+        a.bstride = bstride;
+        a.x = (float*)x;
+        a.xstride = xstride;
+        a.tmap = B.tmap_dev;
+        a.tmap_identity = 0;
+        a.active = mask;
+        a.nslots = B.nslots;
+        a.epi_out = epi ? (float*)epi->out : nullptr;
+        a.epi_os = epi ? epi->os : 0;
+        a.epi_add = epi ? epi->add : nullptr;
+        a.epi_add_f32 = epi && epi->add_f32 ? 1 : 0;
+        a.epi_as = epi ? epi->as : 0;
+        a.counts = counts;
+        a.nsweeps = nsweeps;
+        hipStream_t st = B.ctx->stream;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (SweepTiming* tm = B.ctx->timing) {
+            if (tm->next + 2 <= tm->ev.size()) {
+                e0 = tm->ev[tm->next];
+                e1 = tm->ev[tm->next + 1];
+                tm->next += 2;
+                tm->recs.push_back({FUSED_FIRST_VARIANT + SP_F32 * 16 + 4 + 3, G.first_bytes});
+            } else {
+                ++tm->dropped;
+            }
+        }
+        static_assert(PATCH_SHAPES[GROUP_PATCH_SHAPE].pa == 4 && PATCH_SHAPES[GROUP_PATCH_SHAPE].pb == 2 &&
+                          PATCH_SHAPES[GROUP_PATCH_SHAPE].waves == 4,
+                      "the group patch table is built for 4 x 2 patches");
+        first_sweep_launch<4, 2, 4, true>(st, a, dim3(G.npblocks), e0, e1);
+        return ORBIT_OK;
+    }
     orbit_opset* op = B.op;
     const BinOps& O = op->bins[bin];
     const bool T = dir == ADJ;
     const PatchShape sh = PATCH_SHAPES[op->patch];
-    FirstSweepArgs a;
+    FirstSweepArgs a{};
     a.nx = op->nx;
     a.ny = op->ny;
     a.px = op->px;
@@ -2321,6 +2397,8 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
     a.epi_add = epi ? epi->add : nullptr;
     a.epi_add_f32 = epi && epi->add_f32 ? 1 : 0;
     a.epi_as = epi ? epi->as : 0;
+    a.counts = counts;
+    a.nsweeps = nsweeps;
     const int nsg = (B.nslots + 3) / 4;
     const dim3 grid((unsigned)(8 * a.tiles_per_xcd * nsg));
     hipStream_t st = B.ctx->stream;
@@ -2361,9 +2439,12 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
                const Epi* epi = nullptr, int nsweeps = 0) {
     orbit_opset* op = B.op;
     const int ncolours = B.grp ? 2 : op->ncolours;
-    // (the first sweep has no column-block product, so the matrix-core setting does not enter)
-    if (first && !B.grp && op->ncolours == 2 && prec == SP_F32 && !counts && op->ngroups == 1 && op->patch > 0)
-        return launch_first_sweep(B, bin, dir, rhs, bstride, x, xstride, mask, nsweeps == 1 ? epi : nullptr);
+    // (the first sweep has no column-block product, so the matrix-core setting
+    // does not enter; a group solve's sets all have one shared transport and
+    // two colours)
+    if (first && prec == SP_F32 &&
+        (B.grp ? B.grp->ptab != nullptr : op->ncolours == 2 && !counts && op->ngroups == 1 && op->patch > 0))
+        return launch_first_sweep(B, bin, dir, rhs, bstride, x, xstride, mask, epi, counts, nsweeps);
     if (!B.grp && first && op->ncolours == 4) {
         const dim3 g((unsigned)((op->ny * 16 + 255) / 256), B.nslots);
         if (prec == SP_F32)
@@ -3687,6 +3768,34 @@ extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* con
             G.bytes[p][c] = bytes;
         }
     }
+    // the first sweep's (group, patch) of every block: 4 x 2 column patches,
+    // ordered like the half-sweeps' tiles
+    std::vector<int> ptab;
+    {
+        std::vector<int> pairs;
+        for (int g0 = 0; g0 < G.ngroups;) {
+            int g1 = g0;
+            while (g1 < G.ngroups && grps[g1].set == grps[g0].set) ++g1;
+            const orbit_opset* so = sets[grps[g0].set];
+            const int npat = ((so->nx + 3) / 4) * ((so->ny + 1) / 2);
+            if (npat >= 65536) return fail(ORBIT_ERR_INVALID, "group solve: grid too large for the block table");
+            for (int t = 0; t < npat; ++t)
+                for (int g = g0; g < g1; ++g) pairs.push_back(g * 65536 + t);
+            g0 = g1;
+        }
+        const size_t P = pairs.size(), per = (P + 7) / 8;
+        ptab.assign(8 * per, -1);
+        for (size_t x = 0; x < 8; ++x)
+            for (size_t r = 0; r < per; ++r)
+                if (x * per + r < P) ptab[8 * r + x] = pairs[x * per + r];
+        G.npblocks = (int)(8 * per);
+        // each operand once: both colours' P, b and x written, the black columns' stencil
+        for (const Grp& gr : grps) {
+            const orbit_opset* so = sets[gr.set];
+            G.first_bytes += halfsweep_bytes(so->ncols_c[0], gr.count, 4, 2, 4, true, true) +
+                             halfsweep_bytes(so->ncols_c[1], gr.count, 4, 2, 4, true, true) + 256.0 * so->ncols_c[1];
+        }
+    }
     // one upload of every table
     std::vector<int> slen(nt), sncol(nt), snz(nt);
     std::vector<const double*> inp(nt);
@@ -3701,7 +3810,8 @@ extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* con
     size_t off = 0;
     auto place = [&](size_t bytes) { const size_t o0 = off; off += align256(bytes); return o0; };
     const size_t o_gops = place(hg.size() * sizeof(GroupOp)), o_t0 = place(tabs[0].size() * 4),
-                 o_t1 = place(tabs[1].size() * 4), o_slen = place((size_t)nt * 4), o_ncol = place((size_t)nt * 4),
+                 o_t1 = place(tabs[1].size() * 4), o_pt = place(ptab.size() * 4), o_slen = place((size_t)nt * 4),
+                 o_ncol = place((size_t)nt * 4),
                  o_nz = place((size_t)nt * 4), o_in = place((size_t)nt * 8), o_out = place((size_t)nt * 8),
                  o_ic = place((size_t)nt * 4), o_ac = place((size_t)nt * 4), o_cc = place((size_t)nt * 4);
     std::vector<char> blob(off, 0);
@@ -3709,6 +3819,7 @@ extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* con
     put(o_gops, hg.data(), hg.size() * sizeof(GroupOp));
     put(o_t0, tabs[0].data(), tabs[0].size() * 4);
     put(o_t1, tabs[1].data(), tabs[1].size() * 4);
+    put(o_pt, ptab.data(), ptab.size() * 4);
     put(o_slen, slen.data(), (size_t)nt * 4);
     put(o_ncol, sncol.data(), (size_t)nt * 4);
     put(o_nz, snz.data(), (size_t)nt * 4);
@@ -3722,6 +3833,7 @@ extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* con
     G.gops = (const GroupOp*)(dev + o_gops);
     G.btab[0] = (const int*)(dev + o_t0);
     G.btab[1] = (const int*)(dev + o_t1);
+    G.ptab = (const int*)(dev + o_pt);
     G.slen = (const int*)(dev + o_slen);
     G.sncol = (const int*)(dev + o_ncol);
     G.snz = (const int*)(dev + o_nz);
