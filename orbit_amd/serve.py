@@ -310,13 +310,14 @@ def run_poisson(server: OrbitServer, rate: float, n_queries: int, make, rng) -> 
 
 
 def solve_group(jobs, precision: str = "mixed", tol: float = 1e-8, adjoint: bool = False, maxiter: int = 200,
-                restart: int = 30, ctx: Optional[Context] = None):
+                restart: int = 30, ctx: Optional[Context] = None, inner_sweeps: int = 0, check_sweeps: int = 0):
     """Every orbit of every job in one batched solve (orbit_solve_group_dev):
     jobs = [(OperatorSet, input (nt, n_bins, N) CUDA float64, tracers)], the
     input emissions (forward) or receptors (adjoint).  Each launch covers all
     slots of all sets, whatever their sub-domain sizes.  Returns [(orbit
     (nt, n_bins+1, N), infos)] per job, each bit-identical to the job solved
-    alone."""
+    alone.  inner_sweeps / check_sweeps (mixed): sweeps per inner solve of the
+    monodromy and of the residual sweep, 0 = each set's own."""
     import ctypes as C
     import torch
     from . import _lib
@@ -343,7 +344,8 @@ def solve_group(jobs, precision: str = "mixed", tol: float = 1e-8, adjoint: bool
             ins.append(inp[q].data_ptr())
             outs.append(out[q].data_ptr())
     nt = len(tracers)
-    opts = OperatorSet._opts(tol, maxiter, restart, precision, 0, 0, False, 0, False)
+    opts = OperatorSet._opts(tol, maxiter, restart, precision, inner_sweeps, 0, False, 0, False,
+                             check=check_sweeps)
     info = (_lib.Info * nt)()
     arr_i = lambda v: (C.c_int * len(v))(*v)  # noqa: E731
     arr_p = lambda v: (C.c_void_p * len(v))(*v)  # noqa: E731

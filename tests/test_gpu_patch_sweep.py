@@ -14,7 +14,9 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-from orbit_amd import synth
+import torch
+
+from orbit_amd import serve, synth
 from orbit_amd.core import OperatorSet
 
 SHAPES = [1, 2, 3, 4]
@@ -66,3 +68,32 @@ def test_patch_sweeps_at_the_benchmark_shape_with_twelve_slots():
         f, _ = op.solve_forward(wl.e, tol=1e-8, precision="mixed")
         assert np.array_equal(f, ref), shape
     op.close()
+
+
+def test_group_first_sweep_with_one_sweep_per_inner_solve():
+    """A group solve's first sweep takes its patches from the group's table and
+    each slot's sweep count from its set; with one sweep per inner solve it is
+    also the last sweep and forms the next bin's right-hand side.  Sets of
+    three sizes (one periodic in x, patches cut by the grid's edge), forward
+    and adjoint: bit-identical to each set solved alone."""
+    jobs = []
+    for k, (nx, ny, nt, per) in enumerate(((10, 7, 3, False), (12, 9, 2, True), (6, 5, 4, False))):
+        wl = synth.make_workload(nx, ny, 6, nt, seed=[31, k], convection=True, periodic=per)
+        op = OperatorSet.from_workload(wl)
+        jobs.append((wl, op))
+    for adjoint in (False, True):
+        work = []
+        for wl, op in jobs:
+            inp = wl.e if not adjoint else np.random.default_rng(op.nx).random(wl.e.shape)
+            work.append((op, torch.from_numpy(np.ascontiguousarray(inp)).cuda(), list(range(wl.n_tracers))))
+        for sweeps in (1, 3):
+            print(f"adjoint {adjoint}, {sweeps} sweep(s) per inner solve", flush=True)
+            got = serve.solve_group(work, precision="mixed", tol=1e-4, adjoint=adjoint, inner_sweeps=sweeps,
+                                    check_sweeps=sweeps)
+            for (op, inp, tr), (orbit, info) in zip(work, got):
+                fn = op.solve_adjoint if adjoint else op.solve_forward
+                alone, ainfo = fn(inp, tr, tol=1e-4, precision="mixed", inner_sweeps=sweeps, check_sweeps=sweeps)
+                assert torch.equal(orbit, alone), (adjoint, sweeps, op.nx)
+                assert info == ainfo
+    for _, op in jobs:
+        op.close()
