@@ -230,7 +230,7 @@ def main():
         n, u = C.c_int(), C.c_longlong()
         _lib.check(_lib.lib().orbit_timing_stop(ctx.handle, t, 32, C.byref(n), C.byref(u)))
         return {f"prec{t[i].precision}_cs{t[i].slots_per_wave}_mf{t[i].mfma}_oz{t[i].own_zero}_nz{t[i].nbr_zero}"
-                + ("_first" if t[i].first_sweep else ""):
+                + ("_handoff" if t[i].first_sweep == 2 else "_first" if t[i].first_sweep else ""):
                 round(1e3 * t[i].total_ms / t[i].launches, 3) for i in range(min(n.value, 32))}
 
     # the other block product (matrix cores against the VALU loop): the same
@@ -483,6 +483,8 @@ def main():
     def vname(v):
         base = {_lib.ORBIT_PREC_FP64: "fp64", _lib.ORBIT_SWEEP_ACC_MIXED: "accurate",
                 _lib.ORBIT_PREC_MIXED: "inner"}[v.precision]
+        if v.first_sweep == 2:
+            return base + "_handoff"
         if v.first_sweep:
             return base + "_first"
         return base + ("_ownzero" if v.own_zero else "") + ("_nbrzero" if v.nbr_zero else "")
@@ -490,6 +492,9 @@ def main():
     def kname(v):
         xt = {_lib.ORBIT_PREC_FP64: "double,double,double,double", _lib.ORBIT_SWEEP_ACC_MIXED:
               "double,double,unsigned short,double", _lib.ORBIT_PREC_MIXED: "float,float,unsigned short,float"}
+        if v.first_sweep == 2:
+            return (f"k_first_sweep, hand-off form (the previous bin's last black half-sweep and both colours of "
+                    f"this bin's first sweep from x = 0 in one launch over column patches, shape {op.stats()['patch_shape']})")
         if v.first_sweep:
             return (f"k_first_sweep (both colours of the first sweep from x = 0 in one launch over column patches, "
                     f"shape {op.stats()['patch_shape']})")

@@ -166,6 +166,15 @@ int orbit_opset_set_mfma(orbit_opset* op, int on);
  * same bits either way.  No counterpart in the reference.  Not to be called
  * while a solve on the opset runs. */
 int orbit_opset_set_patch(orbit_opset* op, int shape);
+/* Bin hand-off (on by default): in a chain of FP32 inner solves over the bins
+ * (a monodromy apply, the residual sweep), each bin's last black half-sweep
+ * runs inside the next bin's first-sweep launch, whose black columns finish
+ * the previous bin's sweep at their own column before starting their own
+ * (every red of the previous bin is final by then): one launch fewer per bin
+ * boundary, the same bits.  Applies where orbit_opset_set_patch's one-launch
+ * first sweep does, with two or more sweeps per inner solve.  No counterpart
+ * in the reference.  Not to be called while a solve on the opset runs. */
+int orbit_opset_set_handoff(orbit_opset* op, int on);
 /* Replace the per-tracer diagonals (D laid out [bin][t][N]) and refactor the
  * column blocks in place: the outer chemistry loop changes only loss rates. */
 int orbit_opset_update_diag(orbit_ctx* ctx, orbit_opset* op, const double* D);
@@ -525,7 +534,8 @@ typedef struct {
     long long launches;
     double total_ms;
     double total_bytes;
-    int first_sweep;     /* 1: the first sweep from x = 0, both colours in one launch */
+    int first_sweep;     /* 1: the first sweep from x = 0, both colours in one launch; 2: that launch
+                            carrying the previous bin's last black half-sweep too (bin hand-off) */
 } orbit_sweep_timing;
 int orbit_timing_start(orbit_ctx* ctx, int max_launches);
 int orbit_timing_stop(orbit_ctx* ctx, orbit_sweep_timing* out, int max_out, int* n_out, long long* untimed);

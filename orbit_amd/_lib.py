@@ -39,7 +39,7 @@ EXPORTS = [
     "orbit_species_diagonals", "orbit_lateral_inflow", "orbit_opset_set_relaxation", "orbit_forward_fct_dev",
     "orbit_solve_forward_dev_noalloc", "orbit_timing_start", "orbit_timing_stop",
     "orbit_forward_fct_passes_dev", "orbit_lut_partitioning_dev", "orbit_iso_cross_blocks_dev", "orbit_opset_set_mfma",
-    "orbit_solve_group_dev", "orbit_opset_set_patch",
+    "orbit_solve_group_dev", "orbit_opset_set_patch", "orbit_opset_set_handoff",
 ]
 N_SWEEP_PRECS = 3    # rows of a shape table: FP64, accurate mixed, FP32 inner sweeps
 N_SLOT_BUCKETS = 5   # columns: 1, 2-3, 4-7, 8-15, 16+ slots
@@ -141,6 +141,7 @@ def signatures():
                                             _I, C.c_double, _P, _P]),
         "orbit_opset_set_mfma": (_I, [_P, _I]),
         "orbit_opset_set_patch": (_I, [_P, _I]),
+        "orbit_opset_set_handoff": (_I, [_P, _I]),
         "orbit_solve_group_dev": (_I, [_P, _I, _P, _I, _P, _P, _P, _I, C.POINTER(Opts), _P, _P]),
         "orbit_timing_start": (_I, [_P, _I]),
         "orbit_timing_stop": (_I, [_P, _P, _I, _P, _P]),

@@ -169,6 +169,12 @@ class OperatorSet:
         launches; the same bits either way."""
         check(lib().orbit_opset_set_patch(self.handle, int(shape)))
 
+    def set_handoff(self, on: bool) -> None:
+        """Run each bin's last black FP32 half-sweep inside the next bin's
+        first-sweep launch (on, the default; see orbit_opset_set_handoff) or
+        as a launch of its own; the same bits either way."""
+        check(lib().orbit_opset_set_handoff(self.handle, int(bool(on))))
+
     def stats(self) -> dict:
         s = _lib.OpsetStats()
         check(lib().orbit_opset_stats_get(self.handle, C.byref(s)))

@@ -592,14 +592,38 @@ struct FirstSweepArgs {
     // (slot group, patch); geometry and operators are the slot group's
     const GroupOp* gops;
     const int* btab;      // group << 16 | patch, -1 for a block with no work
+    // hand-off (HO): the previous bin's last black half-sweep in the same
+    // launch.  Each black item first finishes the previous bin's sweep at its
+    // column, x_b += omega P_b (b - A x)_b with the previous bin's operator
+    // (every red of the previous bin is final: the launch before wrote them),
+    // and that new value (plus the previous bin's epilogue term, when it has
+    // one) is this bin's right-hand side there, which the unfused sequence
+    // would read back from memory.
+    const float* hVt;     // previous bin: transposed column blocks (shared)
+    const float* hH;      // stencil (shared)
+    const float* hd;      // diagonals, per tracer (stride hdstride)
+    const uint16_t* hP;   // BF16 inverse blocks, per tracer (stride pstride)
+    long long hdstride;
+    const float* hb;      // the previous bin's right-hand side
+    long long hbstride;
+    float* hx;            // the previous bin's x (its reds final)
+    long long hxstride;
+    float hepi_omega;     // the previous bin's relaxation
+    float* hepi_out;      // previous bin's epilogue (null: none): out = x + add
+    long long hepi_os;
+    const void* hepi_add;
+    int hepi_add_f32;
+    long long hepi_as;
 };
 
-template <int PA, int PB, int W, bool GRP = false>
+template <int PA, int PB, int W, bool GRP = false, bool HO = false>
 __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
+    static_assert(!(GRP && HO), "the hand-off form runs on one operator set");
     constexpr int NRO = PA * PB / 2, NR = NRO + PA + PB, NB = PA * PB / 2, NPOS = (PA + 2) * (PB + 2);
     constexpr int RW = (NR + W - 1) / W, BW = (NB + W - 1) / W, IW = RW > BW ? RW : BW;
     __shared__ float sgrid[NPOS][64];  // new red values by ext position
     __shared__ float sr[W][IW][64];    // a column's right-hand side / residual, per item of the wave
+    __shared__ float shv[HO ? W : 1][HO ? BW * 256 : 1];  // hand-off: the black items' column blocks
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, k = l & 15, q = l >> 4, g16 = q << 4;
     // this block's patch and slots, and its operator and grid: the launch's,
     // or (group launch) its slot group's
@@ -684,6 +708,40 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) bh[u][e] = Hc[16 * e];
     }
+    // hand-off loads: the previous bin's rows at the black items' columns, in
+    // the order and form of the four-slot half-sweep (k_halfsweep, CS = 1)
+    float hbv[HO ? BW : 1], hxo[HO ? BW : 1], hnx[HO ? BW : 1][4], hdv[HO ? BW : 1], hhv[HO ? BW : 1][4];
+    float hvr[HO ? BW : 1][4];
+    uint16_t hpr[HO ? BW : 1][16];
+    if constexpr (HO) {
+        const float* __restrict__ hb = a.hb + (size_t)ls * a.hbstride;
+        const float* __restrict__ hx = a.hx + (size_t)ls * a.hxstride;
+        const uint16_t* __restrict__ hP = a.hP + (size_t)tr * a.pstride;
+        const float* __restrict__ hd = a.hd + (size_t)tr * a.hdstride;
+#pragma unroll
+        for (int u = 0; u < BW; ++u) {
+            const int col = bcol[u];
+            const size_t base = (size_t)col * 16 + k;
+            const int ci = col % nx, cj = col / nx;
+            const int cw = west_col(col, ci, nx, px), ce = east_col(col, ci, nx, px);
+            hbv[u] = hb[base];
+            hxo[u] = hx[base];
+            hnx[u][0] = hx[(size_t)(cw >= 0 ? cw : col) * 16 + k];
+            hnx[u][1] = hx[(size_t)(ce >= 0 ? ce : col) * 16 + k];
+            hnx[u][2] = hx[(size_t)(cj > 0 ? col - nx : col) * 16 + k];
+            hnx[u][3] = hx[(size_t)(cj < ny - 1 ? col + nx : col) * 16 + k];
+            const uint16_t* Pr = hP + (size_t)col * 256 + k * 16;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) hpr[u][m] = Pr[m];
+            const float* Vt = a.hVt + (size_t)col * 256 + l * 4;
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) hvr[u][qq] = Vt[qq];
+            hdv[u] = hd[base];
+            const float* Hc = a.hH + (size_t)col * 64 + k;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) hhv[u][e] = Hc[16 * e];
+        }
+    }
     // omega P v with v the wave's item u in sr: the half-sweep's product from x = 0
     auto apply_P = [&](const uint16_t* row, int u) {
         float dx = 0;
@@ -700,6 +758,56 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
             a.epi_out[(size_t)ls * a.epi_os + base] = x1 + (float)ad;
         }
     };
+
+    // ---- hand-off: the previous bin's last black half-sweep at the black
+    // items' columns (k_halfsweep's arithmetic, so its bits); the new value is
+    // stored as the previous bin's x and becomes this bin's right-hand side
+    if constexpr (HO) {
+#pragma unroll
+        for (int u = 0; u < BW; ++u) {
+            const int col = bcol[u];
+            const int ci = col % nx, cj = col / nx;
+            const int cw = west_col(col, ci, nx, px), ce = east_col(col, ci, nx, px);
+            float r = hbv[u];
+            float hx = 0;
+            hx += hhv[u][0] * (cw >= 0 ? hnx[u][0] : 0.0f);
+            hx += hhv[u][1] * (ce >= 0 ? hnx[u][1] : 0.0f);
+            hx += hhv[u][2] * (cj > 0 ? hnx[u][2] : 0.0f);
+            hx += hhv[u][3] * (cj < ny - 1 ? hnx[u][3] : 0.0f);
+            r -= hx;
+            const float xo = hxo[u];
+            float* sv = shv[wv] + u * 256;
+            sr[wv][u][l] = xo;
+            float acc = hdv[u] * xo;
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) sv[l * 4 + qq] = hvr[u][qq];
+            wave_sync();
+#pragma unroll
+            for (int m = 0; m < 16; ++m) acc += sv[m * 16 + k] * sr[wv][u][g16 + m];
+            r -= acc;
+            wave_sync();
+            sr[wv][u][l] = r;
+            wave_sync();
+            float dx = 0;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) dx += (float)ldv(hpr[u][m]) * sr[wv][u][g16 + m];
+            const float xn = (float)(xo + a.hepi_omega * dx);
+            float bn = xn;
+            const size_t base = (size_t)col * 16 + k;
+            if (a.hepi_out) {
+                const size_t ia = (size_t)ls * a.hepi_as + base;
+                const double ad = a.hepi_add_f32 ? (double)((const float*)a.hepi_add)[ia]
+                                                 : ((const double*)a.hepi_add)[ia];
+                bn = xn + (float)ad;
+            }
+            if (bst[u] && on) {
+                a.hx[(size_t)ls * a.hxstride + base] = xn;
+                if (a.hepi_out) a.hepi_out[(size_t)ls * a.hepi_os + base] = bn;
+            }
+            bb[u] = bn;
+        }
+        wave_sync();
+    }
 
     // ---- red: x_r = omega P_r b_r for the patch's reds and its ring reds
 #pragma unroll
@@ -2319,18 +2427,33 @@ struct PatchShape { int pa, pb, waves; };
 constexpr PatchShape PATCH_SHAPES[] = {{0, 0, 0}, {4, 4, 8}, {4, 2, 4}, {8, 2, 8}, {8, 4, 16}};
 constexpr int N_PATCH_SHAPES = (int)(sizeof(PATCH_SHAPES) / sizeof(PATCH_SHAPES[0]));
 
-template <int PA, int PB, int W, bool GRP = false>
+template <int PA, int PB, int W, bool GRP = false, bool HO = false>
 void first_sweep_launch(hipStream_t st, const FirstSweepArgs& a, dim3 grid, hipEvent_t e0, hipEvent_t e1) {
-    if (e0) hipExtLaunchKernelGGL((k_first_sweep<PA, PB, W, GRP>), grid, dim3(W * 64), 0, st, e0, e1, 0, a);
-    else hipLaunchKernelGGL((k_first_sweep<PA, PB, W, GRP>), grid, dim3(W * 64), 0, st, a);
+    if (e0) hipExtLaunchKernelGGL((k_first_sweep<PA, PB, W, GRP, HO>), grid, dim3(W * 64), 0, st, e0, e1, 0, a);
+    else hipLaunchKernelGGL((k_first_sweep<PA, PB, W, GRP, HO>), grid, dim3(W * 64), 0, st, a);
 }
 
 // Group launches use one patch shape (4 x 2 columns, 4 waves), whose block
 // table the group solve builds.
 constexpr int GROUP_PATCH_SHAPE = 2;
 
+// The previous bin's last black half-sweep, held back to run in the next
+// bin's first-sweep launch (k_first_sweep's hand-off form).
+struct Handoff {
+    bool pending = false;
+    int bin = 0;
+    const void* rhs = nullptr;
+    long long bstride = 0;
+    void* x = nullptr;
+    long long xstride = 0;
+    bool has_epi = false;
+    Epi epi;
+};
+
 int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long long bstride, void* x,
-                       long long xstride, const int* mask, const Epi* epi, const int* counts, int nsweeps) {
+                       long long xstride, const int* mask, const Epi* epi, const int* counts, int nsweeps,
+                       const Handoff* ho = nullptr) {
+    if (ho && (B.grp || !ho->pending)) return fail(ORBIT_ERR_INVALID, "first sweep: no hand-off to take");
     if (B.grp) {
         const GroupSpec& G = *B.grp;
         FirstSweepArgs a{};
@@ -2399,6 +2522,31 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
     a.epi_as = epi ? epi->as : 0;
     a.counts = counts;
     a.nsweeps = nsweeps;
+    if (ho) {
+        // the previous bin's last black half-sweep: its value at a black column
+        // is this bin's right-hand side there only if this bin reads it from
+        // where that half-sweep (or its epilogue) would have stored it
+        const void* feeds = ho->has_epi ? ho->epi.out : ho->x;
+        const long long fs = ho->has_epi ? ho->epi.os : ho->xstride;
+        if (counts || feeds != rhs || fs != bstride || op->ncolours != 2 || op->ngroups != 1)
+            return fail(ORBIT_ERR_INVALID, "first sweep: the hand-off does not feed this right-hand side");
+        const BinOps& Q = op->bins[ho->bin];
+        a.hVt = T ? Q.V32 : Q.VT32;
+        a.hH = T ? Q.HT32 : Q.H32;
+        a.hd = Q.d32;
+        a.hP = T ? Q.PT16 : Q.P16;
+        a.hdstride = (long long)op->n16;
+        a.hb = (const float*)ho->rhs;
+        a.hbstride = ho->bstride;
+        a.hx = (float*)ho->x;
+        a.hxstride = ho->xstride;
+        a.hepi_omega = (float)op->omega;
+        a.hepi_out = ho->has_epi ? (float*)ho->epi.out : nullptr;
+        a.hepi_os = ho->has_epi ? ho->epi.os : 0;
+        a.hepi_add = ho->has_epi ? ho->epi.add : nullptr;
+        a.hepi_add_f32 = ho->has_epi && ho->epi.add_f32 ? 1 : 0;
+        a.hepi_as = ho->has_epi ? ho->epi.as : 0;
+    }
     const int nsg = (B.nslots + 3) / 4;
     const dim3 grid((unsigned)(8 * a.tiles_per_xcd * nsg));
     hipStream_t st = B.ctx->stream;
@@ -2409,13 +2557,26 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
             e1 = tm->ev[tm->next + 1];
             tm->next += 2;
             // each operand once: both colours' P, b and x written, the black columns' stencil
-            const double bytes = halfsweep_bytes(op->ncols_c[0], B.nslots, 4, 2, 4, true, true) +
-                                 halfsweep_bytes(op->ncols_c[1], B.nslots, 4, 2, 4, true, true) +
-                                 256.0 * op->ncols_c[1];
-            tm->recs.push_back({FUSED_FIRST_VARIANT + SP_F32 * 16 + 4 + 3, bytes});
+            double bytes = halfsweep_bytes(op->ncols_c[0], B.nslots, 4, 2, 4, true, true) +
+                           halfsweep_bytes(op->ncols_c[1], B.nslots, 4, 2, 4, true, true) +
+                           256.0 * op->ncols_c[1];
+            // with the hand-off, the previous bin's black half-sweep on top (its
+            // right-hand side and x of this bin's black columns are not read back)
+            if (ho) bytes += halfsweep_bytes(op->ncols_c[1], B.nslots, 4, 2, 4, false, false);
+            tm->recs.push_back({(ho ? HANDOFF_VARIANT : FUSED_FIRST_VARIANT) + SP_F32 * 16 + 4 + 3, bytes});
         } else {
             ++tm->dropped;
         }
+    }
+    if (ho) {
+        switch (op->patch) {
+            case 1: first_sweep_launch<4, 4, 8, false, true>(st, a, grid, e0, e1); break;
+            case 2: first_sweep_launch<4, 2, 4, false, true>(st, a, grid, e0, e1); break;
+            case 3: first_sweep_launch<8, 2, 8, false, true>(st, a, grid, e0, e1); break;
+            case 4: first_sweep_launch<8, 4, 16, false, true>(st, a, grid, e0, e1); break;
+            default: return fail(ORBIT_ERR_INVALID, "first sweep: unknown patch shape");
+        }
+        return ORBIT_OK;
     }
     switch (op->patch) {
         case 1: first_sweep_launch<4, 4, 8>(st, a, grid, e0, e1); break;
@@ -2462,12 +2623,42 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
     return ORBIT_OK;
 }
 
+// Whether consecutive bins' FP32 inner solves of `sweeps` sweeps can hand the
+// last black half-sweep of one to the first-sweep launch of the next: one
+// shared transport, two colours, the patch first sweep on, no per-slot counts,
+// a last sweep that is not itself a first sweep.
+bool handoff_ok(const Batch& B, int prec, int sweeps, const int* counts) {
+    const orbit_opset* op = B.op;
+    return prec == SP_F32 && !B.grp && !counts && sweeps >= 2 && op->handoff && op->ncolours == 2 &&
+           op->ngroups == 1 && op->patch > 0;
+}
+
 int inner_solve(const Batch& B, int bin, int dir, int prec, int sweeps, const void* rhs,
                 long long bstride, void* x, long long xstride, const int* mask, bool warm = false,
-                const int* counts = nullptr, const Epi* epi = nullptr) {
+                const int* counts = nullptr, const Epi* epi = nullptr, Handoff* hin = nullptr,
+                Handoff* hout = nullptr) {
+    // hin (pending): the previous bin's last black half-sweep runs in this
+    // solve's first-sweep launch; hout: this solve's last black half-sweep is
+    // held back for the next bin's (the caller checked handoff_ok for both)
     for (int s = 0; s < sweeps; ++s) {
-        const int rc = sweep_once(B, bin, dir, prec, rhs, bstride, x, xstride, s == 0 && !warm, mask, counts, s, epi,
+        int rc;
+        if (s == 0 && hin && hin->pending) {
+            rc = launch_first_sweep(B, bin, dir, rhs, bstride, x, xstride, mask, epi, counts, sweeps, hin);
+            hin->pending = false;
+        } else if (s == sweeps - 1 && hout) {
+            rc = launch_halfsweep(B, bin, dir, prec, 0, rhs, bstride, x, xstride, false, false, mask, counts, s, epi,
                                   sweeps);
+            hout->pending = true;
+            hout->bin = bin;
+            hout->rhs = rhs;
+            hout->bstride = bstride;
+            hout->x = x;
+            hout->xstride = xstride;
+            hout->has_epi = epi != nullptr;
+            if (epi) hout->epi = *epi;
+        } else {
+            rc = sweep_once(B, bin, dir, prec, rhs, bstride, x, xstride, s == 0 && !warm, mask, counts, s, epi, sweeps);
+        }
         if (rc) return rc;
     }
     return ORBIT_OK;
@@ -2862,6 +3053,8 @@ int z_sweep(Solver& S) {
     // and fc2 in turn (a one-launch sweep reads the right-hand sides of the
     // columns around its patch, so none is overwritten while the bin runs)
     float* fcb[2] = {S.fc, S.fc2};
+    const bool ho_ok = handoff_ok(S.B, SP_F32, S.chk, S.chk_cnt);
+    Handoff ho;
     for (int k = 0; k < nbn; ++k) {
         const int t = fwd ? k : nbn - 1 - k;
         const bool last = k == nbn - 1;
@@ -2878,7 +3071,7 @@ int z_sweep(Solver& S) {
             epi.as = es;
         }
         int rc = inner_solve(S.B, t, S.dir, SP_F32, S.chk, rhs, rs, out, last ? vs : zs, mask, false, S.chk_cnt,
-                             last ? nullptr : &epi);
+                             last ? nullptr : &epi, k > 0 && ho_ok ? &ho : nullptr, !last && ho_ok ? &ho : nullptr);
         if (rc) return rc;
     }
     hipLaunchKernelGGL((k_convert_slots<float, double>), g, dim3(256), 0, st, S.fb, vs, S.r, vs, (int)n16, mask, S.sl);
@@ -2897,6 +3090,10 @@ int mono_sweeps_f32(Solver& S, float** result, int j) {
     float* in = S.fa;
     long long ins = vs;
     float* pp[2] = {S.fa, S.fb};
+    // bins hand their last black half-sweep to the next one's first-sweep
+    // launch when each bin's states have buffers of their own
+    const bool ho_ok = S.states && handoff_ok(S.B, SP_F32, S.inner, S.inner_cnt);
+    Handoff ho;
     for (int k = 0; k < nbn; ++k) {
         const int t = fwd ? k : nbn - 1 - k;
         const bool last = k == nbn - 1;
@@ -2909,7 +3106,8 @@ int mono_sweeps_f32(Solver& S, float** result, int j) {
         } else {
             out = pp[(k + 1) & 1];
         }
-        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.inner, in, ins, out, outs, S.B.active_dev, false, S.inner_cnt);
+        int rc = inner_solve(S.B, t, S.dir, SP_F32, S.inner, in, ins, out, outs, S.B.active_dev, false, S.inner_cnt,
+                             nullptr, k > 0 && ho_ok ? &ho : nullptr, !last && ho_ok ? &ho : nullptr);
         if (rc) return rc;
         in = out;
         ins = outs;
@@ -4008,7 +4206,7 @@ extern "C" int orbit_timing_stop(orbit_ctx* ctx, orbit_sweep_timing* out, int ma
         const int v = tm->recs[i].variant;
         auto it = agg.find(v);
         if (it == agg.end()) {
-            const int vf = v >= FUSED_FIRST_VARIANT ? v - FUSED_FIRST_VARIANT : v;
+            const int vf = v >= HANDOFF_VARIANT ? v - HANDOFF_VARIANT : v >= FUSED_FIRST_VARIANT ? v - FUSED_FIRST_VARIANT : v;
             const int p = vf / 16;  // internal sweep precision -> ABI constant
             orbit_sweep_timing t{};
             t.precision = p == SP_F64 ? ORBIT_PREC_FP64 : p == SP_ACC_MIXED ? ORBIT_SWEEP_ACC_MIXED : ORBIT_PREC_MIXED;
@@ -4016,7 +4214,7 @@ extern "C" int orbit_timing_stop(orbit_ctx* ctx, orbit_sweep_timing* out, int ma
             t.slots_per_wave = (vf & 4) ? 4 : 1;
             t.own_zero = (vf & 2) ? 1 : 0;
             t.nbr_zero = (vf & 1) ? 1 : 0;
-            t.first_sweep = v >= FUSED_FIRST_VARIANT ? 1 : 0;
+            t.first_sweep = v >= HANDOFF_VARIANT ? 2 : v >= FUSED_FIRST_VARIANT ? 1 : 0;
             it = agg.emplace(v, t).first;
         }
         it->second.launches += 1;
@@ -4058,6 +4256,13 @@ extern "C" int orbit_opset_set_patch(orbit_opset* op, int shape) {
     if (shape < 0 || shape >= N_PATCH_SHAPES) return fail(ORBIT_ERR_INVALID, "set_patch: unknown patch shape");
     op->patch = shape;
     drop_graphs(op);  // captured sweeps carry the old kernels
+    return ORBIT_OK;
+}
+
+extern "C" int orbit_opset_set_handoff(orbit_opset* op, int on) {
+    if (!op) return fail(ORBIT_ERR_INVALID, "set_handoff: NULL operator set");
+    op->handoff = on != 0;
+    drop_graphs(op);  // captured sweeps carry the old launch sequence
     return ORBIT_OK;
 }
 

@@ -57,9 +57,11 @@ struct DevBuf {
 // variant and algorithmic bytes; solves do not replay captured graphs then.
 // Timing variant offset of the first sweep from x = 0 with both colours in one launch.
 constexpr int FUSED_FIRST_VARIANT = 64;
+// ... and of that launch carrying the previous bin's last black half-sweep too.
+constexpr int HANDOFF_VARIANT = 128;
 struct SweepTiming {
     struct Rec {
-        int variant;   // prec * 16 + mfma * 8 + cs * 4 + own_zero * 2 + nbr_zero (+ FUSED_FIRST_VARIANT)
+        int variant;   // prec * 16 + mfma * 8 + cs * 4 + own_zero * 2 + nbr_zero (+ FUSED_FIRST_VARIANT or HANDOFF_VARIANT)
         double bytes;  // algorithmic bytes of the launch
     };
     std::vector<hipEvent_t> ev;  // 2 per launch
@@ -252,6 +254,7 @@ struct Opset {
     double omega = 1.0;  // relaxation of the sweeps' correction (chosen by the calibration)
     bool mfma = false;   // block product of the BF16-inverse sweeps on the matrix cores (opt-in)
     int patch = 2;       // first FP32 sweep of an inner solve in one launch over column patches of this shape (0: two half-sweeps)
+    bool handoff = true; // a bin's last black FP32 half-sweep runs in the next bin's first-sweep launch (needs patch > 0)
     std::vector<BinOps> bins;
     // Column colours of the sweeps: red / black ((i + j) even / odd) over the
     // first nxc columns of every row; with a periodic x and an odd nx the wrap
