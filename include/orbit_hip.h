@@ -172,8 +172,11 @@ int orbit_opset_set_patch(orbit_opset* op, int shape);
  * the previous bin's sweep at their own column before starting their own
  * (every red of the previous bin is final by then): one launch fewer per bin
  * boundary, the same bits.  Applies where orbit_opset_set_patch's one-launch
- * first sweep does, with two or more sweeps per inner solve.  No counterpart
- * in the reference.  Not to be called while a solve on the opset runs. */
+ * first sweep does (group solves included, when every set has it on), with
+ * two or more sweeps per inner solve and at most 48 slots a launch (larger
+ * launches are bound by bandwidth and measured slower with it).  No
+ * counterpart in the reference.  Not to be called while a solve on the opset
+ * runs. */
 int orbit_opset_set_handoff(orbit_opset* op, int on);
 /* Replace the per-tracer diagonals (D laid out [bin][t][N]) and refactor the
  * column blocks in place: the outer chemistry loop changes only loss rates. */

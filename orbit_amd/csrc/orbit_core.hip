@@ -599,6 +599,7 @@ struct FirstSweepArgs {
     // and that new value (plus the previous bin's epilogue term, when it has
     // one) is this bin's right-hand side there, which the unfused sequence
     // would read back from memory.
+    const GroupOp* hgops; // group launches: the slot groups' operators of the previous bin
     const float* hVt;     // previous bin: transposed column blocks (shared)
     const float* hH;      // stencil (shared)
     const float* hd;      // diagonals, per tracer (stride hdstride)
@@ -618,7 +619,6 @@ struct FirstSweepArgs {
 
 template <int PA, int PB, int W, bool GRP = false, bool HO = false>
 __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
-    static_assert(!(GRP && HO), "the hand-off form runs on one operator set");
     constexpr int NRO = PA * PB / 2, NR = NRO + PA + PB, NB = PA * PB / 2, NPOS = (PA + 2) * (PB + 2);
     constexpr int RW = (NR + W - 1) / W, BW = (NB + W - 1) / W, IW = RW > BW ? RW : BW;
     __shared__ float sgrid[NPOS][64];  // new red values by ext position
@@ -633,10 +633,20 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
     const float* Hg;
     long long pstride;
     float omega;
+    // the previous bin's operator (hand-off): the launch's, or the slot group's
+    const float *hVtb = a.hVt, *hHb = a.hH, *hdb = a.hd;
+    const uint16_t* hPb = a.hP;
+    long long hdstride = a.hdstride, hpstride = a.pstride;
+    float homega = a.hepi_omega;
     if constexpr (GRP) {
         const int e = a.btab[blockIdx.x];
         if (e < 0) return;  // uniform over the block
         const GroupOp& go = a.gops[e >> 16];
+        if constexpr (HO) {
+            const GroupOp& hg = a.hgops[e >> 16];
+            hVtb = (const float*)hg.Vt; hHb = (const float*)hg.H; hdb = (const float*)hg.d;
+            hPb = (const uint16_t*)hg.P; hdstride = hg.dstride; hpstride = hg.pstride; homega = hg.omega;
+        }
         patch = e & 0xffff;
         nx = go.nx; ny = go.ny; px = go.px;
         npx = (nx + PA - 1) / PA;
@@ -716,8 +726,8 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
     if constexpr (HO) {
         const float* __restrict__ hb = a.hb + (size_t)ls * a.hbstride;
         const float* __restrict__ hx = a.hx + (size_t)ls * a.hxstride;
-        const uint16_t* __restrict__ hP = a.hP + (size_t)tr * a.pstride;
-        const float* __restrict__ hd = a.hd + (size_t)tr * a.hdstride;
+        const uint16_t* __restrict__ hP = hPb + (size_t)tr * hpstride;
+        const float* __restrict__ hd = hdb + (size_t)tr * hdstride;
 #pragma unroll
         for (int u = 0; u < BW; ++u) {
             const int col = bcol[u];
@@ -733,15 +743,19 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
             const uint16_t* Pr = hP + (size_t)col * 256 + k * 16;
 #pragma unroll
             for (int m = 0; m < 16; ++m) hpr[u][m] = Pr[m];
-            const float* Vt = a.hVt + (size_t)col * 256 + l * 4;
+            const float* Vt = hVtb + (size_t)col * 256 + l * 4;
 #pragma unroll
             for (int qq = 0; qq < 4; ++qq) hvr[u][qq] = Vt[qq];
             hdv[u] = hd[base];
-            const float* Hc = a.hH + (size_t)col * 64 + k;
+            const float* Hc = hHb + (size_t)col * 64 + k;
 #pragma unroll
             for (int e = 0; e < 4; ++e) hhv[u][e] = Hc[16 * e];
         }
     }
+    // a slot whose sweep count (group solves) ended the previous bin's solve
+    // before its last sweep has no black half-sweep left there: its right-hand
+    // side is the one in memory
+    const bool hdo = !a.counts || a.nsweeps - 1 < cnt;
     // omega P v with v the wave's item u in sr: the half-sweep's product from x = 0
     auto apply_P = [&](const uint16_t* row, int u) {
         float dx = 0;
@@ -791,7 +805,7 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
             float dx = 0;
 #pragma unroll
             for (int m = 0; m < 16; ++m) dx += (float)ldv(hpr[u][m]) * sr[wv][u][g16 + m];
-            const float xn = (float)(xo + a.hepi_omega * dx);
+            const float xn = (float)(xo + homega * dx);
             float bn = xn;
             const size_t base = (size_t)col * 16 + k;
             if (a.hepi_out) {
@@ -800,11 +814,11 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
                                                  : ((const double*)a.hepi_add)[ia];
                 bn = xn + (float)ad;
             }
-            if (bst[u] && on) {
+            if (bst[u] && on && hdo) {
                 a.hx[(size_t)ls * a.hxstride + base] = xn;
                 if (a.hepi_out) a.hepi_out[(size_t)ls * a.hepi_os + base] = bn;
             }
-            bb[u] = bn;
+            if (hdo) bb[u] = bn;
         }
         wave_sync();
     }
@@ -2166,6 +2180,7 @@ struct GroupSpec {
     const int* btab[2] = {nullptr, nullptr};
     int nblocks[2] = {0, 0};
     const int* ptab = nullptr;          // the first sweep's (group, 4 x 2 patch) per block; null: half-sweeps
+    bool handoff = false;               // every set has the bin hand-off on
     int npblocks = 0;
     double first_bytes = 0.0;           // algorithmic bytes of a first-sweep launch
     double bytes[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // algorithmic bytes of a full launch, per precision and colour
@@ -2453,7 +2468,24 @@ struct Handoff {
 int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long long bstride, void* x,
                        long long xstride, const int* mask, const Epi* epi, const int* counts, int nsweeps,
                        const Handoff* ho = nullptr) {
-    if (ho && (B.grp || !ho->pending)) return fail(ORBIT_ERR_INVALID, "first sweep: no hand-off to take");
+    if (ho && !ho->pending) return fail(ORBIT_ERR_INVALID, "first sweep: no hand-off to take");
+    // the previous bin's last black half-sweep: its value at a black column is
+    // this bin's right-hand side there only if this bin reads it from where
+    // that half-sweep (or its epilogue) would have stored it
+    auto set_handoff = [&](FirstSweepArgs& a) {
+        a.hb = (const float*)ho->rhs;
+        a.hbstride = ho->bstride;
+        a.hx = (float*)ho->x;
+        a.hxstride = ho->xstride;
+        a.hepi_out = ho->has_epi ? (float*)ho->epi.out : nullptr;
+        a.hepi_os = ho->has_epi ? ho->epi.os : 0;
+        a.hepi_add = ho->has_epi ? ho->epi.add : nullptr;
+        a.hepi_add_f32 = ho->has_epi && ho->epi.add_f32 ? 1 : 0;
+        a.hepi_as = ho->has_epi ? ho->epi.as : 0;
+        const void* feeds = ho->has_epi ? ho->epi.out : ho->x;
+        const long long fs = ho->has_epi ? ho->epi.os : ho->xstride;
+        return feeds == rhs && fs == bstride;
+    };
     if (B.grp) {
         const GroupSpec& G = *B.grp;
         FirstSweepArgs a{};
@@ -2474,6 +2506,10 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
         a.epi_as = epi ? epi->as : 0;
         a.counts = counts;
         a.nsweeps = nsweeps;
+        if (ho) {
+            if (!set_handoff(a)) return fail(ORBIT_ERR_INVALID, "first sweep: the hand-off does not feed this right-hand side");
+            a.hgops = G.gops + ((size_t)ho->bin * N_SWEEP_PRECS + SP_F32) * G.ngroups;
+        }
         hipStream_t st = B.ctx->stream;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (SweepTiming* tm = B.ctx->timing) {
@@ -2481,7 +2517,8 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
                 e0 = tm->ev[tm->next];
                 e1 = tm->ev[tm->next + 1];
                 tm->next += 2;
-                tm->recs.push_back({FUSED_FIRST_VARIANT + SP_F32 * 16 + 4 + 3, G.first_bytes});
+                tm->recs.push_back({(ho ? HANDOFF_VARIANT : FUSED_FIRST_VARIANT) + SP_F32 * 16 + 4 + 3,
+                                    G.first_bytes + (ho ? G.bytes[SP_F32][1] : 0.0)});
             } else {
                 ++tm->dropped;
             }
@@ -2489,7 +2526,8 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
         static_assert(PATCH_SHAPES[GROUP_PATCH_SHAPE].pa == 4 && PATCH_SHAPES[GROUP_PATCH_SHAPE].pb == 2 &&
                           PATCH_SHAPES[GROUP_PATCH_SHAPE].waves == 4,
                       "the group patch table is built for 4 x 2 patches");
-        first_sweep_launch<4, 2, 4, true>(st, a, dim3(G.npblocks), e0, e1);
+        if (ho) first_sweep_launch<4, 2, 4, true, true>(st, a, dim3(G.npblocks), e0, e1);
+        else first_sweep_launch<4, 2, 4, true>(st, a, dim3(G.npblocks), e0, e1);
         return ORBIT_OK;
     }
     orbit_opset* op = B.op;
@@ -2523,12 +2561,7 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
     a.counts = counts;
     a.nsweeps = nsweeps;
     if (ho) {
-        // the previous bin's last black half-sweep: its value at a black column
-        // is this bin's right-hand side there only if this bin reads it from
-        // where that half-sweep (or its epilogue) would have stored it
-        const void* feeds = ho->has_epi ? ho->epi.out : ho->x;
-        const long long fs = ho->has_epi ? ho->epi.os : ho->xstride;
-        if (counts || feeds != rhs || fs != bstride || op->ncolours != 2 || op->ngroups != 1)
+        if (counts || op->ncolours != 2 || op->ngroups != 1 || !set_handoff(a))
             return fail(ORBIT_ERR_INVALID, "first sweep: the hand-off does not feed this right-hand side");
         const BinOps& Q = op->bins[ho->bin];
         a.hVt = T ? Q.V32 : Q.VT32;
@@ -2536,16 +2569,7 @@ int launch_first_sweep(const Batch& B, int bin, int dir, const void* rhs, long l
         a.hd = Q.d32;
         a.hP = T ? Q.PT16 : Q.P16;
         a.hdstride = (long long)op->n16;
-        a.hb = (const float*)ho->rhs;
-        a.hbstride = ho->bstride;
-        a.hx = (float*)ho->x;
-        a.hxstride = ho->xstride;
         a.hepi_omega = (float)op->omega;
-        a.hepi_out = ho->has_epi ? (float*)ho->epi.out : nullptr;
-        a.hepi_os = ho->has_epi ? ho->epi.os : 0;
-        a.hepi_add = ho->has_epi ? ho->epi.add : nullptr;
-        a.hepi_add_f32 = ho->has_epi && ho->epi.add_f32 ? 1 : 0;
-        a.hepi_as = ho->has_epi ? ho->epi.as : 0;
     }
     const int nsg = (B.nslots + 3) / 4;
     const dim3 grid((unsigned)(8 * a.tiles_per_xcd * nsg));
@@ -2627,7 +2651,16 @@ int sweep_once(const Batch& B, int bin, int dir, int prec, const void* rhs, long
 // last black half-sweep of one to the first-sweep launch of the next: one
 // shared transport, two colours, the patch first sweep on, no per-slot counts,
 // a last sweep that is not itself a first sweep.
+// Up to 48 slots a launch: measured faster with the hand-off (per headline-
+// size forward solve at 4 / 12 / 24 / 48 slots 2.63 / 4.41 / 7.87 / 13.98 ms
+// against 2.85 / 4.52 / 8.17 / 14.32; group solves over three sub-domain
+// sizes with 6 / 12 / 24 slots 2.76 / 3.49 / 5.17 against 3.00 / 3.71 /
+// 5.34 ms), but slower for the 144 uneven orbits of one group solve (23.8
+// against 22.9 ms), whose launches are bound by bandwidth, not by their count.
+constexpr int HANDOFF_MAX_SLOTS = 48;
 bool handoff_ok(const Batch& B, int prec, int sweeps, const int* counts) {
+    if (B.nslots > HANDOFF_MAX_SLOTS) return false;
+    if (B.grp) return prec == SP_F32 && sweeps >= 2 && B.grp->ptab && B.grp->handoff;
     const orbit_opset* op = B.op;
     return prec == SP_F32 && !B.grp && !counts && sweeps >= 2 && op->handoff && op->ncolours == 2 &&
            op->ngroups == 1 && op->patch > 0;
@@ -3901,6 +3934,7 @@ extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* con
     std::vector<Grp> grps;
     std::vector<int> tmap(nt);
     G.mf = true;
+    G.handoff = true;
     for (int k = 0; k < nt; ++k) {
         const int s = perm[k], q = slot_set[s];
         orbit_opset* so = sets[q];
@@ -3909,6 +3943,7 @@ extern "C" int orbit_solve_group_dev(orbit_ctx* ctx, int nsets, orbit_opset* con
         G.n16max = std::max(G.n16max, (size_t)so->n16);
         G.ncolmax = std::max(G.ncolmax, so->ncol);
         G.mf = G.mf && so->mfma;
+        G.handoff = G.handoff && so->handoff;
         if (grps.empty() || grps.back().set != q || grps.back().count == 4) grps.push_back({q, k, 1});
         else grps.back().count += 1;
     }

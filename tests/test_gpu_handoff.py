@@ -69,3 +69,46 @@ def test_handoff_at_the_benchmark_shape_and_fct_passes():
     assert np.array_equal(res[True][2], res[False][2])
     assert res[True][3] == res[False][3]
     op.close()
+
+
+def test_group_handoff_gives_the_half_sweeps_bits():
+    """Group solves (slots of several operator sets in one launch) hand off
+    too, with each set's own sweep counts: a slot whose count ended its bin's
+    solve before the last sweep of the launch sequence keeps the right-hand
+    side in memory.  Sets of four sizes (one periodic, one with strong winds
+    and so more sweeps), forward and adjoint, the sets' own sweep counts and
+    two fixed pairs: bit-identical with the hand-off off, and to each set
+    solved alone."""
+    from orbit_amd import serve
+    jobs = []
+    for k, (nx, ny, nt, per, wind) in enumerate(((10, 7, 3, False, 5.0), (12, 9, 2, True, 5.0),
+                                                 (6, 5, 4, False, 5.0), (12, 10, 3, False, 8.0))):
+        wl = synth.make_workload(nx, ny, 6, nt, seed=[41, k], convection=True, periodic=per, wind_std=wind)
+        op = OperatorSet.from_workload(wl)
+        st = op.stats()
+        print(f"set {nx}x{ny}: default sweeps inner {st['default_inner_sweeps']} check {st['default_check_sweeps']}",
+              flush=True)
+        jobs.append((wl, op))
+    for adjoint in (False, True):
+        work = []
+        for wl, op in jobs:
+            inp = wl.e if not adjoint else np.random.default_rng(op.nx).random(wl.e.shape)
+            work.append((op, torch.from_numpy(np.ascontiguousarray(inp)).cuda(), list(range(wl.n_tracers))))
+        for sweeps in ((0, 0), (2, 2), (3, 3)):
+            print(f"adjoint {adjoint}, sweeps {sweeps}", flush=True)
+            kw = dict(precision="mixed", tol=1e-8, adjoint=adjoint, inner_sweeps=sweeps[0], check_sweeps=sweeps[1])
+            res = {}
+            for on in (False, True):
+                for _, op in jobs:
+                    op.set_handoff(on)
+                res[on] = serve.solve_group(work, **kw)
+            for (op, inp, tr), (a, ia), (b, ib) in zip(work, res[True], res[False]):
+                assert torch.equal(a, b), (adjoint, sweeps, op.nx)
+                assert ia == ib
+                fn = op.solve_adjoint if adjoint else op.solve_forward
+                alone, ainfo = fn(inp, tr, tol=1e-8, precision="mixed", inner_sweeps=sweeps[0],
+                                  check_sweeps=sweeps[1])
+                assert torch.equal(a, alone), (adjoint, sweeps, op.nx)
+                assert ia == ainfo
+    for _, op in jobs:
+        op.close()
