@@ -1978,7 +1978,7 @@ __global__ void k_zero_last_col(T* __restrict__ x, long long xslot, int nx, int 
 // residual s - (I - M) c exactly; no accurate inner solve is needed.
 constexpr int CYC_MAXB = 64;
 struct CycArgs {
-    const double* V[CYC_MAXB];  // per bin: column blocks (transposed for the adjoint)
+    const double* V[CYC_MAXB];  // per bin: transposes of the column blocks the residual applies
     const double* H[CYC_MAXB];  // per bin: stencil (transposed for the adjoint)
     const double* d[CYC_MAXB];  // per bin: diagonals of every tracer
     long long vstride, hstride, dstride;  // per tracer
@@ -2014,7 +2014,7 @@ __global__ __launch_bounds__(256) void k_cyc_residual(CycArgs a) {
     int nx, ny, px, first, count, tile;
     if (a.gops) {
         const GroupOp& go = a.gops[((size_t)t * N_SWEEP_PRECS + a.gprec) * a.ngroups + blockIdx.y];
-        V = (const double*)go.V; H = (const double*)go.H; d = (const double*)go.d;
+        V = (const double*)go.Vt; H = (const double*)go.H; d = (const double*)go.d;
         vst = 0; hst = 0; dst = go.dstride;
         nx = go.nx; ny = go.ny; px = go.px;
         first = go.first; count = go.count;
@@ -2039,12 +2039,15 @@ __global__ __launch_bounds__(256) void k_cyc_residual(CycArgs a) {
     const size_t base = (size_t)col * 16 + lane;
     const size_t n16 = (size_t)a.n16;
     const bool shared_op = vst == 0 && hst == 0;
-    // the shared operator row of this lane (reloaded per slot for per-tracer transports)
+    // the shared operator row of this lane (reloaded per slot for per-tracer
+    // transports), read from the block's transpose: element m of row `lane`
+    // is at m * 16 + lane there, so the 16 lanes of a column read 128
+    // contiguous bytes per element
     double vrow[16], hw = 0.0, he = 0.0, hs = 0.0, hn = 0.0;
     auto load_op = [&](int tr) {
-        const double* Vr = V + (size_t)tr * vst + (size_t)col * 256 + lane * 16;
+        const double* Vr = V + (size_t)tr * vst + (size_t)col * 256 + lane;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) vrow[m] = Vr[m];
+        for (int m = 0; m < 16; ++m) vrow[m] = Vr[m * 16];
         const double* Hc = H + (size_t)tr * hst + (size_t)col * 64 + lane;
         hw = Hc[0]; he = Hc[16]; hs = Hc[32]; hn = Hc[48];
     };
@@ -3029,7 +3032,7 @@ int cyc_residual(Solver& S, bool xzero) {
         if (nbn > CYC_MAXB) return fail(ORBIT_ERR_INVALID, "solve: too many bins for the residual launch");
         for (int b = 0; b < nbn; ++b) {
             const BinOps& O = op->bins[b];
-            a.V[b] = T ? O.VT64 : O.V64;
+            a.V[b] = T ? O.V64 : O.VT64;  // the transpose of the applied block
             a.H[b] = T ? O.HT64 : O.H64;
             a.d[b] = O.d64;
         }
