@@ -465,9 +465,11 @@ __global__ __launch_bounds__(256) void k_halfsweep(SweepArgs a) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) vrow[q] = Vt[q];
         } else {
-            const AT* Vr = V + (size_t)col * 256 + lane * 16;
+            // row `lane` from the block's transpose (element m at m * 16 + lane):
+            // the 16 lanes of a column read contiguous bytes per element
+            const AT* Vr = (const AT*)Vtb + (size_t)tr * vstride + (size_t)col * 256 + lane;
 #pragma unroll
-            for (int m = 0; m < 16; ++m) vrow[m] = Vr[m];
+            for (int m = 0; m < 16; ++m) vrow[m] = Vr[m * 16];
         }
         dv = (CT)d[base];
     }
