@@ -859,7 +859,9 @@ __global__ __launch_bounds__(W * 64) void k_first_sweep(FirstSweepArgs a) {
 }
 
 // y = A x (one operator apply, FP64), used for residual checks of the inner solves.
-__global__ __launch_bounds__(256) void k_apply_A(const double* __restrict__ V, long long vstride,
+// VT: the transposes of the applied column blocks (row `lane` of a block read
+// as element m at m * 16 + lane: contiguous bytes over a column's 16 lanes).
+__global__ __launch_bounds__(256) void k_apply_A(const double* __restrict__ VT, long long vstride,
                                                  const double* __restrict__ H, long long hstride,
                                                  const double* __restrict__ d, long long dstride,
                                                  const double* __restrict__ x, double* __restrict__ y,
@@ -873,14 +875,14 @@ __global__ __launch_bounds__(256) void k_apply_A(const double* __restrict__ V, l
     const int col = valid ? col0 : 0;
     const int tr = tmap[slot];
     const double* xs = x + (size_t)slot * xstride;
-    const double* Vt = V + (size_t)tr * vstride;
+    const double* Vt = VT + (size_t)tr * vstride;
     const double* Ht = H + (size_t)tr * hstride;
     const double* dt = d + (size_t)tr * dstride;
     const size_t base = (size_t)col * 16 + lane;
     sx[threadIdx.x] = xs[base];
     __syncthreads();
     double acc = dt[base] * sx[threadIdx.x];
-    for (int m = 0; m < 16; ++m) acc += Vt[(size_t)col * 256 + lane * 16 + m] * sx[(grp << 4) + m];
+    for (int m = 0; m < 16; ++m) acc += Vt[(size_t)col * 256 + m * 16 + lane] * sx[(grp << 4) + m];
     const int j = col / nx, i = col - j * nx;
     const double* Hc = Ht + (size_t)col * 64 + lane;
     const int cw = west_col(col, i, nx, px), ce = east_col(col, i, nx, px);
@@ -2726,7 +2728,7 @@ int measure_contraction(orbit_ctx* ctx, orbit_opset* op, int ns, int* tr, const 
             const int rc = sweep_once(B, bin, FWD, SP_F64, b, 0, x, xs, k == 1, nullptr, nullptr, 0);
             if (rc) return rc;
             if (k == 3 || k == 8) {
-                hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, ns), dim3(256), 0, st, O.V64, vstride, O.H64,
+                hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, ns), dim3(256), 0, st, O.VT64, vstride, O.H64,
                                    hstride, O.d64, (long long)n16, x, y, xs, tr, op->ncol, op->nx, op->ny, op->px);
                 hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, ns), dim3(256), 0, st, b, 0LL, y, xs, -1.0, y, xs,
                                    (int)n16, nullptr, SlotLen{});
@@ -4134,7 +4136,7 @@ extern "C" int orbit_apply_P(orbit_ctx* ctx, orbit_opset* op, int bin, int trace
             first = false;
         }
         total += chunk;
-        hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, 1), dim3(256), 0, st, trans ? O.VT64 : O.V64, vstride,
+        hipLaunchKernelGGL(k_apply_A, dim3((op->ncol + 15) / 16, 1), dim3(256), 0, st, trans ? O.V64 : O.VT64, vstride,
                            trans ? O.HT64 : O.H64, hstride, O.d64, (long long)n16, xs, y, (long long)n16, tmap,
                            op->ncol, op->nx, op->ny, op->px);
         hipLaunchKernelGGL(k_axpby, dim3(grid1(n16).x, 1), dim3(256), 0, st, rhs, (long long)n16, y, (long long)n16, -1.0,
